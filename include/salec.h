@@ -1,0 +1,254 @@
+/*
+ * salec.h -- C ABI of the MI355X-native SALEc-2D per-timestep solver.
+ *
+ * The library owns device memory; host pointers are borrowed for the duration
+ * of a call.  Every entry point returns 0 on success or a negative SALEC_E*
+ * code; nothing calls exit().  One context per host thread; all calls on a
+ * context are serialised on the context's own HIP stream.
+ *
+ * The interface replaces the reference's step loop body and the state object
+ * it operates on:
+ *   - sale2d_var (src/sale2d.h:34-174) and its field allocation in
+ *     init_sale2d_var (src/sale2d.c:171-415)        -> salec_create + fields
+ *   - the loop body src/salec2d.c:115-272           -> salec_step
+ *   - state_reference / eos_table (src/eos/eos_state.h:43-151,225-238) and the
+ *     strength registry str_func_list (src/strength.c:860-875)
+ *                                                    -> salec_material
+ *   - boundary options mesh2d_info.go_list (src/sale2d_back.c:219-238)
+ *                                                    -> salec_config.go_list
+ *
+ * Data layout (host and device): every field is structure-of-arrays FP64,
+ * component-major: value(c, x, y) = buf[c*W*H + y*W + x], x fastest, on the
+ * reference's padded index space (2 ghost layers per side):
+ *   cells     W = nx+4, H = ny+4      (src/mpi_vec/mpi_vec.c:612-627)
+ *   vertices  W = nx+5, H = ny+5      (src/mpi_vec/mpi_vec.c:628-642)
+ *   x-faces   W = nx+5, H = ny+4      (bx_phi, src/sale2d.c:305-308)
+ *   y-faces   W = nx+4, H = ny+5      (by_phi, src/sale2d.c:309-312)
+ *   columns   W = nx+4, H = 1         (y_profile, src/sale2d.c:386-389)
+ * Vertex (k,j) is the lower-left corner of cell (k,j), as in the reference.
+ */
+#ifndef SALEC_H
+#define SALEC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SALEC_MAXMAT 10   /* MAXMAT, src/sale2d_constant.h:305 */
+#define SALEC_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define SALEC_OK            0
+#define SALEC_EINVAL       -1   /* bad argument / unsupported option          */
+#define SALEC_ENOMEM       -2   /* device or host allocation failed          */
+#define SALEC_EDEVICE      -3   /* HIP runtime error / no device             */
+#define SALEC_ENONFINITE   -4   /* a non-finite value found by salec_check    */
+#define SALEC_EDTCOLLAPSE  -5   /* dt fell below the configured floor         */
+#define SALEC_ESTATE       -6   /* call out of order (e.g. step before init)  */
+#define SALEC_ENOSYM       -7   /* unknown model name                          */
+
+/* ---- enumerations mirroring the reference ------------------------------ */
+/* eos_type, src/eos/eos_state.h:225-231 */
+enum { SALEC_EOS_ANEOS = 0, SALEC_EOS_TILLOTSON = 1, SALEC_EOS_ANEOS_SD = 2, SALEC_EOS_NONE = 3 };
+
+/* str_func_list registry order, src/strength.c:860-874 */
+enum {
+    SALEC_FN_SIMPLEROCK = 0, SALEC_FN_SIMPLESHEAR = 1, SALEC_FN_JOHNSONCOOK1 = 2,
+    SALEC_FN_JOHNSONCOOK2 = 3, SALEC_FN_ROCKSTRENGTHT = 4, SALEC_FN_ICESTRENGTH = 5,
+    SALEC_FN_COLLINS = 6, SALEC_FN_IVANOV = 7, SALEC_FN_SIMPLETENSILE = 8,
+    SALEC_FN_NONE = 9, SALEC_FN_WUNNEMANN = 10, SALEC_FN_VONMISES1 = 11,
+    SALEC_FN_VONMISES2 = 12, SALEC_FN_COUNT = 13
+};
+
+/* damp_type_t, src/sale2d.h:252-260 */
+enum { SALEC_DAMP_PML = 0, SALEC_DAMP_PML2 = 1, SALEC_DAMP_PML3 = 2,
+       SALEC_DAMP_SBC = 3, SALEC_DAMP_CTLG = 4, SALEC_DAMP_NONE = 5 };
+
+/* ght_opt, src/mpi_vec/mpi_vec_types.h (ghost_table options) */
+enum { SALEC_GHT_OUTFLOW = 0, SALEC_GHT_OUTFLOW_VERTEX, SALEC_GHT_FIXED, SALEC_GHT_ZERO,
+       SALEC_GHT_CLEAN, SALEC_GHT_MIRROR, SALEC_GHT_FREESLIP, SALEC_GHT_REVERSE_VEL_Y,
+       SALEC_GHT_REVERSE_VEL_ABSY, SALEC_GHT_REVERSE_VEL_X, SALEC_GHT_REVERSE_STE_1,
+       SALEC_GHT_COPY_STE, SALEC_RST_X_ZERO, SALEC_RST_Y_ZERO, SALEC_RST_Y_POSITIVE,
+       SALEC_RST_X_POSITIVE, SALEC_RST_ZERO, SALEC_RST_NONE, SALEC_GHT_UNDEFINED };
+
+/* grid kinds */
+enum { SALEC_GRID_CELL = 0, SALEC_GRID_VERTEX = 1, SALEC_GRID_XFACE = 2,
+       SALEC_GRID_YFACE = 3, SALEC_GRID_COLUMN = 4 };
+
+/* ---- fields (sale2d_var members, src/sale2d.h:37-130) -------------------
+ * ncomp < 0 means "nmat components". */
+#define SALEC_FIELD_LIST(X) \
+    X(M_VOF,      SALEC_GRID_CELL,   -1)  /* m_vof = e_vof                  */ \
+    X(M_DEN,      SALEC_GRID_CELL,   -1)                                        \
+    X(M_ENG,      SALEC_GRID_CELL,   -1)  /* energy density rho*e            */ \
+    X(M_PTY,      SALEC_GRID_CELL,   -1)  /* distension (porosity)           */ \
+    X(E_STE,      SALEC_GRID_CELL,    4)  /* rr, rz, zz, theta-theta         */ \
+    X(E_DAM,      SALEC_GRID_CELL,    1)                                        \
+    X(E_TPS,      SALEC_GRID_CELL,    1)                                        \
+    X(E_VIB,      SALEC_GRID_CELL,    1)                                        \
+    X(E_EJT,      SALEC_GRID_CELL,    1)                                        \
+    X(E_STE2S,    SALEC_GRID_CELL,    1)                                        \
+    X(E_STA2S,    SALEC_GRID_CELL,    1)                                        \
+    X(E_STRENGTH, SALEC_GRID_CELL,    1)                                        \
+    X(E_DVEL,     SALEC_GRID_CELL,    2)                                        \
+    X(E_ALPHA,    SALEC_GRID_CELL,    2)                                        \
+    X(E_WPT,      SALEC_GRID_CELL,    4)                                        \
+    X(E_DEN,      SALEC_GRID_CELL,    1)                                        \
+    X(E_PRE,      SALEC_GRID_CELL,    1)                                        \
+    X(E_CSD,      SALEC_GRID_CELL,    1)                                        \
+    X(E_TEM,      SALEC_GRID_CELL,    1)                                        \
+    X(E_MU,       SALEC_GRID_CELL,    1)                                        \
+    X(E_CVS,      SALEC_GRID_CELL,    1)                                        \
+    X(E_VEL,      SALEC_GRID_CELL,    2)                                        \
+    X(E_GRAD_VEL, SALEC_GRID_CELL,    5)                                        \
+    X(E_DIV_VEL,  SALEC_GRID_CELL,    1)                                        \
+    X(E_Q,        SALEC_GRID_CELL,    1)                                        \
+    X(E_POS,      SALEC_GRID_CELL,    4)  /* centre r, z, scale r, scale z   */ \
+    X(E_VOL,      SALEC_GRID_CELL,    1)                                        \
+    X(E_GRAD,     SALEC_GRID_CELL,   12)                                        \
+    X(E_SIGMA,    SALEC_GRID_CELL,    2)                                        \
+    X(E_DAMPREF,  SALEC_GRID_CELL,    3)                                        \
+    X(V_VEL,      SALEC_GRID_VERTEX,  2)                                        \
+    X(V_MASS,     SALEC_GRID_VERTEX,  2)  /* mass, void-corner count         */ \
+    X(V_ALPHA,    SALEC_GRID_VERTEX,  2)                                        \
+    X(V_VOF,      SALEC_GRID_VERTEX, -1)                                        \
+    X(V_ACC,      SALEC_GRID_VERTEX,  2)                                        \
+    X(V_DIS,      SALEC_GRID_VERTEX,  2)                                        \
+    X(V_POS,      SALEC_GRID_VERTEX,  2)                                        \
+    X(V_BF,       SALEC_GRID_VERTEX,  2)  /* body force (gravity)            */ \
+    X(V_VOL,      SALEC_GRID_VERTEX,  5)                                        \
+    X(V_SUBVOL,   SALEC_GRID_VERTEX,  4)                                        \
+    X(V_SUF,      SALEC_GRID_VERTEX,  8)                                        \
+    X(V_LAPLACE,  SALEC_GRID_VERTEX,  9)                                        \
+    X(BX_NORMAL,  SALEC_GRID_XFACE,   2)                                        \
+    X(BY_NORMAL,  SALEC_GRID_YFACE,   2)                                        \
+    X(Y_PROFILE,  SALEC_GRID_COLUMN,  1)
+
+#define SALEC_FIELD_ENUM_(name, grid, nc) SALEC_##name,
+enum { SALEC_FIELD_LIST(SALEC_FIELD_ENUM_) SALEC_NFIELDS };
+#undef SALEC_FIELD_ENUM_
+
+/* ---- material: state_reference + eos_table, src/eos/eos_state.h:43-151 -- */
+typedef struct salec_material {
+    int    eos_type;               /* SALEC_EOS_*                            */
+    /* Tillotson constants, struct TillotsonTable (src/eos/eos_state.h:178-197) */
+    double TLRho0, TLCv, TLA, TLB, TLE0, TLa, TLb, TLAlpha, TLBeta, TLEiv, TLEcv, TLTref;
+    int    till_nden;              /* cold-energy table length (RhoStep)     */
+    double till_stepden;
+    const double *till_yeng;       /* [till_nden] cold energy                */
+    /* tabulated EOS, struct ANEOSTable: data[(ie*nden + jd)*3 + k]           */
+    int    tab_nden, tab_ntem;
+    const double *tab_yden;        /* density axis [nden]                    */
+    const double *tab_xtem;        /* energy (SieDen) / temperature axis     */
+    const double *tab_data;        /* (T,P,c) for SieDen; (rho*e,P,c) ANEOS  */
+    /* strength / damage / AF parameters (state_reference members)           */
+    double VaporDen, MeltDen, GaCoef;
+    double Yint0, Yfricint, Ylimint, Ydam0, Yfricdam, Ylimdam, Yten0;
+    double IvanA, IvanB, IvanC, Pbd, Pbp, minPint, minPdam;
+    double JCA, JCB, JCN, JCC, JCM, JCTREF, JCMaxPlastic, JCMinP;
+    double Asimon, Csimon, Tmelt0, Tfrac, Tdelta;
+    int    PolyLiq, PolySol;
+    double Tlidc1, Tlidc2, Tlidc3, Tsolc1, Tsolc2, Tsolc3;
+    double Toff, Cvib, VibMax, Tdamp, Pvlim, Acvis;
+    double alpha0, chi, gamfrac, den0, melt_viscosity;
+    int    yfunc, dfunc, tfunc, pfunc; /* SALEC_FN_* (shear, damage, tensile, porosity) */
+} salec_material;
+
+/* ---- run configuration (sale2d_var scalars + mesh2d_info options) ------ */
+typedef struct salec_config {
+    int    nx, ny;                 /* physical cells (mesh.npx, mesh.npy)     */
+    int    nmat;                   /* materials including vacuum (index 0)   */
+    const salec_material *mat;     /* [nmat]; mat[0] (vacuum) is ignored     */
+    double avis1, avis2, anc;
+    double density_cutoff, velocity_max, velocity_min, acceleration_min;
+    int    partpres, tensile_failure;
+    int    damp_type;              /* SALEC_DAMP_*                           */
+    double alpha_damp_x, alpha_damp_y, damp_time;
+    double threshold_z, flow_resistance, flow_friction;
+    double max_dt;                 /* cycle.maxdt                            */
+    int    go_list[4];             /* left, right, bottom, top: SALEC_GHT_*  */
+    long   max_tracers;            /* capacity of the tracer arrays          */
+} salec_config;
+
+/* ---- tracers (tracer_t, src/mpi_vec/linked_list.h:18-28) --------------- */
+typedef struct salec_tracers {
+    long    n;
+    double *pos;      /* [2][n]  */
+    double *vel;      /* [2][n]  */
+    float  *mpre;     /* [n]     */
+    float  *mtem;     /* [n]     */
+    int    *gx, *gy, *tag, *matid;  /* [n] */
+    int    *idx, *idy;              /* [n] current cell (padded index)      */
+    int    *alive;                  /* [n] 1 = in list, 0 = deleted        */
+} salec_tracers;
+
+typedef struct salec_stats {
+    double dt;        /* dt used by this step                             */
+    double t;         /* time after the step                              */
+    long   step;      /* steps taken by this context                      */
+} salec_stats;
+
+typedef struct salec_ctx salec_ctx;
+
+/* step flags */
+#define SALEC_STEP_NOSYNC   0x1u  /* do not wait for / return dt and t      */
+
+/* Field metadata: grid kind and component count (nmat expanded). */
+int  salec_field_info(int field, int nmat, int *grid, int *ncomp);
+const char *salec_field_name(int field);
+/* Number of doubles of one component plane of a grid kind. */
+long salec_grid_points(int grid, int nx, int ny);
+
+/* Create a context on the current HIP device.  Replaces init_sale2d_var
+ * (src/sale2d.c:171-415), init_sale_mat (src/sale2d.c:429-514),
+ * select_strength_model (src/strength.c:602-623) and init_sale2d_ghost
+ * (src/sale2d.c:2018-2079).  Tables in cfg->mat are copied to the device. */
+int  salec_create(const salec_config *cfg, salec_ctx **out);
+void salec_destroy(salec_ctx *ctx);
+
+/* Copy one whole field (all components, padded grid) host <-> device.
+ * Replaces direct access to field_var data (src/mpi_vec/mpi_vec.c:861-880). */
+int  salec_upload(salec_ctx *ctx, int field, const double *host);
+int  salec_download(salec_ctx *ctx, int field, double *host);
+
+/* Tracer upload/download (fl_tracer_init output, src/tracer.c:380-450). */
+int  salec_upload_tracers(salec_ctx *ctx, const salec_tracers *tr);
+int  salec_download_tracers(salec_ctx *ctx, salec_tracers *tr);
+
+/* Setup tail of main(), src/salec2d.c:76-94: update_state, ghost fills of
+ * e_den/e_pre/e_ste/m_eng/m_pty/e_cvs, record_avg_value (e_dampref and the
+ * damping time), init_v_field, and zeroing of e_vel/e_ste/e_dvel.
+ * damp_time_io: in = numerical.damp_time (or cycle.time1), out = the damping
+ * time used by the step, max(2 * latest wave arrival, in). */
+int  salec_init(salec_ctx *ctx, double *damp_time_io);
+/* Use the uploaded state as is (already initialised) with this damp time. */
+int  salec_set_damp_time(salec_ctx *ctx, double damp_time);
+
+/* Time state: step_time and dt of cycle_control (src/sale2d.h:176-208). */
+int  salec_set_time(salec_ctx *ctx, double t, double dt);
+int  salec_get_time(salec_ctx *ctx, double *t, double *dt, long *step);
+
+/* One iteration of the loop body src/salec2d.c:115-272 (output excluded). */
+int  salec_step(salec_ctx *ctx, unsigned flags, salec_stats *stats);
+
+/* Block until all work queued on the context has finished. */
+int  salec_sync(salec_ctx *ctx);
+
+/* Scan the persistent cell/vertex state for NaN/Inf.  Returns SALEC_OK or
+ * SALEC_ENONFINITE with the first offending (field, flat index). */
+int  salec_check(salec_ctx *ctx, int *field, long *index);
+
+/* Per-phase timing of the last salec_step when profiling is enabled. */
+int  salec_set_profiling(salec_ctx *ctx, int on);
+int  salec_phase_times(salec_ctx *ctx, int max_phases, const char **names, float *ms);
+
+const char *salec_strerror(int code);
+int  salec_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SALEC_H */

@@ -1,0 +1,67 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, single-threaded restatement of the reference solver's per-step
+ * path (huachengli/SALEc-2D-public, src/salec2d.c:115-272).  It exists to
+ * check the HIP product path: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  It is never linked into the
+ * product library and the product never falls back to it.
+ *
+ * It operates on the same host field layout as include/salec.h (SoA FP64 on
+ * the reference's padded grids) so that states can be compared element by
+ * element.  Loops run in the reference's order and ghost tables are applied
+ * entry by entry in the reference's index order.
+ */
+#ifndef SALEC_ORACLE_H
+#define SALEC_ORACLE_H
+#include "../include/salec.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_state {
+    int nx, ny, nmat;
+    double *f[SALEC_NFIELDS];      /* field buffers (caller-owned)           */
+    double *bx_phi, *by_phi;       /* face packets (oracle-owned scratch)    */
+    double t, dt;                  /* cycle_control.step_time / dt           */
+    long   step;
+    salec_tracers tr;              /* caller-owned tracer arrays             */
+    double *tr_vx, *tr_vy;         /* vertex coordinate lines (oracle-owned) */
+} oracle_state;
+
+/* Allocate oracle scratch for a state whose field buffers are set. */
+int  oracle_attach(oracle_state *s, const salec_config *cfg);
+void oracle_detach(oracle_state *s);
+
+/* Setup tail of main(): src/salec2d.c:76-94 (update_state, ghost fills,
+ * record_avg_value, damp time, init_v_field, cleaning).  Returns damp time. */
+double oracle_init(oracle_state *s, const salec_config *cfg, double *damp_time_out);
+
+/* One loop iteration, src/salec2d.c:115-272. */
+void oracle_step(oracle_state *s, const salec_config *cfg);
+
+/* Individual phases, exported for unit tests. */
+void oracle_update_state(oracle_state *s, const salec_config *cfg);
+double oracle_calculate_state(const salec_material *m, double den, double eng,
+                              double *tem, double *pre, double *csd);
+double oracle_strength_fn(int fn, const double *paras, const salec_material *m);
+double oracle_get_flux_volcyl(const double *a, const double *b, const double *da, const double *db);
+double oracle_cut_length(double vf, double a, double b);
+double oracle_geometric_flux_ratio(double p1, double p2);
+double oracle_vof_mstacs(double nvofd, double gf, double courant);
+double oracle_till_pres(const salec_material *m, double eng, double rho, double *pre, double *cs);
+double oracle_till_temp(const salec_material *m, double eng, double rho);
+void   oracle_till_cold_energy(const salec_material *m, int nden, double *xden, double *yeng);
+int    oracle_bisearch(double x, const double *xl, int n);
+void   oracle_ghost_fill(oracle_state *s, const salec_config *cfg, int field, int var_type);
+void   oracle_restrict(oracle_state *s, const salec_config *cfg, int field);
+
+/* ght_var_type, src/mpi_vec/mpi_vec_types.h */
+enum { O_GHT_ALWAYS_COPY = 0, O_GHT_NORMAL, O_GHT_VELOCITY_INC, O_GHT_VELOCITY,
+       O_GHT_STRESS, O_GHT_DENSITY };
+
+#ifdef __cplusplus
+}
+#endif
+#endif

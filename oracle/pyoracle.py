@@ -1,0 +1,115 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of liboracle.so.
+
+Used by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+to run the plain-C restatement of the reference step as the checker.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "salec-2d-public_amd"))
+import pysalec as ps  # noqa: E402
+
+ORACLE_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OracleState(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nmat", C.c_int),
+                ("f", C.POINTER(C.c_double) * ps.NFIELDS),
+                ("bx_phi", C.POINTER(C.c_double)), ("by_phi", C.POINTER(C.c_double)),
+                ("t", C.c_double), ("dt", C.c_double), ("step", C.c_long),
+                ("tr", ps.Tracers),
+                ("tr_vx", C.POINTER(C.c_double)), ("tr_vy", C.POINTER(C.c_double))]
+
+
+_OL = None
+
+
+def olib():
+    global _OL
+    if _OL is not None:
+        return _OL
+    if not os.path.exists(ORACLE_PATH):
+        raise RuntimeError("liboracle.so is not built: run __graft_entry__.build()")
+    L = C.CDLL(ORACLE_PATH)
+    P = C.POINTER(OracleState)
+    CP = C.POINTER(ps.Config)
+    L.oracle_attach.argtypes = [P, CP]
+    L.oracle_detach.argtypes = [P]
+    L.oracle_init.argtypes = [P, CP, C.POINTER(C.c_double)]
+    L.oracle_init.restype = C.c_double
+    L.oracle_step.argtypes = [P, CP]
+    L.oracle_update_state.argtypes = [P, CP]
+    L.oracle_ghost_fill.argtypes = [P, CP, C.c_int, C.c_int]
+    L.oracle_restrict.argtypes = [P, CP, C.c_int]
+    MP = C.POINTER(ps.Material)
+    d, dp = C.c_double, C.POINTER(C.c_double)
+    L.oracle_calculate_state.argtypes = [MP, d, d, dp, dp, dp]
+    L.oracle_calculate_state.restype = d
+    L.oracle_strength_fn.argtypes = [C.c_int, dp, MP]
+    L.oracle_strength_fn.restype = d
+    L.oracle_get_flux_volcyl.argtypes = [dp, dp, dp, dp]
+    L.oracle_get_flux_volcyl.restype = d
+    L.oracle_cut_length.argtypes = [d, d, d]
+    L.oracle_cut_length.restype = d
+    L.oracle_geometric_flux_ratio.argtypes = [d, d]
+    L.oracle_geometric_flux_ratio.restype = d
+    L.oracle_vof_mstacs.argtypes = [d, d, d]
+    L.oracle_vof_mstacs.restype = d
+    L.oracle_till_pres.argtypes = [MP, d, d, dp, dp]
+    L.oracle_till_pres.restype = d
+    L.oracle_till_temp.argtypes = [MP, d, d]
+    L.oracle_till_temp.restype = d
+    L.oracle_till_cold_energy.argtypes = [MP, C.c_int, dp, dp]
+    L.oracle_bisearch.argtypes = [d, dp, C.c_int]
+    L.oracle_bisearch.restype = C.c_int
+    _OL = L
+    return L
+
+
+class Oracle:
+    """Holds numpy copies of every field and steps them with liboracle."""
+
+    def __init__(self, cfg, state, tracers=None, t=0.0, dt=0.0):
+        self.L = olib()
+        self.cfg = cfg
+        self.nx, self.ny, self.nmat = cfg.nx, cfg.ny, cfg.nmat
+        self.state = {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in state.items()}
+        for name, _, _ in ps.FIELDS:
+            if name not in self.state:
+                self.state[name] = np.zeros(ps.field_shape(name, self.nmat, self.nx, self.ny))
+        self.tr = tracers.copy() if tracers is not None else ps.TracerSet(0)
+        s = OracleState()
+        s.nx, s.ny, s.nmat = self.nx, self.ny, self.nmat
+        for i, (name, _, _) in enumerate(ps.FIELDS):
+            s.f[i] = self.state[name].ctypes.data_as(C.POINTER(C.c_double))
+        s.t, s.dt, s.step = t, dt, 0
+        s.tr = self.tr.struct()
+        self.s = s
+        if self.L.oracle_attach(C.byref(s), C.byref(cfg)) != 0:
+            raise MemoryError("oracle_attach")
+
+    def init(self):
+        d = C.c_double(0.0)
+        self.L.oracle_init(C.byref(self.s), C.byref(self.cfg), C.byref(d))
+        return d.value
+
+    def step(self):
+        self.L.oracle_step(C.byref(self.s), C.byref(self.cfg))
+
+    @property
+    def t(self):
+        return self.s.t
+
+    @property
+    def dt(self):
+        return self.s.dt
+
+    def __del__(self):
+        try:
+            self.L.oracle_detach(C.byref(self.s))
+        except Exception:
+            pass

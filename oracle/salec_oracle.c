@@ -1,0 +1,1757 @@
+/*
+ * salec_oracle.c -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Single-threaded C restatement of the per-timestep path of
+ * huachengli/SALEc-2D-public.  Every function names the reference
+ * file:line it follows.  Arithmetic keeps the reference's operand order and
+ * is compiled without FMA contraction (-ffp-contract=off), matching the
+ * reference's x86-64 SSE2 build.  Loops run in the reference's order.
+ *
+ * Pins (tests/test_oracle_*.py): Tillotson closed form and regime
+ * continuity, cold-energy ODE dE/drho = p/rho^2, table lookup at nodes and
+ * bilinear in between, strength closed forms, MSTACS piecewise form, swept
+ * volume against brute-force quadrature, cut_length as the inverse of the
+ * area fraction, conservation of mass/volume by the remap, rest state under
+ * gravity, ghost-table brute force.  Functions without such a pin are listed
+ * as "parity unpinned" in DESIGN.md.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+/* src/sale2d_constant.h:307-316 */
+#define TOLVOF 1.0e-6
+#define TOLRHO 1.0e-4
+#define TOLQPRE 1e-4
+#define DTF 0.2
+#define CISSAMK1 1.0
+#define VACUUM 0
+/* src/geometry/interpolate_fn.h:19 */
+#define BETA 0.57735026918962576451
+
+/* src/eos/eos_state.c:7-54 ; src/mpi_vec/mpi_vec_types.h:261-262 */
+static double Wind(double x, double l, double r) { if (x < l) return l; else if (x > r) return r; else return x; }
+static double Max(double a, double b) { if (a > b) return a; else return b; }
+static double Min(double a, double b) { if (a < b) return a; else return b; }
+#define MAXM(a, b) ((a) > (b) ? (a) : (b))
+#define MINM(a, b) ((a) < (b) ? (a) : (b))
+static double Sqrt_trunc(double x) { if (x < 1.0e-2) return -sqrt(fabs(x)); else return sqrt(x); }
+
+/* field metadata (layout of include/salec.h) */
+static void field_info(int field, int nmat, int *grid, int *ncomp)
+{
+#define O_META_(name, gr, nc) {gr, nc},
+    static const int meta[SALEC_NFIELDS][2] = { SALEC_FIELD_LIST(O_META_) };
+#undef O_META_
+    *grid = meta[field][0];
+    *ncomp = meta[field][1] < 0 ? nmat : meta[field][1];
+}
+
+/* ---------------------------------------------------------------- grid -- */
+typedef struct {
+    int nx, ny, nmat;
+    int CW, CH, VW, VH, BXW, BXH, BYW, BYH;
+    long NC, NV, NBX, NBY;
+} grid_t;
+
+static grid_t mkgrid(const oracle_state *s)
+{
+    grid_t g;
+    g.nx = s->nx; g.ny = s->ny; g.nmat = s->nmat;
+    g.CW = s->nx + 4; g.CH = s->ny + 4; g.VW = s->nx + 5; g.VH = s->ny + 5;
+    g.BXW = s->nx + 5; g.BXH = s->ny + 4; g.BYW = s->nx + 4; g.BYH = s->ny + 5;
+    g.NC = (long)g.CW * g.CH; g.NV = (long)g.VW * g.VH;
+    g.NBX = (long)g.BXW * g.BXH; g.NBY = (long)g.BYW * g.BYH;
+    return g;
+}
+
+#define CI(x, y) ((long)(x) + (long)g.CW * (y))
+#define VI(x, y) ((long)(x) + (long)g.VW * (y))
+#define BXI(x, y) ((long)(x) + (long)g.BXW * (y))
+#define BYI(x, y) ((long)(x) + (long)g.BYW * (y))
+#define CF(fld, c, i) (s->f[SALEC_##fld][(long)(c) * g.NC + (i)])
+#define VF(fld, c, i) (s->f[SALEC_##fld][(long)(c) * g.NV + (i)])
+
+/* node types, src/mpi_vec/mpi_vec.c:204-230 (cells) and :111-202 (vertices) */
+static int cell_physical(const grid_t *g, int x, int y)
+{
+    int xd = MINM(x, g->CW - 1 - x), yd = MINM(y, g->CH - 1 - y);
+    return !(xd < 2 || yd < 2);
+}
+static int cell_receive(const grid_t *g, int x, int y) { return !cell_physical(g, x, y); }
+static int vert_active(const grid_t *g, int x, int y)   /* update | internal */
+{
+    int xd = MINM(x, g->VW - 1 - x), yd = MINM(y, g->VH - 1 - y);
+    return !(xd <= 0 || yd <= 0);
+}
+/* set: 1 = internal (distance >= 4), 2 = update (distance 1..3), 3 = both */
+static int vert_in(const grid_t *g, int x, int y, int set)
+{
+    int xd = MINM(x, g->VW - 1 - x), yd = MINM(y, g->VH - 1 - y);
+    if (xd <= 0 || yd <= 0) return 0;
+    int internal = !(xd <= 3 || yd <= 3);
+    return internal ? (set & 1) : (set & 2);
+}
+static int vert_pad(const grid_t *g, int x, int y)
+{
+    int xd = MINM(x, g->VW - 1 - x), yd = MINM(y, g->VH - 1 - y);
+    return (xd <= 1 || yd <= 1);
+}
+
+/* ------------------------------------------------------------- vectors -- */
+/* src/geometry/linear.c:96-104 */
+static double vdot(const double *x, const double *y, int n)
+{ double z = 0; for (int k = 0; k < n; ++k) z += x[k] * y[k]; return z; }
+/* src/geometry/linear.c:153-162 */
+static double vlen(const double *x, int n)
+{ double r = 0.; for (int k = 0; k < n; ++k) r += x[k] * x[k]; return sqrt(r); }
+/* src/geometry/linear.c:116-131, including the NaN -> 1.0 quirk */
+static void vnorm(double *x, int n)
+{
+    double l = 0.;
+    for (int k = 0; k < n; ++k) l += x[k] * x[k];
+    l = sqrt(l);
+    for (int k = 0; k < n; ++k) { x[k] /= l; if (isnan(x[k])) x[k] = 1.0; }
+}
+/* src/geometry/linear.c:188-194 */
+static double second_inv(const double *t)
+{
+    double tXX = t[0], tYY = t[2], tXY = t[1], tTH = t[3];
+    return ((tXX - tYY) * (tXX - tYY) + (tYY - tTH) * (tYY - tTH) + (tTH - tXX) * (tTH - tXX)) / 6.0 + tXY * tXY;
+}
+/* src/geometry/linear.c:196-218 */
+int oracle_bisearch(double x, const double *xl, int n)
+{
+    int left = 0, right = n - 1;
+    if (x < xl[left]) return left;
+    if (x >= xl[right]) return right - 1;
+    while (left + 1 < right) {
+        int mid = (left + right) / 2;
+        if (xl[mid] > x) right = mid; else left = mid;
+    }
+    return left;
+}
+/* src/geometry/interpolate.c:36-44 with IpB = {Ip2d2,Ip2d3,Ip2d0,Ip2d1}
+ * (src/geometry/interpolate_fn.c:22-83,249) */
+static double interp2d(const double *Xi, const double *xl)
+{
+    double r = 0;
+    r += 0.25 * (1 - xl[0]) * (1 - xl[1]) * Xi[0];
+    r += 0.25 * (1 + xl[0]) * (1 - xl[1]) * Xi[1];
+    r += 0.25 * (1 + xl[0]) * (1 + xl[1]) * Xi[2];
+    r += 0.25 * (1 - xl[0]) * (1 + xl[1]) * Xi[3];
+    return r;
+}
+
+/* ----------------------------------------------------------------- EOS -- */
+/* src/eos/eos_state.c:322-341 */
+static double TillCold(const salec_material *t, double E, double R, double *P, double *Cs)
+{
+    double Eta = R / t->TLRho0, Mu = Eta - 1.0;
+    double Phi0 = E / (t->TLE0 * Eta * Eta), Phi1 = 1.0 / (Phi0 + 1.0);
+    double B = Eta < 1.0 ? t->TLA : t->TLB;
+    *P = (t->TLa + t->TLb * Phi1) * E * R + t->TLA * Mu + B * Mu * Mu;
+    double c1 = (1.0 / t->TLRho0) * (t->TLA + 2.0 * B * Mu) + E * (t->TLa + t->TLb * Phi1 * (3.0 - 2.0 * Phi1));
+    double c2 = (t->TLa + t->TLb * Phi1 * Phi1) * (*P) / R;
+    *Cs = Sqrt_trunc(c1 + c2);
+    return *Cs;
+}
+/* src/eos/eos_state.c:343-369 */
+static double TillHot(const salec_material *t, double E, double R, double *P, double *Cs)
+{
+    double Eta = R / t->TLRho0, Mu = Eta - 1.0, Xi = t->TLRho0 / R - 1.0;
+    double Phi0 = E / (t->TLE0 * Eta * Eta), Phi1 = 1.0 / (Phi0 + 1.0);
+    double A2 = 0.0; if (t->TLAlpha * Xi * Xi < 100.0) A2 = exp(-1.0 * t->TLAlpha * Xi * Xi);
+    double B1 = 0.0; if (t->TLBeta * Xi < 100.0) B1 = exp(-1.0 * t->TLBeta * Xi);
+    *P = t->TLa * R * E + (t->TLb * E * R * Phi1 + t->TLA * Mu * B1) * A2;
+    double c1 = t->TLa * E + t->TLb * E * Phi1 * (3.0 + 2.0 * t->TLAlpha * Xi / Eta - 2.0 * Phi1) * A2
+              + t->TLA / t->TLRho0 * (1.0 - (Xi / Eta) * (t->TLBeta + 2.0 * t->TLAlpha * Xi)) * A2 * B1;
+    double c2 = (t->TLa + t->TLb * Phi1 * Phi1 * A2) * (*P) / R;
+    *Cs = Sqrt_trunc(c1 + c2);
+    return *Cs;
+}
+/* src/eos/eos_state.c:371-422 */
+static double TillTran(const salec_material *t, double E, double R, double *P, double *Cs)
+{
+    double ET = t->TLEcv - t->TLEiv, dEr = E - t->TLEiv, dEl = t->TLEcv - E;
+    double Eta = R / t->TLRho0, Mu = Eta - 1.0;
+    double Phi0 = E / (t->TLE0 * Eta * Eta), Phi1 = 1.0 / (Phi0 + 1.0);
+    double Xi = t->TLRho0 / R - 1.0;
+    double B = Eta < 1.0 ? t->TLA : t->TLB;
+    double Pc = (t->TLa + t->TLb * Phi1) * E * R + t->TLA * Mu + B * Mu * Mu;
+    double A2 = 0.0; if (t->TLAlpha * Xi * Xi < 100.0) A2 = exp(-1.0 * t->TLAlpha * Xi * Xi);
+    double B1 = 0.0; if (t->TLBeta * Xi < 100.0) B1 = exp(-1.0 * t->TLBeta * Xi);
+    double Ph = t->TLa * R * E + (t->TLb * E * R * Phi1 + t->TLA * Mu * B1) * A2;
+    *P = (dEl * Pc + dEr * Ph) / ET;
+    double cc1 = (1.0 / t->TLRho0) * (t->TLA + 2.0 * B * Mu) + E * (t->TLa + t->TLb * Phi1 * (3.0 - 2.0 * Phi1));
+    double cc2 = (t->TLa + t->TLb * Phi1 * Phi1) * (*P) / R;
+    double ch1 = t->TLa * E + t->TLb * E * Phi1 * (3.0 + 2.0 * t->TLAlpha * Xi / Eta - 2.0 * Phi1) * A2
+               + t->TLA / t->TLRho0 * (1.0 - (Xi / Eta) * (t->TLBeta + 2.0 * t->TLAlpha * Xi)) * A2 * B1;
+    double ch2 = (t->TLa + t->TLb * Phi1 * Phi1 * A2) * (*P) / R;
+    double c2c = cc1 + cc2, c2h = ch1 + ch2;
+    double s1 = (dEl * c2c + dEr * c2h) / ET;
+    double s2 = (Ph - Pc) / ET * (*P) / (R * R);
+    *Cs = Sqrt_trunc(s1 + s2);
+    return *Cs;
+}
+/* src/eos/eos_state.c:425-438 */
+double oracle_till_pres(const salec_material *t, double E, double R, double *P, double *Cs)
+{
+    if ((R >= t->TLRho0) || (E < t->TLEiv)) TillCold(t, E, R, P, Cs);
+    else if (E > t->TLEcv) TillHot(t, E, R, P, Cs);
+    else TillTran(t, E, R, P, Cs);
+    if (*P <= 1.0e-2) *Cs = Sqrt_trunc(t->TLA / t->TLRho0);
+    return *Cs;
+}
+/* src/eos/eos_state.c:463-517 (RK4 cold-energy table) */
+void oracle_till_cold_energy(const salec_material *t, int nDen, double *xDen, double *yEng)
+{
+    double RefDen = t->TLRho0, MaxDen = t->TLRho0 * 10.0;
+    double StepDen = MaxDen / (nDen - 1);
+    for (int k = 0; k < nDen; k++) { xDen[k] = k * StepDen; yEng[k] = 0.0; }
+    int IndexM = (int)(floor(RefDen / MaxDen * nDen));
+    xDen[IndexM] = t->TLRho0; yEng[IndexM] = 0.0;
+    double K1, K2, K3, K4, tp, tc;
+#define DEDR(E, R) (oracle_till_pres(t, (E), (R), &tp, &tc), tp / ((R) * (R)))
+    for (int k = IndexM + 1; k < nDen; k++) {
+        StepDen = xDen[k] - xDen[k - 1];
+        K1 = DEDR(yEng[k - 1], xDen[k - 1]);
+        K2 = DEDR(yEng[k - 1] + 0.50 * K1 * StepDen, xDen[k - 1] + 0.50 * StepDen);
+        K3 = DEDR(yEng[k - 1] + 0.50 * K2 * StepDen, xDen[k - 1] + 0.50 * StepDen);
+        K4 = DEDR(yEng[k - 1] + K3 * StepDen, xDen[k - 1] + StepDen);
+        yEng[k] = yEng[k - 1] + (K1 + 2.0 * K2 + 2.0 * K3 + K4) * StepDen / 6.0;
+    }
+    for (int k = IndexM - 1; k >= 1; k--) {
+        StepDen = xDen[k] - xDen[k + 1];
+        K1 = DEDR(yEng[k + 1], xDen[k + 1]);
+        K2 = DEDR(yEng[k + 1] + 0.50 * K1 * StepDen, xDen[k + 1] + 0.50 * StepDen);
+        K3 = DEDR(yEng[k + 1] + 0.50 * K2 * StepDen, xDen[k + 1] + 0.50 * StepDen);
+        K4 = DEDR(yEng[k + 1] + K3 * StepDen, xDen[k + 1] + StepDen);
+        yEng[k] = yEng[k + 1] + (K1 + 2.0 * K2 + 2.0 * K3 + K4) * StepDen / 6.0;
+    }
+#undef DEDR
+    yEng[0] = yEng[1];
+}
+/* src/eos/eos_state.c:519-526 */
+double oracle_till_temp(const salec_material *t, double E, double R)
+{
+    int DL = (int)(Wind(floor(R / t->till_stepden), 0, t->till_nden - 2));
+    double Dxl = DL + 1.0 - R / t->till_stepden;
+    double Dxr = 1.0 - Dxl;
+    double rE = Dxl * t->till_yeng[DL] + Dxr * t->till_yeng[DL + 1];
+    return Max((E - rE) / t->TLCv + t->TLTref, 0.0);
+}
+#define TAB(m, i, j, k) ((m)->tab_data[((long)(i) * (m)->tab_nden + (j)) * 3 + (k)])
+/* src/eos/aneos_sieden.c:264-289 (asymmetric clamp: Dxr, Exr unclamped) */
+static void sd_state(const salec_material *t, double den, double eng, double *tem, double *pre, double *csd)
+{
+    double tDen = den, tEng = eng / den;
+    int DI = oracle_bisearch(tDen, t->tab_yden, t->tab_nden), DJ = DI + 1;
+    int TI = oracle_bisearch(tEng, t->tab_xtem, t->tab_ntem), TJ = TI + 1;
+    double Dxr = (tDen - t->tab_yden[DI]) / (t->tab_yden[DJ] - t->tab_yden[DI]);
+    double Dxl = Wind(1.0 - Dxr, 0.0, 1.0);
+    double Exr = (tEng - t->tab_xtem[TI]) / (t->tab_xtem[TJ] - t->tab_xtem[TI]);
+    double Exl = Wind(1.0 - Exr, 0., 1.0);
+    *tem = (TAB(t, TI, DI, 0) * Dxl + TAB(t, TI, DJ, 0) * Dxr) * Exl + (TAB(t, TJ, DI, 0) * Dxl + TAB(t, TJ, DJ, 0) * Dxr) * Exr;
+    *pre = (TAB(t, TI, DI, 1) * Dxl + TAB(t, TI, DJ, 1) * Dxr) * Exl + (TAB(t, TJ, DI, 1) * Dxl + TAB(t, TJ, DJ, 1) * Dxr) * Exr;
+    *csd = (TAB(t, TI, DI, 2) * Dxl + TAB(t, TI, DJ, 2) * Dxr) * Exl + (TAB(t, TJ, DI, 2) * Dxl + TAB(t, TJ, DJ, 2) * Dxr) * Exr;
+}
+/* src/eos/eos_state.c:793-841 (ANEOS temperature-density table) */
+static void aneos_state(const salec_material *t, double den, double eng, double *tem, double *pre, double *csd)
+{
+    int DI = 0, DJ = t->tab_nden - 1;
+    while (DI + 1 < DJ) { int DM = (DI + DJ) / 2; if (t->tab_yden[DM] > den) DJ = DM; else DI = DM; }
+    double Dxr = (den - t->tab_yden[DI]) / (t->tab_yden[DJ] - t->tab_yden[DI]);
+    Dxr = Wind(Dxr, 0.0, 1.0);
+    double Dxl = 1.0 - Dxr;
+    double Energy = eng;
+    int TI = 0;
+    double IRE = TAB(t, TI, DI, 0) * Dxl + TAB(t, TI, DJ, 0) * Dxr;
+    int TJ = t->tab_ntem - 1;
+    double JRE = TAB(t, TJ, DI, 0) * Dxl + TAB(t, TJ, DJ, 0) * Dxr;
+    while (TI + 1 < TJ) {
+        int TM = (TI + TJ) / 2;
+        double MRE = TAB(t, TM, DI, 0) * Dxl + TAB(t, TM, DJ, 0) * Dxr;
+        if (MRE > Energy) { TJ = TM; JRE = MRE; } else { TI = TM; IRE = MRE; }
+    }
+    double Txr = (Energy - IRE) / (JRE - IRE);
+    Txr = Wind(Txr, 0.0, 1.0);
+    double Txl = 1.0 - Txr;
+    *tem = t->tab_xtem[TI] * Txl + t->tab_xtem[TJ] * Txr;
+    *pre = (TAB(t, TI, DI, 1) * Dxl + TAB(t, TI, DJ, 1) * Dxr) * Txl + (TAB(t, TJ, DI, 1) * Dxl + TAB(t, TJ, DJ, 1) * Dxr) * Txr;
+    *csd = (TAB(t, TI, DI, 2) * Dxl + TAB(t, TI, DJ, 2) * Dxr) * Txl + (TAB(t, TJ, DI, 2) * Dxl + TAB(t, TJ, DJ, 2) * Dxr) * Txr;
+}
+/* src/eos/eos_state.c:727-745, 844-850 */
+double oracle_calculate_state(const salec_material *m, double den, double eng, double *tem, double *pre, double *csd)
+{
+    switch (m->eos_type) {
+    case SALEC_EOS_TILLOTSON: {
+        double E = eng / den, R = den;
+        *tem = oracle_till_temp(m, E, R);
+        oracle_till_pres(m, E, R, pre, csd);
+        break;
+    }
+    case SALEC_EOS_ANEOS_SD: sd_state(m, den, eng, tem, pre, csd); break;
+    case SALEC_EOS_ANEOS: aneos_state(m, den, eng, tem, pre, csd); break;
+    default: break;
+    }
+    return -1.0;
+}
+/* src/eos/eos_state.c:747-791 with CAP_MINPRESSURE (eos_state.h:38) */
+static double calculate_state_correct(const salec_material *m, double den, double eng, double plim,
+                                      double *tem, double *pre, double *csd, const double *pty)
+{
+    double tt = 0., tp = 0., tc = 0., cf = 1.0;
+    oracle_calculate_state(m, den * pty[0], eng * pty[0], &tt, &tp, &tc);
+    tp = tp / pty[0];
+    if (pty[0] > 1.0) {
+        double chi = m->chi, a0 = m->alpha0;
+        tc *= (1.0 + (chi - 1.0) * fmax(0.0, fmin(1.0, (*pty - 1.0) / (a0 - 1.0))));
+    }
+    if (tp < plim) tp = Max(tp, plim);
+    *tem = tt; *pre = tp; *csd = tc;
+    return cf;
+}
+
+/* ------------------------------------------------------------ strength -- */
+/* src/strength.c:505-575 */
+static double Ylundborg(double p, double y0, double fric, double ylim) { return y0 + p * fric / (1 + p * fric / (ylim - y0)); }
+static double Ydrucker(double p, double y0, double fric, double ylim) { return Min(y0 + fric * p, ylim); }
+static double SimonMelt0(double pre, const salec_material *s) { return s->Tmelt0 * pow(s->Asimon * pre + 1.0, s->Csimon); }
+static double PolyLiquids(double pre, const salec_material *s)
+{ double p = pre * 1.0e-9; double T = 0.0; T = s->Tlidc1 + s->Tlidc2 * p + s->Tlidc3 * p * p; return T; }
+static double PolySolidus(double pre, const salec_material *s)
+{ double p = pre * 1.0e-9; return s->Tsolc1 + s->Tsolc2 * p + s->Tsolc3 * p * p; }
+static double SimonMelt(double pre, const salec_material *s)
+{
+    double st = SimonMelt0(pre, s);
+    if (s->PolyLiq <= 0 && s->PolySol <= 0) return st;
+    if (s->PolySol > 0) st = PolySolidus(pre, s);
+    double lt = PolyLiquids(pre, s);
+    return Max(st, lt);
+}
+static double OhnakaSoft(double tem, const salec_material *s, double Tm) { return tanh(((Tm + s->Tdelta) / tem - 1.0) * s->Tfrac); }
+/* src/strength.c:521-527 */
+static double JNCKSoft(double tem, double pre, const salec_material *s)
+{
+    double Tm = SimonMelt(pre, s);
+    double f = (tem - s->JCTREF) / (Tm - s->JCTREF);
+    f = Wind(f, 0.0, 1.0);
+    return Wind(1.0 - pow(f, s->JCM), 0.0, 1.0);
+}
+/* src/strength.c:724-751 */
+static double IceMelt(double p)
+{
+    const double p0 = 0.0, p1 = 0.2e9, p2 = 0.34e9, t0 = 273.0, t1 = 252.0, t2 = 257.0;
+    const double p3 = 0.61e9, p4 = 2.17e9, t3 = 274.0, t4 = 354.8;
+    if (p <= p1) return t0 + (t1 - t0) * (p - p0) / (p1 - p0);
+    else if (p > p1 && p <= p2) return t1 + (t2 - t1) * (p - p1) / (p2 - p1);
+    else if (p > p2 && p <= p3) return t2 + (t3 - t2) * (p - p2) / (p3 - p2);
+    else if (p > p3 && p <= p4) return t3 + (t4 - t3) * (p - p3) / (p4 - p3);
+    else return t4 * pow(((p - p4) / 1.253e9) + 1, 1.0 / 3.0);
+}
+/* strength registry, src/strength.c:379-503,697-791 */
+double oracle_strength_fn(int fn, const double *pa, const salec_material *s)
+{
+    switch (fn) {
+    case SALEC_FN_SIMPLEROCK: { /* :412-430 */
+        double Yint = Ylundborg(pa[0], s->Yint0, s->Yfricint, s->Ylimint);
+        double Ydam = Ydrucker(pa[0], s->Ydam0, s->Yfricdam, s->Ylimdam);
+        Ydam = Min(Yint, Ydam);
+        return Ydam * pa[1] + Yint * (1.0 - pa[1]);
+    }
+    case SALEC_FN_SIMPLESHEAR: case SALEC_FN_IVANOV: /* :470-479 */
+        return 1.0 / Max(s->IvanA, s->IvanB * (pa[0] - s->IvanC));
+    case SALEC_FN_JOHNSONCOOK1: case SALEC_FN_JOHNSONCOOK2: { /* :379-409 */
+        double PS = Wind(pa[2], 0.0, s->JCMaxPlastic), DPS = pa[3];
+        double Y1 = s->JCA + s->JCB * pow(PS, s->JCN), Y2 = 1.0;
+        if (DPS > 1.0 && s->JCC > 0.0) Y2 = 1.0 + s->JCC * log(DPS);
+        if (fn == SALEC_FN_JOHNSONCOOK2) return Y1 * Y2 * JNCKSoft(pa[4], pa[0], s) * BETA;
+        return Y1 * Y2 * BETA;
+    }
+    case SALEC_FN_ROCKSTRENGTHT: { /* :697-722 */
+        double pre = pa[0], dam = pa[1], DPS = pa[3], temp = pa[4];
+        double Yint = Ylundborg(pre, s->Yint0, s->Yfricint, s->Ylimint);
+        double Ydam = Ydrucker(pre, s->Ydam0, s->Yfricdam, s->Ylimdam);
+        Ydam = Min(Yint, Ydam);
+        double mt = SimonMelt(pre, s);
+        double sf = OhnakaSoft(temp, s, mt);
+        sf = Wind(sf, 0.0, 1.0);
+        double ymed = (Ydam * dam + Yint * (1.0 - dam)) * sf;
+        double yvis = DPS * s->melt_viscosity * (1.0 - sf);
+        return Max(ymed, yvis);
+    }
+    case SALEC_FN_ICESTRENGTH: { /* :755-772 */
+        double pre = pa[0], dam = pa[1], temp = pa[4];
+        double Yint = Ylundborg(pre, s->Yint0, s->Yfricint, s->Ylimint);
+        double Ydam = Ylundborg(pre, s->Ydam0, s->Yfricdam, s->Ylimdam);
+        Ydam = Min(Yint, Ydam);
+        double sf = OhnakaSoft(temp, s, IceMelt(pre));
+        sf = Wind(sf, 0.0, 1.0);
+        return (Ydam * dam + Yint * (1.0 - dam)) * sf;
+    }
+    case SALEC_FN_COLLINS: { /* :481-503 */
+        double pre = pa[0], inc = 0.;
+        if (pre < s->Pbd) inc = Max(0.01, 0.01 + 0.04 * pre / s->Pbd);
+        else if (pre < s->Pbp) inc = 0.05 + 0.05 * (pre - s->Pbd) / (s->Pbp - s->Pbd);
+        else inc = 0.1 + 0.5 * (pre - s->Pbp) / s->Pbp;
+        return 1.0 / inc;
+    }
+    case SALEC_FN_SIMPLETENSILE: { /* :776-785 */
+        double mt = SimonMelt(pa[0], s);
+        double sf = OhnakaSoft(pa[4], s, mt);
+        sf = Wind(sf, 0.0, 1.0);
+        return s->Yten0 * (1.0 - pa[1]) * sf;
+    }
+    case SALEC_FN_VONMISES1: { /* :432-446 */
+        double Yint = s->Yint0, Ydam = Min(Yint, s->Ydam0);
+        return Ydam * pa[1] + Yint * (1.0 - pa[1]);
+    }
+    case SALEC_FN_VONMISES2: { /* :448-466 */
+        double Yint = s->Yint0, Ydam = Min(Yint, s->Ydam0);
+        double sf = OhnakaSoft(pa[4], s, SimonMelt(pa[0], s));
+        sf = Wind(sf, 0.0, 1.0);
+        return (Ydam * pa[1] + Yint * (1.0 - pa[1])) * sf;
+    }
+    case SALEC_FN_NONE: default:
+        return 0.0; /* ZeroFunc, :788-791 */
+    }
+}
+
+/* ---------------------------------------------------------- ghost cells -- */
+/* ghost table build + set_ghost_pattern + ghost_pad_fill,
+ * src/mpi_vec/mpi_vec_ght.c:12-356 and src/mpi_vec/mpi_vec.c:427-436 */
+enum { BT_L = 1, BT_R = 2, BT_B = 4, BT_T = 8 };
+static void pattern_opts(const salec_config *cfg, int var_type, int lo[4])
+{
+    const int *go = cfg->go_list;
+    switch (var_type) {
+    case O_GHT_ALWAYS_COPY: lo[0] = lo[1] = lo[2] = lo[3] = SALEC_GHT_OUTFLOW; break;
+    case O_GHT_NORMAL: for (int k = 0; k < 4; ++k) lo[k] = go[k]; break;
+    case O_GHT_VELOCITY_INC:
+        for (int k = 0; k < 4; ++k) lo[k] = go[k];
+        if (go[0] == SALEC_GHT_MIRROR) lo[0] = SALEC_GHT_REVERSE_VEL_X;
+        if (go[3] == SALEC_GHT_OUTFLOW) lo[3] = SALEC_GHT_REVERSE_VEL_ABSY;
+        break;
+    case O_GHT_VELOCITY:
+        for (int k = 0; k < 4; ++k) lo[k] = SALEC_GHT_OUTFLOW_VERTEX;
+        if (go[0] == SALEC_GHT_MIRROR) lo[0] = SALEC_GHT_REVERSE_VEL_X;
+        break;
+    case O_GHT_STRESS:
+        for (int k = 0; k < 4; ++k) lo[k] = SALEC_GHT_COPY_STE;
+        if (go[0] == SALEC_GHT_MIRROR) lo[0] = SALEC_GHT_REVERSE_STE_1;
+        break;
+    case O_GHT_DENSITY:
+        lo[0] = lo[1] = lo[2] = SALEC_GHT_OUTFLOW; lo[3] = SALEC_GHT_CLEAN; break;
+    default: lo[0] = lo[1] = lo[2] = lo[3] = SALEC_GHT_OUTFLOW; break;
+    }
+}
+
+void oracle_ghost_fill(oracle_state *s, const salec_config *cfg, int field, int var_type)
+{
+    grid_t g = mkgrid(s);
+    int grid, nc;
+    field_info(field, s->nmat, &grid, &nc);
+    double *d = s->f[field];
+    int lo[4];
+    pattern_opts(cfg, var_type, lo);
+    int W = g.CW, H = g.CH;
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            int bt = 0;
+            if (x == 0) bt |= BT_L;
+            if (y == 0) bt |= BT_B;
+            if (x == W - 1) bt |= BT_R;
+            if (y == H - 1) bt |= BT_T;
+            if (!bt) continue;
+            int ax = x, ay = y, bx = x, by = y, cx = x, cy = y;
+            if (bt & BT_L) { ax = x + 1; bx = x + 2; cx = x + 3; }
+            if (bt & BT_R) { ax = x - 1; bx = x - 2; cx = x - 3; }
+            if (bt & BT_B) { ay = y + 1; by = y + 2; cy = y + 3; }
+            if (bt & BT_T) { ay = y - 1; by = y - 2; cy = y - 3; }
+            if (MINM(x, abs(W - 1 - x)) + MINM(y, abs(H - 1 - y)) <= 1) {
+                /* ghost_pad_move, src/mpi_vec/mpi_vec.c:427-436 */
+                if (bx < 2) bx = 2; if (bx > W - 3) bx = W - 3;
+                if (by < 2) by = 2; if (by > H - 3) by = H - 3;
+                if (cx < 2) cx = 2; if (cx > W - 3) cx = W - 3;
+                if (cy < 2) cy = 2; if (cy > H - 3) cy = H - 3;
+            }
+            int opt;
+            if (bt & BT_L) opt = lo[0];
+            else if (bt & BT_R) opt = lo[1];
+            else if (bt & BT_B) opt = lo[2];
+            else opt = lo[3];
+            long gt = x + (long)W * y, ra = ax + (long)W * ay, rb = bx + (long)W * by, rc = cx + (long)W * cy;
+#define P(i, c) d[(long)(c) * g.NC + (i)]
+            switch (opt) {
+            case SALEC_GHT_OUTFLOW: case SALEC_GHT_COPY_STE:
+                for (int c = 0; c < nc; ++c) P(gt, c) = P(rc, c);
+                for (int c = 0; c < nc; ++c) P(ra, c) = P(rb, c);
+                break;
+            case SALEC_GHT_FIXED:
+                for (int c = 0; c < nc; ++c) P(gt, c) = -1.0 * P(rc, c);
+                for (int c = 0; c < nc; ++c) P(ra, c) = -1.0 * P(rb, c);
+                break;
+            case SALEC_GHT_ZERO:
+                for (int c = 0; c < nc; ++c) P(gt, c) = 0.0;
+                for (int c = 0; c < nc; ++c) P(ra, c) = 0.0;
+                break;
+            case SALEC_GHT_REVERSE_VEL_Y: case SALEC_GHT_REVERSE_STE_1:
+                for (int c = 0; c < nc; ++c) P(gt, c) = P(rc, c);
+                for (int c = 0; c < nc; ++c) P(ra, c) = P(rb, c);
+                P(gt, 1) = -1.0 * P(rc, 1); P(ra, 1) = -1.0 * P(rb, 1);
+                break;
+            case SALEC_GHT_REVERSE_VEL_X:
+                for (int c = 0; c < nc; ++c) P(gt, c) = P(rc, c);
+                for (int c = 0; c < nc; ++c) P(ra, c) = P(rb, c);
+                P(gt, 0) = -1.0 * P(rc, 0); P(ra, 0) = -1.0 * P(rb, 0);
+                break;
+            case SALEC_GHT_REVERSE_VEL_ABSY:
+                for (int c = 0; c < nc; ++c) P(gt, c) = P(rc, c);
+                for (int c = 0; c < nc; ++c) P(ra, c) = P(rb, c);
+                P(gt, 1) = fabs(P(rc, 1)); P(ra, 1) = fabs(P(rb, 1));
+                break;
+            case SALEC_GHT_OUTFLOW_VERTEX:
+                for (int c = 0; c < nc; ++c) P(gt, c) = P(rc, c);
+                for (int c = 0; c < nc; ++c) P(ra, c) = P(rc, c);
+                for (int c = 0; c < nc; ++c) P(rb, c) = P(rc, c);
+                break;
+            case SALEC_GHT_CLEAN:
+                for (int c = 0; c < nc; ++c) P(ra, c) = 0.0;
+                for (int c = 0; c < nc; ++c) P(rb, c) = 0.0;
+                for (int c = 0; c < nc; ++c) P(gt, c) = 0.0;
+                break;
+            default: break; /* unknown option: no-op (returns error) */
+            }
+#undef P
+        }
+    }
+}
+
+/* restrict table + apply_restrict_vertex, src/mpi_vec/mpi_vec_ght.c:440-696 */
+void oracle_restrict(oracle_state *s, const salec_config *cfg, int field)
+{
+    grid_t g = mkgrid(s);
+    double *d = s->f[field];
+    const int *go = cfg->go_list;
+    int ro[4] = {SALEC_RST_NONE, SALEC_RST_NONE, SALEC_RST_NONE, SALEC_RST_NONE};
+    switch (go[1]) {
+    case SALEC_GHT_OUTFLOW: ro[1] = SALEC_RST_NONE; break;
+    case SALEC_GHT_FREESLIP: case SALEC_GHT_REVERSE_VEL_X: ro[1] = SALEC_RST_X_ZERO; break;
+    default: ro[1] = SALEC_RST_ZERO;
+    }
+    switch (go[2]) {
+    case SALEC_GHT_OUTFLOW: ro[2] = SALEC_RST_NONE; break;
+    case SALEC_GHT_FREESLIP: case SALEC_GHT_REVERSE_VEL_Y: ro[2] = SALEC_RST_Y_ZERO; break;
+    default: ro[2] = SALEC_RST_ZERO;
+    }
+    switch (go[0]) {
+    case SALEC_GHT_REVERSE_VEL_X: case SALEC_GHT_MIRROR: ro[0] = SALEC_RST_X_ZERO; break;
+    default: ro[0] = SALEC_RST_NONE;
+    }
+    int W = g.VW, H = g.VH;
+    enum { LP = 0x10, RP = 0x20, TP = 0x40, BP = 0x80 };
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            int bt = 0;
+            if (x == 2) bt |= LP;
+            if (y == 2) bt |= BP;
+            if (x == W - 3) bt |= RP;
+            if (y == H - 3) bt |= TP;
+            if (!bt) continue;
+            int ax = x, ay = y, bx = x, by = y;
+            if (bt & LP) { ax = x + 1; bx = x - 1; }
+            if (bt & RP) { ax = x - 1; bx = x + 1; }
+            if (bt & BP) { ay = y + 1; by = y - 1; }
+            if (bt & TP) { ay = y - 1; by = y + 1; }
+            int opt = SALEC_RST_NONE;
+            if (bt & LP) opt = ro[0];
+            if (bt & RP) opt = ro[1];
+            if (bt & TP) opt = ro[3];
+            if (bt & BP) opt = ro[2];
+            if ((bt & BP) && (bt & RP)) {
+                if (ro[2] != SALEC_RST_NONE && ro[1] != SALEC_RST_NONE) opt = SALEC_RST_ZERO;
+            }
+            long dt_ = x + (long)W * y, ra = ax + (long)W * ay, rb = bx + (long)W * by;
+#define P(i, c) d[(long)(c) * g.NV + (i)]
+            for (int c = 0; c < 2; ++c) P(rb, c) = P(ra, c);
+            for (int c = 0; c < 2; ++c) P(dt_, c) = P(ra, c);
+            switch (opt) {
+            case SALEC_RST_ZERO: P(dt_, 0) = 0.0; P(dt_, 1) = 0.0; break;
+            case SALEC_RST_X_ZERO: P(dt_, 0) = 0.; P(dt_, 1) = P(ra, 1); break;
+            case SALEC_RST_Y_ZERO: P(dt_, 0) = P(ra, 0); P(dt_, 1) = 0.; break;
+            case SALEC_RST_Y_POSITIVE: P(dt_, 1) = fabs(P(dt_, 1)); break;
+            case SALEC_RST_X_POSITIVE: P(dt_, 0) = fabs(P(dt_, 0)); break;
+            default: break;
+            }
+#undef P
+        }
+    }
+}
+
+/* --------------------------------------------------------------- phases -- */
+/* update_v_vof, src/sale2d.c:751-790 */
+static void update_v_vof(oracle_state *s, const grid_t g, int set)
+{
+    int n = g.nmat;
+    for (int j = 0; j < g.VH; ++j) for (int k = 0; k < g.VW; ++k) {
+        if (!vert_in(&g, k, j, set)) continue;
+        long v = VI(k, j);
+        long e[4] = {CI(k - 1, j - 1), CI(k, j - 1), CI(k, j), CI(k - 1, j)};
+        for (int m = 0; m < n; ++m) VF(V_VOF, m, v) = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int m = 0; m < n; ++m) VF(V_VOF, m, v) += 0.25 * CF(M_VOF, m, e[i]);
+    }
+}
+
+/* update_v_vel_inc, src/sale2d.c:930-1048 */
+static void update_v_vel_inc(oracle_state *s, const salec_config *cfg, const grid_t g, int set)
+{
+    static const int voff[4] = {2, 3, 0, 1};
+    for (int j = 0; j < g.VH; ++j) for (int k = 0; k < g.VW; ++k) {
+        if (!vert_in(&g, k, j, set)) continue;
+        long v = VI(k, j);
+        long e[4] = {CI(k - 1, j - 1), CI(k, j - 1), CI(k, j), CI(k - 1, j)};
+        double inc[2] = {0., 0.}, old[2] = {VF(V_VEL, 0, v), VF(V_VEL, 1, v)};
+        double vvol = 0., vmass = 0.;
+        int void_corner = 0;
+        for (int i = 0; i < 4; ++i) {
+            if (CF(E_WPT, voff[i], e[i]) > 0.) void_corner++;
+            double den = CF(E_DEN, 0, e[i]);
+            if (den < TOLRHO) continue;
+            double lv = VF(V_VOL, i + 1, v);
+            vvol += lv;
+            vmass += den * lv;
+            double pa = den * lv;
+            inc[0] += pa * CF(E_DVEL, 0, e[i]);
+            inc[1] += pa * CF(E_DVEL, 1, e[i]);
+        }
+        double vel[2];
+        if (vmass > TOLRHO * vvol) {
+            double pa = 1.0 / vmass, pb = VF(V_MASS, 0, v) / vmass;
+            vel[0] = pa * inc[0] + pb * old[0];
+            vel[1] = pa * inc[1] + pb * old[1];
+            VF(V_MASS, 0, v) = vmass;
+            VF(V_MASS, 1, v) = vvol;
+        } else {
+            VF(V_MASS, 0, v) = 0.; VF(V_MASS, 1, v) = 0.;
+            vel[0] = 0.0; vel[1] = 0.0;
+        }
+        VF(V_MASS, 1, v) = 1.0 * void_corner;
+        double vn = vlen(vel, 2);
+        if (vn > cfg->velocity_max) {
+            double px = cfg->velocity_max / vn;
+            vel[0] *= px; vel[1] *= px;
+        }
+        if (vn < cfg->velocity_min * 1) { vel[0] = 0.0; vel[1] = 0.0; }
+        VF(V_VEL, 0, v) = vel[0]; VF(V_VEL, 1, v) = vel[1];
+    }
+}
+
+/* update_e_vel, src/sale2d.c:838-927 */
+static void update_e_vel(oracle_state *s, const grid_t g)
+{
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        long vv[4] = {VI(k, j), VI(k + 1, j), VI(k + 1, j + 1), VI(k, j + 1)};
+        double ev[2] = {0.0, 0.0};
+        double den = CF(E_DEN, 0, c);
+        CF(E_VEL, 0, c) = 0.0; CF(E_VEL, 1, c) = 0.0;
+        if (den < TOLRHO) continue;
+        int vac = 0;
+        for (int kv = 0; kv < 4; ++kv)
+            if (VF(V_MASS, 1, vv[kv]) >= 4.0 || (VF(V_MASS, 1, vv[kv]) >= 3.0 && CF(E_WPT, kv, c) < 0.)) vac++;
+        if (4 == vac) continue;
+        double sm = 0.;
+        for (int kv = 0; kv < 4; ++kv) {
+            if (CF(E_WPT, kv, c) > 0.0 || den < TOLRHO) continue;
+            double m = VF(V_MASS, 0, vv[kv]);
+            ev[0] += m * VF(V_VEL, 0, vv[kv]);
+            ev[1] += m * VF(V_VEL, 1, vv[kv]);
+            sm += m;
+        }
+        if (sm > 0.) { double px = 1.0 / sm; ev[0] *= px; ev[1] *= px; }
+        else { ev[0] = 0.0; ev[1] = 0.0; }
+        CF(E_VEL, 0, c) = ev[0]; CF(E_VEL, 1, c) = ev[1];
+    }
+}
+
+/* check_v_vel, src/sale2d.c:1051-1141 */
+static void check_v_vel(oracle_state *s, const grid_t g)
+{
+    static const int voff[4] = {2, 3, 0, 1};
+    for (int j = 0; j < g.VH; ++j) for (int k = 0; k < g.VW; ++k) {
+        if (!vert_active(&g, k, j)) continue;
+        long v = VI(k, j);
+        long e[4] = {CI(k - 1, j - 1), CI(k, j - 1), CI(k, j), CI(k - 1, j)};
+        int vc = 0, vk = 0;
+        for (int i = 0; i < 4; ++i) {
+            if (CF(E_DEN, 0, e[i]) < TOLRHO) vc++;
+            if (CF(E_WPT, voff[i], e[i]) > 0.) vk++;
+        }
+        if (vc >= 4) continue;
+        if (vk >= 4) {
+            double av[2] = {0., 0.}, ad = 0.;
+            for (int kv = 0; kv < 4; ++kv) {
+                double d = CF(E_DEN, 0, e[kv]);
+                if (d < TOLRHO) continue;
+                ad += d;
+                av[0] += d * CF(E_VEL, 0, e[kv]);
+                av[1] += d * CF(E_VEL, 1, e[kv]);
+            }
+            double pa = 1.0 / ad;
+            VF(V_VEL, 0, v) = pa * av[0];
+            VF(V_VEL, 1, v) = pa * av[1];
+        }
+    }
+}
+
+/* update_e_gradvel, src/sale2d.c:1144-1235 */
+static void update_e_gradvel(oracle_state *s, const salec_config *cfg, const grid_t g)
+{
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        long vv[4] = {VI(k, j), VI(k + 1, j), VI(k + 1, j + 1), VI(k, j + 1)};
+        double vx[4], vy[4], gr[12];
+        for (int i = 0; i < 4; ++i) { vx[i] = VF(V_VEL, 0, vv[i]); vy[i] = VF(V_VEL, 1, vv[i]); }
+        for (int i = 0; i < 12; ++i) gr[i] = CF(E_GRAD, i, c);
+        double g0 = vdot(gr + 0, vx, 4), g2 = vdot(gr + 0, vy, 4), g1 = vdot(gr + 4, vx, 4);
+        double g3 = vdot(gr + 4, vy, 4), g4 = vdot(gr + 8, vx, 4);
+        CF(E_GRAD_VEL, 0, c) = g0; CF(E_GRAD_VEL, 1, c) = g1; CF(E_GRAD_VEL, 2, c) = g2;
+        CF(E_GRAD_VEL, 3, c) = g3; CF(E_GRAD_VEL, 4, c) = g4;
+        double div = g0 + g3 + g4;
+        CF(E_DIV_VEL, 0, c) = div;
+        if (CF(M_VOF, 0, c) > TOLVOF || CF(E_PRE, 0, c) <= 0.0) {
+            CF(E_Q, 0, c) = 0.;
+        } else {
+            double ea = CF(E_VOL, 0, c) / CF(E_POS, 0, c);
+            ea = fabs(ea);
+            double ql = cfg->avis1 * CF(E_CSD, 0, c) * sqrt(ea);
+            double qq = cfg->avis2 * div * ea;
+            double q = MINM(0., div) * CF(E_DEN, 0, c) * (qq - ql);
+            if (q < TOLQPRE) q = 0.;
+            CF(E_Q, 0, c) = q;
+        }
+    }
+}
+
+/* calculate_dt, src/sale2d.c:1959-2016 */
+static double calculate_dt(oracle_state *s, const grid_t g, double cur_dt)
+{
+    double r = 1.1 * cur_dt;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        if (CF(M_VOF, 0, c) > 1.0 - TOLVOF) continue;
+        double cyl = 1.0;
+        double dr = cyl * CF(E_POS, 2, c) / (fabs(CF(E_VEL, 0, c)) + CF(E_CSD, 0, c));
+        double dz = 1.0 * CF(E_POS, 3, c) / (fabs(CF(E_VEL, 1, c)) + CF(E_CSD, 0, c));
+        double drz = DTF * MINM(dr, dz);
+        r = MINM(r, drz);
+    }
+    return r;
+}
+
+static int acoustic_fluid(const salec_config *cfg)
+{
+    for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].Toff > 0.0) return 1;
+    return 0;
+}
+
+/* block_vibration, src/strength.c:625-695 */
+static void block_vibration(oracle_state *s, const salec_config *cfg, const grid_t g, double dt, double time)
+{
+    if (!acoustic_fluid(cfg)) return;
+    int n = g.nmat;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        if (CF(M_VOF, 0, c) > TOLVOF || CF(E_DAM, 0, c) < 0.5) { CF(E_VIB, 0, c) = 0.; continue; }
+        double Td = -1.0, Cv = 0.0, To = -1.0, VM = 0.0, Pv = 0.0;
+        for (int m = 1; m < n; ++m) {
+            if (CF(M_VOF, m, c) < TOLVOF) continue;
+            const salec_material *r = cfg->mat + m;
+            Td = Max(r->Tdamp, Td); Cv = Max(r->Cvib, Cv); To = Max(r->Toff, To);
+            VM = Max(r->VibMax, VM); Pv = Max(r->Pvlim, Pv);
+        }
+        if (Td <= 0. || To <= 0.) { CF(E_VIB, 0, c) = 0.; continue; }
+        double ev[2] = {CF(E_VEL, 0, c), CF(E_VEL, 1, c)};
+        double nv = Cv * vlen(ev, 2), nv2 = nv * nv;
+        double vib = CF(E_VIB, 0, c);
+        if ((time <= To) && (nv2 >= vib) && (nv >= 1.0) && (CF(E_Q, 0, c) > 0.))
+            CF(E_VIB, 0, c) = Min(nv2, VM * VM);
+        else
+            CF(E_VIB, 0, c) = Max(vib * (1.0 - 2.0 * dt / Td + 2.0 * (dt / Td) * (dt / Td)), 0.0);
+    }
+}
+
+/* update_e_stress, src/strength.c:7-349 (strength_mod = avg_strength) */
+static void update_e_stress(oracle_state *s, const salec_config *cfg, const grid_t g, double dt)
+{
+    int n = g.nmat, af = acoustic_fluid(cfg);
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        double ste[4];
+        for (int i = 0; i < 4; ++i) ste[i] = CF(E_STE, i, c);
+        if (CF(M_VOF, 0, c) > TOLVOF) {
+            for (int i = 0; i < 4; ++i) CF(E_STE, i, c) = 0.0;
+            CF(E_STE2S, 0, c) = 0.; CF(E_STA2S, 0, c) = 0.; CF(E_STRENGTH, 0, c) = 0.;
+            continue;
+        }
+        double vref = 0., mref = 0.;
+        for (int m = 1; m < n; ++m) {
+            if (CF(M_VOF, m, c) < TOLVOF) continue;
+            vref += CF(M_VOF, m, c) * cfg->mat[m].VaporDen;
+            mref += CF(M_VOF, m, c) * cfg->mat[m].MeltDen;
+        }
+        double mp = 1.0;
+        mp = Max(mp, 1.0);
+        vref /= mp; mref /= mp;
+        int vapor = 0, melt = 0;
+        double s2o = CF(E_STE2S, 0, c);
+        double eden = CF(E_DEN, 0, c);
+        double ste2s = s2o, sta2s = CF(E_STA2S, 0, c), tps = CF(E_TPS, 0, c), dam = CF(E_DAM, 0, c);
+        double strength = CF(E_STRENGTH, 0, c);
+        if (eden <= vref) {
+            for (int i = 0; i < 4; ++i) ste[i] = 0.0;
+            ste2s = 0.; tps = 0.; vapor = 1; dam = 0.; strength = 0.;
+        } else if (eden <= mref) {
+            for (int i = 0; i < 4; ++i) ste[i] = 0.0;
+            ste2s = 0.; melt = 1;
+            double si[4] = {CF(E_GRAD_VEL, 0, c), 0.5 * CF(E_GRAD_VEL, 1, c) + 0.5 * CF(E_GRAD_VEL, 2, c),
+                            CF(E_GRAD_VEL, 3, c), CF(E_GRAD_VEL, 4, c)};
+            double tr = si[0] + si[2] + si[3];
+            si[0] -= tr / 3.0; si[2] -= tr / 3.0; si[3] -= tr / 3.0;
+            sta2s = sqrt(second_inv(si));
+            tps = Min(tps + sta2s * dt, 10.);
+            dam = 1.; strength = 0.;
+        } else {
+            double gv[5];
+            for (int i = 0; i < 5; ++i) gv[i] = CF(E_GRAD_VEL, i, c);
+            double si[4] = {gv[0], 0.5 * gv[1] + 0.5 * gv[2], gv[3], gv[4]};
+            double tr = si[0] + si[2] + si[3];
+            si[0] -= tr / 3.0; si[2] -= tr / 3.0; si[3] -= tr / 3.0;
+            double s2w = (gv[2] - gv[1]) * dt;
+            s2w = Wind(s2w, -1., 1.);
+            double c2w = sqrt(1.0 - s2w * s2w) - 1.0;
+            double ss2 = (ste[0] - ste[2]) / 2.0;
+            double rot[4] = {ss2 * c2w + ste[1] * s2w, ste[1] * c2w - ss2 * s2w, -ss2 * c2w - ste[1] * s2w, 0.0};
+            for (int i = 0; i < 4; ++i) ste[i] += 1.0 * rot[i];
+            double pa = 2.0 * CF(E_MU, 0, c) * dt;
+            for (int i = 0; i < 4; ++i) ste[i] += pa * si[i];
+            ste2s = sqrt(second_inv(ste));
+            sta2s = sqrt(second_inv(si));
+        }
+        double pre = CF(E_PRE, 0, c), tem = CF(E_TEM, 0, c), vib = CF(E_VIB, 0, c), csd = CF(E_CSD, 0, c);
+        if (vapor) {
+            dam = 0.;
+        } else if (melt) {
+            dam = 1.0;
+        } else {
+            double lss = 0.0, lts = 0.;
+            for (int m = 1; m < n; ++m) {
+                if (CF(M_VOF, m, c) < TOLVOF) continue;
+                const salec_material *r = cfg->mat + m;
+                double pa[5] = {pre, dam, tps, sta2s, tem};
+                double my = oracle_strength_fn(r->yfunc, pa, r);
+                if (af && pre >= 0.) {
+                    double vp = Min(eden * csd * sqrt(vib), pre);
+                    double mt = SimonMelt(pre, r);
+                    double sf = OhnakaSoft(tem, r, mt);
+                    sf = Wind(sf, 0.0, 1.0);
+                    double lp = pre - vp * dam * dam * sqrt(sf);
+                    double pb[5] = {lp, dam, tps, sta2s, tem};
+                    double myv = oracle_strength_fn(r->yfunc, pb, r) + eden * r->Acvis * sta2s;
+                    my = Min(myv, my);
+                }
+                lss += CF(M_VOF, m, c) * Max(my, 0.0);
+                double mtn = oracle_strength_fn(r->tfunc, pa, r);
+                lts += CF(M_VOF, m, c) * mtn;
+            }
+            strength = lss;
+            if (lss < ste2s) {
+                sta2s = sta2s / Max(0.5, (ste2s - s2o) / (ste2s - lss));
+                double sfac = lss / ste2s;
+                for (int i = 0; i < 4; ++i) ste[i] *= sfac;
+                ste2s = lss;
+                double dsd = 0.;
+                for (int m = 1; m < n; ++m) {
+                    double pd[5] = {pre, dam, 0., 0., 0.};
+                    dsd += CF(M_VOF, m, c) * oracle_strength_fn(cfg->mat[m].dfunc, pd, cfg->mat + m);
+                }
+                dam = Wind(dam + sta2s * dsd * dt, 0.0, 1.0);
+                tps = Min(tps + sta2s * dt, 10.);
+            }
+            if (cfg->tensile_failure) {
+                double Sxx = ste[0], Szz = ste[2], Sxz = ste[1], Sth = ste[3];
+                double Ts = sqrt(Sxz * Sxz + 0.25 * (Sxx - Szz) * (Sxx - Szz));
+                double Tmax = Max(0.5 * (Sxx + Szz) - Ts, 0.5 * (Sxx + Szz) + Ts);
+                Tmax = Max(Tmax, Sth);
+                if ((Tmax - pre) > lts) {
+                    double dinc = 0.4 * csd / Min(CF(E_POS, 1, c), CF(E_POS, 2, c));
+                    double dold = dam;
+                    double t3 = pow(dold, 1.0 / 3.);
+                    t3 += dinc;
+                    dam = Wind(pow(t3, 3.0), 0., 1.);
+                    double sfac = dold < 0.98 ? (1.0 - dam) / (1.0 - dold) : 0.0;
+                    for (int i = 0; i < 4; ++i) ste[i] *= sfac;
+                    ste2s *= sfac;
+                }
+            }
+        }
+        for (int i = 0; i < 4; ++i) CF(E_STE, i, c) = ste[i];
+        CF(E_STE2S, 0, c) = ste2s; CF(E_STA2S, 0, c) = sta2s; CF(E_TPS, 0, c) = tps;
+        CF(E_DAM, 0, c) = dam; CF(E_STRENGTH, 0, c) = strength;
+    }
+}
+
+/* calculate_acc_pml, src/velocity.c:158-340 */
+static void calculate_acc_pml(oracle_state *s, const salec_config *cfg, const grid_t g, double dt, int set)
+{
+    for (int j = 0; j < g.VH; ++j) for (int k = 0; k < g.VW; ++k) {
+        if (!vert_in(&g, k, j, set)) continue;
+        long v = VI(k, j);
+        long e[4] = {CI(k - 1, j - 1), CI(k, j - 1), CI(k, j), CI(k - 1, j)};
+        double acc[2] = {0.0, 0.0};
+        double vm = 0.0, vv = 0.0;
+        double vs[2] = {0., 0.};
+        for (int i = 0; i < 4; ++i) { vs[0] += 0.25 * CF(E_SIGMA, 0, e[i]); vs[1] += 0.25 * CF(E_SIGMA, 1, e[i]); }
+        double al[2] = {VF(V_ALPHA, 0, v), VF(V_ALPHA, 1, v)};
+        int damp = (vs[0] > 0. || vs[1] > 0.);
+        for (int i = 0; i < 4; ++i) {
+            double lp = CF(E_PRE, 0, e[i]) + CF(E_Q, 0, e[i]);
+            double lpr = CF(E_DAMPREF, 2, e[i]);
+            double Sxx = CF(E_STE, 0, e[i]) - lp, Szz = CF(E_STE, 2, e[i]) - lp, Sxz = CF(E_STE, 1, e[i]), Sth = CF(E_STE, 3, e[i]) - lp;
+            double su0 = VF(V_SUF, 2 * i, v), su1 = VF(V_SUF, 2 * i + 1, v), sv = VF(V_SUBVOL, i, v);
+            double f0 = 0, f1 = 0;
+            f0 += su0 * Sxx; f0 += su1 * Sxz;
+            f1 += su0 * Sxz; f1 += su1 * Szz;
+            f0 += sv * Sth;
+            acc[0] += -1.0 * f0; acc[1] += -1.0 * f1;
+            if (damp) {
+                Sxx += lpr; Szz += lpr; Sth += lpr;
+                double a0 = 0, a1 = 0;
+                a0 += su0 * (vs[1] * Sxx); a0 += su1 * (vs[0] * Sxz);
+                a1 += su0 * (vs[1] * Sxz); a1 += su1 * (vs[0] * Szz);
+                al[0] += a0 * dt;
+                al[1] += a1 * dt;
+                al[0] += (vs[0] + vs[1]) * sv * Sth * dt;
+            }
+            double lv = VF(V_VOL, 1 + i, v);
+            vm += lv * CF(E_DEN, 0, e[i]);
+            vv += lv;
+        }
+        if (damp) {
+            al[0] -= cfg->alpha_damp_x * al[0] * dt;
+            al[1] -= cfg->alpha_damp_y * al[1] * dt;
+            acc[0] += -1.0 * al[0]; acc[1] += -1.0 * al[1];
+        }
+        VF(V_ALPHA, 0, v) = al[0]; VF(V_ALPHA, 1, v) = al[1];
+        double anc[2] = {0., 0.};
+        if (vert_pad(&g, k, j)) {
+            anc[0] = 0.0; anc[1] = 0.0;
+        } else {
+            long nb[9] = {VI(k - 1, j - 1), VI(k, j - 1), VI(k + 1, j - 1), VI(k - 1, j), VI(k, j), VI(k + 1, j),
+                          VI(k - 1, j + 1), VI(k, j + 1), VI(k + 1, j + 1)};
+            double Lx = 0, Ly = 0;
+            for (int q = 0; q < 9; ++q) { double w = VF(V_LAPLACE, q, v); Lx += w * VF(V_VEL, 0, nb[q]); Ly += w * VF(V_VEL, 1, nb[q]); }
+            double sq = sqrt(fabs(VF(V_SUBVOL, 0, v)) + fabs(VF(V_SUBVOL, 2, v)));
+            anc[0] = 5.0e3 * sq * Lx;
+            anc[1] = 5.0e3 * sq * Ly;
+        }
+        if (VF(V_MASS, 1, v) >= 1.) { anc[0] = 0.0; anc[1] = 0.0; }
+        if (vm > TOLRHO * vv) {
+            double px = 1.0 / vm;
+            acc[0] *= px; acc[1] *= px;
+            acc[0] += 1.0 * VF(V_BF, 0, v); acc[1] += 1.0 * VF(V_BF, 1, v);
+            acc[0] += cfg->anc * anc[0]; acc[1] += cfg->anc * anc[1];
+        } else {
+            acc[0] = VF(V_BF, 0, v); acc[1] = VF(V_BF, 1, v);
+        }
+        if (fabs(acc[0]) < cfg->acceleration_min * 1) acc[0] = 0.;
+        if (fabs(acc[1]) < cfg->acceleration_min * 1) acc[1] = 0.;
+        VF(V_ACC, 0, v) = acc[0]; VF(V_ACC, 1, v) = acc[1];
+    }
+}
+
+/* calculate_acc_default, src/velocity.c:9-155 (pml, sbc, ctlg, none) */
+static void calculate_acc_default(oracle_state *s, const salec_config *cfg, const grid_t g, int set)
+{
+    for (int j = 0; j < g.VH; ++j) for (int k = 0; k < g.VW; ++k) {
+        if (!vert_in(&g, k, j, set)) continue;
+        long v = VI(k, j);
+        long e[4] = {CI(k - 1, j - 1), CI(k, j - 1), CI(k, j), CI(k - 1, j)};
+        double acc[2] = {0.0, 0.0};
+        double vm = 0.0, vv = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            double su0 = VF(V_SUF, 2 * i, v), su1 = VF(V_SUF, 2 * i + 1, v), sv = VF(V_SUBVOL, i, v);
+            double fp[2] = {0., 0.};
+            double lp = CF(E_PRE, 0, e[i]) + CF(E_Q, 0, e[i]);
+            fp[0] += lp * su0; fp[1] += lp * su1;
+            fp[0] += sv * lp;
+            double Sxx = CF(E_STE, 0, e[i]), Szz = CF(E_STE, 2, e[i]), Sxz = CF(E_STE, 1, e[i]), Sth = CF(E_STE, 3, e[i]);
+            double f0 = 0, f1 = 0;
+            f0 += su0 * Sxx; f0 += su1 * Sxz;
+            f1 += su0 * Sxz; f1 += su1 * Szz;
+            f0 += sv * Sth;
+            double lv = VF(V_VOL, 1 + i, v);
+            acc[0] += -1.0 * f0; acc[1] += -1.0 * f1;
+            acc[0] += 1.0 * fp[0]; acc[1] += 1.0 * fp[1];
+            vm += lv * CF(E_DEN, 0, e[i]);
+            vv += lv;
+        }
+        double anc[2] = {0., 0.};
+        if (vert_pad(&g, k, j)) {
+            anc[0] = 0.0; anc[1] = 0.0;
+        } else {
+            long nb[9] = {VI(k - 1, j - 1), VI(k, j - 1), VI(k + 1, j - 1), VI(k - 1, j), VI(k, j), VI(k + 1, j),
+                          VI(k - 1, j + 1), VI(k, j + 1), VI(k + 1, j + 1)};
+            double Lx = 0, Ly = 0;
+            for (int q = 0; q < 9; ++q) { double w = VF(V_LAPLACE, q, v); Lx += w * VF(V_VEL, 0, nb[q]); Ly += w * VF(V_VEL, 1, nb[q]); }
+            double sq = sqrt(fabs(VF(V_SUBVOL, 0, v)) + fabs(VF(V_SUBVOL, 2, v)));
+            anc[0] = 5.0e3 * sq * Lx;
+            anc[1] = 5.0e3 * sq * Ly;
+        }
+        if (VF(V_MASS, 1, v) >= 1.) { anc[0] = 0.0; anc[1] = 0.0; }
+        if (vm > TOLRHO * vv) {
+            double px = 1.0 / vm;
+            acc[0] *= px; acc[1] *= px;
+            acc[0] += 1.0 * VF(V_BF, 0, v); acc[1] += 1.0 * VF(V_BF, 1, v);
+            acc[0] += cfg->anc * anc[0]; acc[1] += cfg->anc * anc[1];
+        } else {
+            acc[0] = 0.0; acc[1] = 0.0;
+        }
+        if (fabs(acc[0]) < cfg->acceleration_min * 1) acc[0] = 0.;
+        if (fabs(acc[1]) < cfg->acceleration_min * 1) acc[1] = 0.;
+        VF(V_ACC, 0, v) = acc[0]; VF(V_ACC, 1, v) = acc[1];
+    }
+}
+
+/* calculate_v_dis, src/sale2d.c:1238-1268 */
+static void calculate_v_dis(oracle_state *s, const grid_t g, double dt)
+{
+    for (int j = 0; j < g.VH; ++j) for (int k = 0; k < g.VW; ++k) {
+        if (!vert_active(&g, k, j)) continue;
+        long v = VI(k, j);
+        double pb = 0.5 * dt * dt;
+        for (int c = 0; c < 2; ++c) {
+            VF(V_DIS, c, v) = dt * VF(V_VEL, c, v) + pb * VF(V_ACC, c, v);
+            VF(V_VEL, c, v) += dt * VF(V_ACC, c, v);
+        }
+    }
+}
+
+/* update_energy, src/sale2d.c:564-712 (porosity model off) */
+static void update_energy(oracle_state *s, const salec_config *cfg, const grid_t g, double dt)
+{
+    int n = g.nmat;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        if (CF(M_VOF, 0, c) > TOLVOF || CF(E_DEN, 0, c) < TOLRHO) continue;
+        double lp = CF(E_PRE, 0, c) + CF(E_Q, 0, c);
+        double div = CF(E_DIV_VEL, 0, c);
+        double dEp = -lp * div;
+        double dEs = CF(E_STE, 0, c) * CF(E_GRAD_VEL, 0, c)
+                   + CF(E_STE, 1, c) * (CF(E_GRAD_VEL, 1, c) + CF(E_GRAD_VEL, 2, c))
+                   + CF(E_STE, 2, c) * CF(E_GRAD_VEL, 3, c) + CF(E_STE, 3, c) * CF(E_GRAD_VEL, 4, c);
+        double eg = 0.;
+        for (int m = 1; m < n; ++m)
+            if (CF(M_VOF, m, c) > TOLVOF) eg += CF(M_VOF, m, c) * cfg->mat[m].gamfrac / CF(M_DEN, m, c);
+        double dcvs = div - eg * (dEp + dEs);
+        for (int m = 1; m < n; ++m) {
+            if (CF(M_VOF, m, c) <= TOLVOF || CF(M_DEN, m, c) <= TOLRHO) {
+                CF(M_ENG, m, c) = 0.;
+            } else {
+                double f = 1.0;
+                CF(M_ENG, m, c) = Max(0., CF(M_ENG, m, c) + (dEp + dEs) * dt * f);
+            }
+        }
+        CF(E_CVS, 0, c) = Wind(CF(E_CVS, 0, c) + dcvs * dt, -10.0, 10.0);
+    }
+}
+
+/* get_flux_volcyl, src/advect.c:1347-1354 */
+double oracle_get_flux_volcyl(const double *ba, const double *bb, const double *da, const double *db)
+{
+    double xa = ba[0], ya = ba[1], xb = bb[0], yb = bb[1];
+    double x1 = -1.0 * da[0], y1 = -1.0 * da[1], x2 = -1.0 * db[0], y2 = -1.0 * db[1];
+    double vol = x1 * (ya - yb - y2) + x2 * (ya - yb + y1) + (y1 + y2) * (xb - xa);
+    return vol * fabs(2.0 * xa + 2.0 * xb + x1 + x2) / 8.0;
+}
+/* cut_length(_l/_s), src/advect.c:1356-1408 */
+double oracle_cut_length(double vf, double a, double b)
+{
+    if (a < b) { double t = a; a = b; b = t; }
+    if (1.0e-5 * a > b) {
+        double d = 0.0;
+        if (vf <= 0.0) d = 0.0; else if (vf <= 1.0) d = vf * a; else d = a;
+        return d + b;
+    } else {
+        double d, crf1 = 0.5 * b / a;
+        if (vf <= crf1) d = sqrt(2.0 * a * b * vf);
+        else if (vf <= 1.0 - crf1) d = 0.5 * b + a * vf;
+        else if (vf <= 1.0) d = a + b - sqrt(2.0 * a * b * (1.0 - vf));
+        else d = a + b;
+        return d;
+    }
+}
+/* geometric_flux_ratio, src/advect.c:1312-1330 */
+double oracle_geometric_flux_ratio(double p1, double p2)
+{
+    double a = p1, b = p2;
+    if (b > a) { a = p2; b = p1; }
+    if (b >= 0.0) return 1.0;
+    else if (a <= 0.0) return 0.0;
+    else return a / (a + fabs(b));
+}
+/* vofMSTACS, src/advect.c:1259-1274 */
+double oracle_vof_mstacs(double nv, double gf, double cd)
+{
+    double cds, sto;
+    if (cd < 1.0 / 3.0) cds = Min(nv / cd, 1.0); else cds = Min(3.0 * nv, 1.0);
+    if (nv < 1.0 / 5.0) sto = 3.0 * nv;
+    else if (nv < 1.0 / 2.0) sto = (1.0 + nv) / 2.0;
+    else sto = Min((6.0 * nv + 3.0) / 8.0, 1.0);
+    return gf * cds + (1.0 - gf) * sto;
+}
+/* algebraic_flux_ratio, src/advect.c:1226-1252 */
+static double algebraic_flux_ratio(double U, double D, double A, double cd, const double *bn, const double *gv)
+{
+    double tmp = A - U;
+    double nD = (D - U) / tmp;
+    double bv = 0.0;
+    if (fabs(tmp) < TOLVOF) bv = D;
+    else if (nD < TOLVOF || nD > 1.0 - TOLVOF) bv = D;
+    else {
+        double ct = vdot(gv, bn, 2) / vlen(gv, 2);
+        ct = ct * ct;
+        double gf = Min(CISSAMK1 * ct * ct, 1.0);
+        double nf = oracle_vof_mstacs(nD, gf, cd);
+        bv = U + nf * (A - U);
+    }
+    return bv;
+}
+
+/* calculate_flux_mixed + _calculate_flux_set_bphi, src/advect.c:861-1224 */
+static void calculate_flux_mixed(oracle_state *s, const grid_t g, int physical)
+{
+    int n = g.nmat, PL = 3 * n + 10;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (cell_physical(&g, k, j) != physical) continue;
+        long c = CI(k, j);
+        long vv[4] = {VI(k, j), VI(k + 1, j), VI(k + 1, j + 1), VI(k, j + 1)};
+        double vp[4][2], vd[4][2];
+        for (int i = 0; i < 4; ++i) {
+            vp[i][0] = VF(V_POS, 0, vv[i]); vp[i][1] = VF(V_POS, 1, vv[i]);
+            vd[i][0] = VF(V_DIS, 0, vv[i]); vd[i][1] = VF(V_DIS, 1, vv[i]);
+        }
+        double lv = oracle_get_flux_volcyl(vp[0], vp[3], vd[0], vd[3]);
+        double rv = oracle_get_flux_volcyl(vp[2], vp[1], vd[2], vd[1]);
+        double bv = oracle_get_flux_volcyl(vp[1], vp[0], vd[1], vd[0]);
+        double uv = oracle_get_flux_volcyl(vp[3], vp[2], vd[3], vd[2]);
+        double lbru[4] = {lv, bv, rv, uv};
+        double ev[SALEC_MAXMAT];
+        for (int m = 0; m < n; ++m) ev[m] = CF(M_VOF, m, c);
+        double evol = CF(E_VOL, 0, c);
+        double fv[SALEC_MAXMAT][4];
+        memset(fv, 0, sizeof(fv));
+        int pm = 0;
+        if (ev[0] < TOLVOF)
+            for (int q = 1; q < n; ++q) if (ev[q] > ev[pm]) pm = q;
+        if (ev[pm] > 1.0 - TOLVOF) {
+            for (int q = 0; q < n; ++q) {
+                if (q == pm) {
+                    fv[q][0] = MINM(lv, 0); fv[q][2] = -1.0 * MINM(rv, 0);
+                    fv[q][1] = MINM(bv, 0); fv[q][3] = -1.0 * MINM(uv, 0);
+                } else { fv[q][0] = fv[q][1] = fv[q][2] = fv[q][3] = 0.; }
+            }
+            for (int i = 0; i < 4; ++i) CF(E_WPT, i, c) = (pm == 0) ? 1.0 : -0.25;
+        } else {
+            double gr[12];
+            for (int i = 0; i < 12; ++i) gr[i] = CF(E_GRAD, i, c);
+            double lvv[4], gvec[2];
+            for (int i = 0; i < 4; ++i) lvv[i] = VF(V_VOF, pm, vv[i]);
+            gvec[0] = vdot(lvv, gr + 0, 4); gvec[1] = vdot(lvv, gr + 4, 4);
+            vnorm(gvec, 2);
+            double ca = CF(E_POS, 2, c) * fabs(gvec[0]), cb = CF(E_POS, 3, c) * fabs(gvec[1]);
+            if (ca < cb) { double t = ca; ca = cb; cb = t; }
+            double dstar = oracle_cut_length(ev[pm], ca, cb);
+            double w[4], mw;
+            for (int i = 0; i < 4; ++i) w[i] = vdot(gvec, vp[i], 2);
+            mw = MAXM(MAXM(w[0], w[1]), MAXM(w[2], w[3]));
+            for (int i = 0; i < 4; ++i) w[i] = w[i] - mw + dstar;
+            if (pm == 0) for (int i = 0; i < 4; ++i) CF(E_WPT, i, c) = w[i];
+            else for (int i = 0; i < 4; ++i) CF(E_WPT, i, c) = -0.25;
+            double fp[4];
+            fp[0] = oracle_geometric_flux_ratio(w[3], w[0]);
+            fp[2] = oracle_geometric_flux_ratio(w[2], w[1]);
+            fp[1] = oracle_geometric_flux_ratio(w[0], w[1]);
+            fp[3] = oracle_geometric_flux_ratio(w[3], w[2]);
+            double cc = MINM(lv, 0.0) + MINM(rv, 0) + MINM(bv, 0) + MINM(uv, 0);
+            cc = fabs(cc) / evol;
+            long nb[4] = {CI(k - 1, j), CI(k, j - 1), CI(k + 1, j), CI(k, j + 1)};
+            int nbok[4] = {k - 1 >= 0, j - 1 >= 0, k + 1 < g.CW, j + 1 < g.CH};
+            double bnm[4][2];
+            bnm[0][0] = s->f[SALEC_BX_NORMAL][BXI(k, j)]; bnm[0][1] = s->f[SALEC_BX_NORMAL][g.NBX + BXI(k, j)];
+            bnm[2][0] = s->f[SALEC_BX_NORMAL][BXI(k + 1, j)]; bnm[2][1] = s->f[SALEC_BX_NORMAL][g.NBX + BXI(k + 1, j)];
+            bnm[1][0] = s->f[SALEC_BY_NORMAL][BYI(k, j)]; bnm[1][1] = s->f[SALEC_BY_NORMAL][g.NBY + BYI(k, j)];
+            bnm[3][0] = s->f[SALEC_BY_NORMAL][BYI(k, j + 1)]; bnm[3][1] = s->f[SALEC_BY_NORMAL][g.NBY + BYI(k, j + 1)];
+            double fnp[SALEC_MAXMAT][4];
+            memset(fnp, 0, sizeof(fnp));
+            double snp[4] = {0., 0., 0., 0.};
+            for (int m = 1; m < n; ++m) {
+                if (pm == m) continue;
+                double nvv[4], ng[2];
+                for (int i = 0; i < 4; ++i) nvv[i] = VF(V_VOF, m, vv[i]);
+                ng[0] = vdot(nvv, gr + 0, 4); ng[1] = vdot(nvv, gr + 4, 4);
+                vnorm(ng, 2);
+                double nbv[4];
+                for (int i = 0; i < 4; ++i) nbv[i] = nbok[i] ? CF(M_VOF, m, nb[i]) : ev[m];
+                double up[4] = {nbv[0], nbv[1], nbv[2], nbv[3]};
+                double ac[4] = {nbv[2], nbv[3], nbv[0], nbv[1]};
+                for (int i = 0; i < 4; ++i) {
+                    fnp[m][i] = algebraic_flux_ratio(up[i], ev[m], ac[i], cc, bnm[i], ng);
+                    snp[i] += fnp[m][i];
+                }
+            }
+            for (int i = 0; i < 4; ++i) {
+                if (snp[i] > TOLVOF) { for (int m = 1; m < n; ++m) fnp[m][i] = fnp[m][i] / snp[i]; }
+                else { for (int m = 1; m < n; ++m) fnp[m][i] = ev[m] / (1.0 - ev[pm]); }
+            }
+            for (int q = 0; q < n; ++q) {
+                if (q == pm) {
+                    fv[q][0] = MINM(lv, 0.0) * fp[0]; fv[q][2] = -1.0 * MINM(rv, 0.0) * fp[2];
+                    fv[q][1] = MINM(bv, 0.0) * fp[1]; fv[q][3] = -1.0 * MINM(uv, 0.0) * fp[3];
+                } else {
+                    fv[q][0] = MINM(lv, 0.0) * (1.0 - fp[0]) * fnp[q][0];
+                    fv[q][2] = -1.0 * MINM(rv, 0.0) * (1.0 - fp[2]) * fnp[q][2];
+                    fv[q][1] = MINM(bv, 0.0) * (1.0 - fp[1]) * fnp[q][1];
+                    fv[q][3] = -1.0 * MINM(uv, 0.0) * (1.0 - fp[3]) * fnp[q][3];
+                }
+            }
+        }
+        for (int q = 1; q < n; ++q) {
+            double dir[4] = {1.0, 1.0, -1.0, -1.0};
+            double cvv = -1.0 * vdot(fv[q], dir, 4);
+            if (cvv > ev[q] * evol) {
+                double px = ev[q] * evol / cvv;
+                for (int i = 0; i < 4; ++i) fv[q][i] *= px;
+            }
+        }
+        /* _calculate_flux_set_bphi */
+        double *ph[4] = {s->bx_phi + BXI(k, j), s->by_phi + BYI(k, j), s->bx_phi + BXI(k + 1, j), s->by_phi + BYI(k, j + 1)};
+        long st[4] = {g.NBX, g.NBY, g.NBX, g.NBY};
+        for (int i = 0; i < 4; ++i) {
+            if (lbru[i] >= 0.) continue;
+            for (int q = 0; q < n; ++q) {
+                ph[i][(long)(q + 0 * n) * st[i]] = fv[q][i];
+                ph[i][(long)(q + 1 * n) * st[i]] = CF(M_DEN, q, c);
+                ph[i][(long)(q + 2 * n) * st[i]] = CF(M_ENG, q, c);
+            }
+            double fm = 1.0;
+            ph[i][(long)(3 * n + 0) * st[i]] = CF(E_VEL, 0, c);
+            ph[i][(long)(3 * n + 1) * st[i]] = CF(E_VEL, 1, c);
+            for (int q = 0; q < 4; ++q) ph[i][(long)(3 * n + 2 + q) * st[i]] = fm * CF(E_STE, q, c);
+            ph[i][(long)(3 * n + 6) * st[i]] = fm * CF(E_DAM, 0, c);
+            ph[i][(long)(3 * n + 7) * st[i]] = fm * CF(E_TPS, 0, c);
+            ph[i][(long)(3 * n + 8) * st[i]] = fm * CF(E_VIB, 0, c);
+            ph[i][(long)(3 * n + 9) * st[i]] = fm * CF(E_EJT, 0, c);
+        }
+        (void)PL;
+    }
+}
+
+/* clean_flux, src/advect.c:597-626 */
+static void clean_flux(oracle_state *s, const grid_t g)
+{
+    int PL = 3 * g.nmat + 10;
+    memset(s->bx_phi, 0, sizeof(double) * PL * g.NBX);
+    memset(s->by_phi, 0, sizeof(double) * PL * g.NBY);
+}
+
+/* advect_flux_default, src/advect.c:26-302 (porosity model off) */
+static void advect_flux_default(oracle_state *s, const salec_config *cfg, const grid_t g)
+{
+    int n = g.nmat, PL = 3 * n + 10;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        const double *ph[4] = {s->bx_phi + BXI(k, j), s->by_phi + BYI(k, j), s->bx_phi + BXI(k + 1, j), s->by_phi + BYI(k, j + 1)};
+        long st[4] = {g.NBX, g.NBY, g.NBX, g.NBY};
+        double dir[4] = {1.0, 1.0, -1.0, -1.0};
+        double dphi[64], sphi[64];
+        for (int q = 0; q < PL; ++q) { dphi[q] = 0.0; sphi[q] = 0.0; }
+        double dm = 0.;
+        for (int i = 0; i < 4; ++i) {
+#define PH(q) ph[i][(long)(q) * st[i]]
+            for (int m = 0; m < n; ++m) {
+                double mv = dir[i] * PH(m);
+                dphi[m] += mv;
+                dphi[m + n] += mv * PH(m + n);
+                dphi[m + 2 * n] += mv * PH(m + 2 * n);
+                dm += mv * PH(m + n);
+                double pa = mv * PH(m + n);
+                for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * PH(3 * n + q);
+            }
+#undef PH
+        }
+        (void)dm;
+        double evol = CF(E_VOL, 0, c);
+        double emass = evol * CF(E_DEN, 0, c);
+        for (int m = 0; m < n; ++m) {
+            sphi[m] += CF(M_VOF, m, c) * evol;
+            sphi[m + n] += CF(M_VOF, m, c) * evol * CF(M_DEN, m, c);
+            sphi[m + 2 * n] += CF(M_VOF, m, c) * evol * CF(M_ENG, m, c);
+        }
+        for (int q = 0; q < 4; ++q) sphi[3 * n + 2 + q] += emass * CF(E_STE, q, c);
+        sphi[3 * n + 6] = emass * CF(E_DAM, 0, c);
+        sphi[3 * n + 7] = emass * CF(E_TPS, 0, c);
+        sphi[3 * n + 8] = emass * CF(E_VIB, 0, c);
+        sphi[3 * n + 9] = emass * CF(E_EJT, 0, c);
+        for (int q = 0; q < PL; ++q) sphi[q] += 1.0 * dphi[q];
+        double Z = 0.0;
+        emass = 0.;
+        for (int m = 0; m < n; ++m) {
+            if (sphi[m] < 0.) { sphi[m] = 0.; sphi[m + n] = 0.; }
+            if (VACUUM == m) continue;
+            Z += sphi[m];
+            emass += sphi[m + n];
+        }
+        double ev[SALEC_MAXMAT];
+        for (int m = 0; m < n; ++m) ev[m] = CF(M_VOF, m, c);
+        if (Z > evol * (1.0 - TOLVOF) || (Z > 0. && sphi[0] < evol * TOLVOF)) {
+            double px = 1.0 / Z;
+            for (int m = 0; m < n; ++m) ev[m] = px * sphi[m];
+            ev[0] = 0.0;
+        } else if (Z < evol * TOLVOF) {
+            for (int m = 0; m < n; ++m) ev[m] = 0.0;
+            ev[0] = 1.0;
+        } else {
+            if (cfg->partpres) {
+                double px = 1.0 / (Z + sphi[0]);
+                for (int m = 0; m < n; ++m) ev[m] = px * sphi[m];
+            } else {
+                sphi[0] = evol - Z;
+                double px = 1.0 / evol;
+                for (int m = 0; m < n; ++m) ev[m] = px * sphi[m];
+            }
+        }
+        double md[SALEC_MAXMAT], me[SALEC_MAXMAT];
+        for (int m = 0; m < n; ++m) { md[m] = CF(M_DEN, m, c); me[m] = CF(M_ENG, m, c); }
+        for (int m = 1; m < n; ++m) {
+            if (ev[m] < TOLVOF || sphi[m + n] < TOLRHO * TOLVOF * evol) {
+                me[m] = 0.0; md[m] = 0.0; ev[m] = 0.0;
+            } else {
+                me[m] = sphi[m + 2 * n] / (ev[m] * evol);
+                md[m] = sphi[m + n] / (ev[m] * evol);
+            }
+        }
+        double dv[2], ste[4], dam, tps, vib, ejt;
+        if (emass > TOLRHO * evol) {
+            double px = 1.0 / emass;
+            dv[0] = px * sphi[3 * n]; dv[1] = px * sphi[3 * n + 1];
+            for (int q = 0; q < 4; ++q) ste[q] = px * sphi[3 * n + 2 + q];
+            dam = sphi[3 * n + 6] / emass; tps = sphi[3 * n + 7] / emass;
+            vib = sphi[3 * n + 8] / emass; ejt = sphi[3 * n + 9] / emass;
+        } else {
+            dv[0] = dv[1] = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
+            dam = tps = vib = ejt = 0.;
+        }
+        double eden = emass / evol;
+        tps = Wind(tps, 0.0, 10.0);
+        dam = Wind(dam, 0.0, 1.0);
+        ejt = Wind(ejt, 0.0, 1.0);
+        if (eden < cfg->density_cutoff || eden < 10.0 * cfg->density_cutoff * (1.0 - ev[VACUUM])) {
+            for (int m = 0; m < n; ++m) { ev[m] = 0.0; md[m] = 0.0; me[m] = 0.0; }
+            ev[VACUUM] = 1.0;
+            dv[0] = dv[1] = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
+            dam = tps = vib = ejt = 0.;
+        }
+        for (int m = 0; m < n; ++m) { CF(M_VOF, m, c) = ev[m]; CF(M_DEN, m, c) = md[m]; CF(M_ENG, m, c) = me[m]; }
+        CF(E_DVEL, 0, c) = dv[0]; CF(E_DVEL, 1, c) = dv[1];
+        for (int q = 0; q < 4; ++q) CF(E_STE, q, c) = ste[q];
+        CF(E_DAM, 0, c) = dam; CF(E_TPS, 0, c) = tps; CF(E_VIB, 0, c) = vib; CF(E_EJT, 0, c) = ejt;
+        CF(E_DEN, 0, c) = eden;
+    }
+}
+
+/* damp_on_cell_pml2, src/velocity.c:487-588 */
+static void damp_on_cell_pml2(oracle_state *s, const salec_config *cfg, const grid_t g, double dt, double t)
+{
+    if (cfg->damp_time <= t) return;
+    int n = g.nmat;
+    double adx = cfg->alpha_damp_x, ady = cfg->alpha_damp_y;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        long c = CI(k, j);
+        if (CF(M_VOF, 0, c) > TOLVOF) continue;
+        double sx = CF(E_SIGMA, 0, c), sy = CF(E_SIGMA, 1, c);
+        if (sy > 0.0 || sx > 0.0) {
+            CF(E_DVEL, 0, c) += -1.0 * (sx + sy) * CF(E_VEL, 0, c) * dt;
+            CF(E_DVEL, 1, c) += -1.0 * (sx + sy) * CF(E_VEL, 1, c) * dt;
+            double g0 = CF(E_GRAD_VEL, 0, c), g1 = CF(E_GRAD_VEL, 1, c), g2 = CF(E_GRAD_VEL, 2, c);
+            double g3 = CF(E_GRAD_VEL, 3, c), g4 = CF(E_GRAD_VEL, 4, c);
+            if (CF(M_VOF, 0, c) > TOLVOF) {
+                CF(E_ALPHA, 0, c) += -1.0 * (adx + ady) * CF(E_ALPHA, 0, c) * dt;
+                CF(E_ALPHA, 1, c) += -1.0 * (adx + ady) * CF(E_ALPHA, 1, c) * dt;
+            } else {
+                double a0 = CF(E_ALPHA, 0, c);
+                CF(E_ALPHA, 0, c) += (sy * g0 + sy * g4 - ady * a0) * dt + (sx * g3 + sx * g4 - adx * a0) * dt;
+                double lp = CF(E_PRE, 0, c) + CF(E_Q, 0, c) - CF(E_DAMPREF, 2, c);
+                double sd = (CF(E_STE, 0, c) - lp) * sy * g0 + CF(E_STE, 1, c) * (sy * g1 + sx * g2)
+                          + (CF(E_STE, 2, c) - lp) * sx * g3 + (sy + sx) * CF(E_STE, 3, c) * g4;
+                double a1 = CF(E_ALPHA, 1, c);
+                CF(E_ALPHA, 1, c) += -1.0 * sd * dt - 1.0 * (adx + ady) * a1 * dt;
+            }
+            for (int m = 1; m < n; ++m) {
+                if (CF(M_VOF, m, c) < 1.0 - TOLVOF) continue;
+                double dd = (-(sx + sy) * (CF(M_DEN, m, c) - CF(E_DAMPREF, 0, c)) - CF(E_DAMPREF, 0, c) * CF(E_ALPHA, 0, c)) * dt;
+                double de = (-(sx + sy) * (CF(M_ENG, m, c) - CF(E_DAMPREF, 1, c)) - CF(E_ALPHA, 1, c)) * dt;
+                CF(M_DEN, m, c) += dd / CF(M_PTY, m, c);
+                CF(M_ENG, m, c) += de / CF(M_PTY, m, c);
+                break;
+            }
+        }
+    }
+}
+
+/* damp_on_cell_pml, src/velocity.c:378-483 */
+static void damp_on_cell_pml(oracle_state *s, const salec_config *cfg, const grid_t g, double dt, double t)
+{
+    if (cfg->damp_time <= t) return;
+    int n = g.nmat;
+    double adx = cfg->alpha_damp_x, ady = cfg->alpha_damp_y;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        long c = CI(k, j);
+        if (CF(M_VOF, 0, c) > TOLVOF) continue;
+        double sx = CF(E_SIGMA, 0, c), sy = CF(E_SIGMA, 1, c);
+        if (sy > 0.0) {
+            CF(E_ALPHA, 1, c) += (sy * CF(E_GRAD_VEL, 0, c) + sy * CF(E_GRAD_VEL, 4, c) - ady * CF(E_ALPHA, 1, c)) * dt;
+            CF(E_DVEL, 1, c) += -1.0 * sy * CF(E_VEL, 1, c) * dt;
+            for (int m = 1; m < n; ++m) {
+                if (CF(M_VOF, m, c) < 1.0 - TOLVOF) continue;
+                double dd = (-sy * (CF(M_DEN, m, c) - CF(E_DAMPREF, 0, c)) - CF(E_DAMPREF, 0, c) * CF(E_ALPHA, 1, c)) * dt;
+                double de = (-sy * (CF(M_ENG, m, c) - CF(E_DAMPREF, 1, c)) - CF(E_PRE, 0, c) * CF(E_ALPHA, 1, c)) * dt;
+                CF(M_DEN, m, c) += dd / CF(M_PTY, m, c);
+                CF(M_ENG, m, c) += de / CF(M_PTY, m, c);
+                break;
+            }
+        }
+        if (sx > 0.0) {
+            if (CF(M_VOF, 0, c) > TOLVOF) {
+                CF(E_ALPHA, 0, c) += -1.0 * adx * CF(E_ALPHA, 0, c) * dt;
+                CF(E_DVEL, 0, c) += -1.0 * sx * CF(E_VEL, 0, c) * dt;
+            } else {
+                CF(E_ALPHA, 0, c) += (sx * CF(E_GRAD_VEL, 3, c) + sx * CF(E_GRAD_VEL, 4, c) - adx * CF(E_ALPHA, 0, c)) * dt;
+                CF(E_DVEL, 0, c) += -1.0 * sx * CF(E_VEL, 0, c) * dt;
+            }
+            for (int m = 1; m < n; ++m) {
+                if (CF(M_VOF, m, c) > 1.0 - TOLVOF) {
+                    double dd = (-sx * (CF(M_DEN, m, c) - CF(E_DAMPREF, 0, c)) - CF(E_DAMPREF, 0, c) * (CF(E_ALPHA, 0, c) + CF(E_ALPHA, 1, c))) * dt;
+                    double de = (-sx * (CF(M_ENG, m, c) - CF(E_DAMPREF, 1, c)) - CF(E_PRE, 0, c) * (CF(E_ALPHA, 0, c) + CF(E_ALPHA, 1, c))) * dt;
+                    CF(M_DEN, m, c) += dd / CF(M_PTY, m, c);
+                    CF(M_ENG, m, c) += de / CF(M_PTY, m, c);
+                    break;
+                }
+            }
+        }
+    }
+}
+
+/* damp_on_cell_sbc, src/velocity.c:688-767 */
+static void damp_on_cell_sbc(oracle_state *s, const salec_config *cfg, const grid_t g, double t)
+{
+    if (cfg->damp_time <= t) return;
+    int n = g.nmat;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        long c = CI(k, j);
+        if (CF(M_VOF, 0, c) > TOLVOF) continue;
+        double sx = CF(E_SIGMA, 0, c), sy = CF(E_SIGMA, 1, c);
+        if (sy < 1.0) {
+            for (int m = 1; m < n; ++m) {
+                if (CF(M_VOF, m, c) < TOLVOF) continue;
+                double dd = CF(M_DEN, m, c) - CF(E_DAMPREF, 0, c), de = CF(M_ENG, m, c) - CF(E_DAMPREF, 1, c);
+                CF(M_DEN, m, c) += dd * sy + CF(E_DAMPREF, 0, c);
+                CF(M_ENG, m, c) += de * sy + CF(E_DAMPREF, 1, c);
+            }
+        }
+        if (sx < 1.0) {
+            for (int m = 1; m < n; ++m) {
+                if (CF(M_VOF, m, c) < TOLVOF) continue;
+                double dd = CF(M_DEN, m, c) - CF(E_DAMPREF, 0, c), de = CF(M_ENG, m, c) - CF(E_DAMPREF, 1, c);
+                CF(M_DEN, m, c) = dd * sx + CF(E_DAMPREF, 0, c);
+                CF(M_ENG, m, c) = de * sx * sx + CF(E_DAMPREF, 1, c);
+            }
+        }
+    }
+}
+
+/* update_state, src/sale2d.c:1635-1742 */
+void oracle_update_state(oracle_state *s, const salec_config *cfg)
+{
+    grid_t g = mkgrid(s);
+    int n = g.nmat;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        double mp[SALEC_MAXMAT] = {0.}, mc[SALEC_MAXMAT] = {0.}, mt[SALEC_MAXMAT] = {0.};
+        double dam = CF(E_DAM, 0, c);
+        for (int m = 1; m < n; ++m) {
+            if (CF(M_VOF, m, c) < TOLVOF) { mp[m] = 0.; mc[m] = 1.; mt[m] = 0.; }
+            else {
+                const salec_material *r = cfg->mat + m;
+                double pmin = (1.0 - dam) * r->minPint + dam * r->minPdam;
+                double pty = CF(M_PTY, m, c);
+                calculate_state_correct(r, CF(M_DEN, m, c), CF(M_ENG, m, c), pmin, mt + m, mp + m, mc + m, &pty);
+            }
+        }
+        double pre = 0.0, csd = 0.0, tem = 0.0, den = 0.0, mu = 0.0;
+        for (int m = 1; m < n; ++m) {
+            double vf = CF(M_VOF, m, c), md = CF(M_DEN, m, c);
+            pre += vf * mp[m];
+            csd += vf / Max(1.0, mc[m]);
+            tem += vf * mt[m];
+            den += vf * md;
+            mu += vf / Max(1.0, md * mc[m] * mc[m] * cfg->mat[m].GaCoef);
+        }
+        if (mu > 0.) mu = 1.0 / mu;
+        if (csd > 0.) csd = 1.0 / csd;
+        double v0 = CF(M_VOF, 0, c);
+        if ((cfg->partpres == 0 && v0 > TOLVOF) || (cfg->partpres == 1 && v0 > 1.0 - TOLVOF)) {
+            pre = 0.; csd = 1.0; mu = 0.;
+        } else if (cfg->partpres == 1) {
+            csd = csd * (1.0 - v0); tem = tem / (1.0 - v0); mu = mu * (1.0 - v0);
+        }
+        CF(E_PRE, 0, c) = pre; CF(E_CSD, 0, c) = csd; CF(E_TEM, 0, c) = tem; CF(E_DEN, 0, c) = den; CF(E_MU, 0, c) = mu;
+    }
+}
+
+/* trace_profile, src/ejecta.c:8-63 */
+static void trace_profile(oracle_state *s, const salec_config *cfg, const grid_t g)
+{
+    double *yp = s->f[SALEC_Y_PROFILE];
+    for (int k = 0; k < g.CW; ++k) yp[k] = cfg->threshold_z * 2.0;
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        long c = CI(k, j);
+        if (CF(M_VOF, 0, c) > 0.1) yp[k] = MINM(CF(E_POS, 1, c), yp[k]);
+        if (CF(E_POS, 1, c) > cfg->threshold_z && CF(M_VOF, 0, c) < TOLVOF) CF(E_EJT, 0, c) = 1.0;
+    }
+}
+
+/* ejecta_flow, src/ejecta.c:65-134 */
+static void ejecta_flow(oracle_state *s, const salec_config *cfg, const grid_t g, double dt)
+{
+    if (cfg->flow_resistance <= 0. && cfg->flow_friction <= 0.) return;
+    double *yp = s->f[SALEC_Y_PROFILE];
+    for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
+        if (!cell_physical(&g, k, j)) continue;
+        long c = CI(k, j);
+        long v0 = VI(k - 1, j - 1), v1 = VI(k, j - 1);
+        double gy = 0.;
+        gy += 0.5 * VF(V_BF, 1, v0);
+        gy += 0.5 * VF(V_BF, 1, v1);
+        if (CF(E_POS, 1, c) <= yp[k] && CF(E_EJT, 0, c) > 0.1) {
+            if (cfg->flow_resistance > 0.) {
+                double pa = -cfg->flow_resistance * dt;
+                CF(E_DVEL, 0, c) += pa * CF(E_VEL, 0, c);
+                CF(E_DVEL, 1, c) += pa * CF(E_VEL, 1, c);
+            }
+            if (cfg->flow_friction > 0.) {
+                for (int q = 0; q < 4; ++q) CF(E_STE, q, c) = 0.0;
+                if (CF(E_VEL, 0, c) >= 0.) CF(E_DVEL, 0, c) += -1.0 * cfg->flow_friction * fabs(gy) * dt;
+                else CF(E_DVEL, 0, c) += 1.0 * cfg->flow_friction * fabs(gy) * dt;
+            }
+        }
+    }
+}
+
+/* fl_update_node, src/tracer.c:52-175 (single domain: leaving the physical
+ * cells deletes the tracer) */
+static void tracer_update(oracle_state *s, const salec_config *cfg, const grid_t g, double dt)
+{
+    salec_tracers *t = &s->tr;
+    const double *vx = s->tr_vx, *vy = s->tr_vy;
+    for (long i = 0; i < t->n; ++i) {
+        if (!t->alive[i]) continue;
+        int k = t->idx[i], j = t->idy[i];
+        long vv[4] = {VI(k, j), VI(k + 1, j), VI(k + 1, j + 1), VI(k, j + 1)};
+        long c = CI(k, j);
+        double pos[2] = {t->pos[i], t->pos[t->n + i]};
+        double vel[2] = {t->vel[i], t->vel[t->n + i]};
+        double rel[2];
+        rel[0] = 1.0 * pos[0] + -1.0 * VF(V_POS, 0, vv[0]);
+        rel[1] = 1.0 * pos[1] + -1.0 * VF(V_POS, 1, vv[0]);
+        rel[0] = rel[0] / (vx[k + 1] - vx[k]) * 2.0 - 1.0;
+        rel[1] = rel[1] / (vy[j + 1] - vy[j]) * 2.0 - 1.0;
+        if (CF(M_VOF, 0, c) > TOLVOF) {
+            double a0[4], a1[4];
+            for (int q = 0; q < 4; ++q) { a0[q] = VF(V_ACC, 0, vv[q]); a1[q] = VF(V_ACC, 1, vv[q]); }
+            double bf[2] = {interp2d(a0, rel), interp2d(a1, rel)};
+            double pb = 0.5 * dt * dt;
+            double dis[2] = {dt * vel[0] + pb * bf[0], dt * vel[1] + pb * bf[1]};
+            vel[0] += dt * bf[0]; vel[1] += dt * bf[1];
+            pos[0] += 1.0 * dis[0]; pos[1] += 1.0 * dis[1];
+        } else {
+            double d0[4], d1[4], u0[4], u1[4];
+            for (int q = 0; q < 4; ++q) {
+                d0[q] = VF(V_DIS, 0, vv[q]); d1[q] = VF(V_DIS, 1, vv[q]);
+                u0[q] = VF(V_VEL, 0, vv[q]); u1[q] = VF(V_VEL, 1, vv[q]);
+            }
+            double dis[2] = {interp2d(d0, rel), interp2d(d1, rel)};
+            double tv[2] = {interp2d(u0, rel), interp2d(u1, rel)};
+            pos[0] += 1.0 * dis[0]; pos[1] += 1.0 * dis[1];
+            vel[0] = 1.0 * tv[0]; vel[1] = 1.0 * tv[1];
+        }
+        if (t->matid[i] > 0 && pos[1] > cfg->threshold_z) t->matid[i] = -t->matid[i];
+        double ep = CF(E_PRE, 0, c), et = CF(E_TEM, 0, c);
+        t->mpre[i] = (float)MAXM((double)t->mpre[i], ep);
+        t->mtem[i] = (float)MAXM((double)t->mtem[i], et);
+#define OFF2(tt, a, b) (((tt) < (a)) ? -1 : ((tt) < (b) && (tt) >= (a)) ? 0 : 1)
+        int xo = OFF2(pos[0], vx[k], vx[k + 1]);
+        int yo = OFF2(pos[1], vy[j], vy[j + 1]);
+#undef OFF2
+        k += xo; j += yo;
+        t->pos[i] = pos[0]; t->pos[t->n + i] = pos[1];
+        t->vel[i] = vel[0]; t->vel[t->n + i] = vel[1];
+        t->idx[i] = k; t->idy[i] = j;
+        if (k < 2 || k > g.CW - 3 || j < 2 || j > g.CH - 3) t->alive[i] = 0;
+    }
+}
+
+/* --------------------------------------------------------------- driver -- */
+int oracle_attach(oracle_state *s, const salec_config *cfg)
+{
+    grid_t g = mkgrid(s);
+    int PL = 3 * cfg->nmat + 10;
+    s->bx_phi = (double *)calloc((size_t)PL * g.NBX, sizeof(double));
+    s->by_phi = (double *)calloc((size_t)PL * g.NBY, sizeof(double));
+    s->tr_vx = (double *)malloc(sizeof(double) * g.VW);
+    s->tr_vy = (double *)malloc(sizeof(double) * g.VH);
+    if (!s->bx_phi || !s->by_phi || !s->tr_vx || !s->tr_vy) return -1;
+    /* fl_mpi_init, src/tracer.c:347-361 */
+    for (int k = 0; k < g.VW; ++k) s->tr_vx[k] = s->f[SALEC_V_POS][VI(k, 1)];
+    for (int j = 0; j < g.VH; ++j) s->tr_vy[j] = s->f[SALEC_V_POS][g.NV + VI(1, j)];
+    return 0;
+}
+
+void oracle_detach(oracle_state *s)
+{
+    free(s->bx_phi); free(s->by_phi); free(s->tr_vx); free(s->tr_vy);
+    s->bx_phi = s->by_phi = s->tr_vx = s->tr_vy = NULL;
+}
+
+/* src/salec2d.c:76-94 with record_avg_value (src/sale2d.c:1562-1627) and
+ * init_v_field (src/sale2d.c:1897-1956) */
+double oracle_init(oracle_state *s, const salec_config *cfg, double *damp_time_out)
+{
+    grid_t g = mkgrid(s);
+    oracle_update_state(s, cfg);
+    oracle_ghost_fill(s, cfg, SALEC_E_DEN, O_GHT_DENSITY);
+    oracle_ghost_fill(s, cfg, SALEC_E_PRE, O_GHT_ALWAYS_COPY);
+    oracle_ghost_fill(s, cfg, SALEC_E_STE, O_GHT_STRESS);
+    oracle_ghost_fill(s, cfg, SALEC_M_ENG, O_GHT_DENSITY);
+    oracle_ghost_fill(s, cfg, SALEC_M_PTY, O_GHT_ALWAYS_COPY);
+    oracle_ghost_fill(s, cfg, SALEC_E_CVS, O_GHT_ALWAYS_COPY);
+    double ldt = 0.;
+    for (int j = 0; j < g.CH; j++) for (int k = 0; k < g.CW; ++k) {
+        long c = CI(k, j);
+        CF(E_DAMPREF, 0, c) = CF(E_DEN, 0, c);
+        double ep[2] = {CF(E_POS, 0, c), CF(E_POS, 1, c)};
+        double ld = vlen(ep, 2);
+        if ((CF(E_SIGMA, 0, c) > 0. || CF(E_SIGMA, 1, c) > 0.) && CF(E_CSD, 0, c) > 2.0)
+            ldt = MAXM(ldt, ld / CF(E_CSD, 0, c));
+        double e1 = 0.0;
+        for (int m = 1; m < g.nmat; ++m) e1 += CF(M_VOF, m, c) * CF(M_ENG, m, c);
+        CF(E_DAMPREF, 1, c) = e1;
+        CF(E_DAMPREF, 2, c) = CF(E_PRE, 0, c);
+    }
+    double damp_t = ldt * 2.0;
+    double dtime = Max(damp_t, cfg->damp_time);
+    if (damp_time_out) *damp_time_out = dtime;
+    /* init_v_field */
+    for (int k = 0; k < g.VW; ++k) for (int j = 0; j < g.VH; ++j) {
+        if (!vert_active(&g, k, j)) continue;
+        long v = VI(k, j);
+        long e[4] = {CI(k - 1, j - 1), CI(k, j - 1), CI(k, j), CI(k - 1, j)};
+        double vel[2] = {0.0, 0.0}, em = 0., evv = 0.;
+        for (int i = 0; i < 4; i++) {
+            double lv = VF(V_VOL, i + 1, v), d = CF(E_DEN, 0, e[i]);
+            double pa = d * lv;
+            vel[0] += pa * CF(E_VEL, 0, e[i]);
+            vel[1] += pa * CF(E_VEL, 1, e[i]);
+            em += d * lv;
+            evv += lv;
+        }
+        if (em > TOLRHO * evv) {
+            double px = 1.0 / em;
+            vel[0] *= px; vel[1] *= px;
+            VF(V_MASS, 0, v) = em; VF(V_MASS, 1, v) = evv;
+        } else {
+            vel[0] = vel[1] = 0.0;
+            VF(V_MASS, 0, v) = 0.; VF(V_MASS, 1, v) = 0.;
+        }
+        VF(V_VEL, 0, v) = vel[0]; VF(V_VEL, 1, v) = vel[1];
+    }
+    memset(s->f[SALEC_E_VEL], 0, sizeof(double) * 2 * g.NC);
+    memset(s->f[SALEC_E_STE], 0, sizeof(double) * 4 * g.NC);
+    memset(s->f[SALEC_E_DVEL], 0, sizeof(double) * 2 * g.NC);
+    return dtime;
+}
+
+/* loop body src/salec2d.c:115-272 */
+void oracle_step(oracle_state *s, const salec_config *cfg)
+{
+    grid_t g = mkgrid(s);
+    trace_profile(s, cfg, g);
+    update_v_vof(s, g, 1);
+    update_v_vel_inc(s, cfg, g, 1);
+    oracle_ghost_fill(s, cfg, SALEC_M_VOF, O_GHT_ALWAYS_COPY);
+    update_v_vof(s, g, 2);
+    oracle_ghost_fill(s, cfg, SALEC_E_DVEL, O_GHT_VELOCITY_INC);
+    oracle_ghost_fill(s, cfg, SALEC_E_DEN, O_GHT_DENSITY);
+    update_v_vel_inc(s, cfg, g, 2);
+    oracle_restrict(s, cfg, SALEC_V_VEL);
+    update_e_vel(s, g);
+    oracle_ghost_fill(s, cfg, SALEC_E_VEL, O_GHT_VELOCITY);
+    check_v_vel(s, g);
+    update_e_gradvel(s, cfg, g);
+    double ldt = Min(cfg->max_dt, calculate_dt(s, g, s->dt));
+    s->dt = ldt;
+    double dt = s->dt;
+    block_vibration(s, cfg, g, dt, s->t);
+    update_e_stress(s, cfg, g, dt);
+    int pmlacc = (cfg->damp_type == SALEC_DAMP_PML2 || cfg->damp_type == SALEC_DAMP_PML3);
+    if (pmlacc) calculate_acc_pml(s, cfg, g, dt, 1); else calculate_acc_default(s, cfg, g, 1);
+    oracle_ghost_fill(s, cfg, SALEC_E_PRE, O_GHT_ALWAYS_COPY);
+    oracle_ghost_fill(s, cfg, SALEC_E_STE, O_GHT_STRESS);
+    oracle_ghost_fill(s, cfg, SALEC_E_Q, O_GHT_ALWAYS_COPY);
+    if (pmlacc) calculate_acc_pml(s, cfg, g, dt, 2); else calculate_acc_default(s, cfg, g, 2);
+    oracle_restrict(s, cfg, SALEC_V_ACC);
+    calculate_v_dis(s, g, dt);
+    oracle_restrict(s, cfg, SALEC_V_DIS);
+    update_energy(s, cfg, g, dt);
+    tracer_update(s, cfg, g, dt);
+    clean_flux(s, g);
+    calculate_flux_mixed(s, g, 1);
+    oracle_ghost_fill(s, cfg, SALEC_M_DEN, O_GHT_DENSITY);
+    oracle_ghost_fill(s, cfg, SALEC_E_VIB, O_GHT_ALWAYS_COPY);
+    oracle_ghost_fill(s, cfg, SALEC_M_ENG, O_GHT_DENSITY);
+    calculate_flux_mixed(s, g, 0);
+    advect_flux_default(s, cfg, g);
+    switch (cfg->damp_type) {
+    case SALEC_DAMP_PML: damp_on_cell_pml(s, cfg, g, dt, s->t); break;
+    case SALEC_DAMP_PML2: damp_on_cell_pml2(s, cfg, g, dt, s->t); break;
+    case SALEC_DAMP_SBC: damp_on_cell_sbc(s, cfg, g, s->t); break;
+    default: break;
+    }
+    ejecta_flow(s, cfg, g, dt);
+    oracle_update_state(s, cfg);
+    s->t += s->dt;
+    s->step++;
+    (void)cell_receive;
+}

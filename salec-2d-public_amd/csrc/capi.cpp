@@ -1,0 +1,736 @@
+/*
+ * capi.cpp -- the extern "C" boundary (include/salec.h): context and device
+ * memory management, ghost/restriction programs and the step sequence that
+ * replaces the loop body src/salec2d.c:115-272.
+ */
+#include "salec_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using namespace salec;
+
+namespace {
+
+struct FieldMeta { const char *name; int grid; int nc; };
+#define SALEC_FIELD_META_(name, grid, nc) {#name, grid, nc},
+const FieldMeta kMeta[SALEC_NFIELDS] = { SALEC_FIELD_LIST(SALEC_FIELD_META_) };
+#undef SALEC_FIELD_META_
+
+/* ght_var_type, src/mpi_vec/mpi_vec_types.h */
+enum { VT_ALWAYS_COPY = 0, VT_NORMAL, VT_VELOCITY_INC, VT_VELOCITY, VT_STRESS, VT_DENSITY, VT_RESTRICT };
+
+struct Program {
+    GhostSlots slots{};
+    int nslots = 0;
+    GhostOp *d_ops = nullptr;
+    int nops = 0;
+    int two_phase = 0;
+    int maxc = 1;
+};
+
+enum {
+    PH_PROFILE = 0, PH_GF_A, PH_VERTEX, PH_RST_VEL, PH_E_VEL, PH_GF_EVEL, PH_CHECK_V, PH_GRADVEL_DT,
+    PH_STRESS, PH_GF_B, PH_ACC, PH_RST_ACC, PH_VDIS, PH_RST_DIS, PH_ENERGY, PH_TRACERS, PH_FLUX,
+    PH_GF_C, PH_FLUX_RING, PH_ADVECT, PH_DAMP, PH_EJECTA, PH_STATE, PH_TIME, PH_COUNT
+};
+const char *kPhaseNames[PH_COUNT] = {
+    "trace_profile", "ghost_vof_dvel_den", "vertex_vof_vel", "restrict_v_vel", "update_e_vel", "ghost_e_vel",
+    "check_v_vel", "gradvel_dt", "vibration_stress", "ghost_pre_ste_q", "calculate_acc", "restrict_v_acc",
+    "calculate_v_dis", "restrict_v_dis", "update_energy", "tracers", "flux", "ghost_den_vib_eng",
+    "flux_ghost_ring", "advect", "damp_on_cell", "ejecta_flow", "update_state", "advance_time"};
+
+} // namespace
+
+struct salec_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    salec_config cfg{};
+    Grid g{};
+    Params P{};
+    Fields F{};
+    DevMat *d_mats = nullptr;
+    std::vector<void *> owned;           /* device allocations to free */
+    double *bx = nullptr, *by = nullptr;
+    DevTime *d_time = nullptr;
+    double *d_part = nullptr;            /* dt / dampref partial minima   */
+    unsigned *d_counter = nullptr;
+    int gv_blocks = 0;
+    double *d_prof = nullptr;            /* trace_profile chunk minima    */
+    int prof_chunks = 0;
+    double *d_stage = nullptr;           /* two-phase ghost staging       */
+    long stage_len = 0;
+    Program gfA, gfE, gfB, gfC, rstV, rstA, rstD, gfInit, gfInit2;
+    Tracers T{};
+    long tr_cap = 0;
+    double *d_vx = nullptr, *d_vy = nullptr;
+    int initialized = 0;
+    int profiling = 0;
+    hipEvent_t ev[PH_COUNT + 1]{};
+    float phase_ms[PH_COUNT]{};
+    unsigned long long *d_nf = nullptr;
+};
+
+static int hip_ok(hipError_t e) { return e == hipSuccess ? SALEC_OK : SALEC_EDEVICE; }
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { std::fprintf(stderr, "salec: %s failed: %s\n", #x, hipGetErrorString(e_)); return SALEC_EDEVICE; } } while (0)
+
+template <class T>
+static int dalloc(salec_ctx *c, T **p, size_t count)
+{
+    void *q = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) return SALEC_ENOMEM;
+    if (hipMemset(q, 0, count * sizeof(T)) != hipSuccess) { hipFree(q); return SALEC_EDEVICE; }
+    c->owned.push_back(q);
+    *p = (T *)q;
+    return SALEC_OK;
+}
+
+extern "C" {
+
+int salec_abi_version(void) { return SALEC_ABI_VERSION; }
+
+const char *salec_strerror(int code)
+{
+    switch (code) {
+    case SALEC_OK: return "ok";
+    case SALEC_EINVAL: return "invalid argument or unsupported option";
+    case SALEC_ENOMEM: return "allocation failed";
+    case SALEC_EDEVICE: return "HIP runtime error or no device";
+    case SALEC_ENONFINITE: return "non-finite value in state";
+    case SALEC_EDTCOLLAPSE: return "time step collapsed";
+    case SALEC_ESTATE: return "call out of order";
+    case SALEC_ENOSYM: return "unknown model name";
+    default: return "unknown error";
+    }
+}
+
+long salec_grid_points(int grid, int nx, int ny)
+{
+    switch (grid) {
+    case SALEC_GRID_CELL: return (long)(nx + 4) * (ny + 4);
+    case SALEC_GRID_VERTEX: return (long)(nx + 5) * (ny + 5);
+    case SALEC_GRID_XFACE: return (long)(nx + 5) * (ny + 4);
+    case SALEC_GRID_YFACE: return (long)(nx + 4) * (ny + 5);
+    case SALEC_GRID_COLUMN: return (long)(nx + 4);
+    default: return 0;
+    }
+}
+
+int salec_field_info(int field, int nmat, int *grid, int *ncomp)
+{
+    if (field < 0 || field >= SALEC_NFIELDS) return SALEC_EINVAL;
+    if (grid) *grid = kMeta[field].grid;
+    if (ncomp) *ncomp = kMeta[field].nc < 0 ? nmat : kMeta[field].nc;
+    return SALEC_OK;
+}
+
+const char *salec_field_name(int field)
+{
+    if (field < 0 || field >= SALEC_NFIELDS) return nullptr;
+    return kMeta[field].name;
+}
+
+} // extern "C"
+
+/* --------------------------------------------------- ghost program build */
+namespace {
+
+struct Expr { int src; uint8_t m[3]; };
+
+/* compose(t, e)(x) = t(e(x)) for one component; modes GM_* */
+static uint8_t compose1(uint8_t t, uint8_t e)
+{
+    switch (t) {
+    case GM_COPY: return e;
+    case GM_ZERO: return GM_ZERO;
+    case GM_NZERO: return GM_NZERO;
+    case GM_NEG: {
+        static const uint8_t r[6] = {GM_NEG, GM_COPY, GM_NABS, GM_ABS, GM_NZERO, GM_ZERO};
+        return r[e];
+    }
+    case GM_ABS: {
+        static const uint8_t r[6] = {GM_ABS, GM_ABS, GM_ABS, GM_ABS, GM_ZERO, GM_ZERO};
+        return r[e];
+    }
+    default: { /* GM_NABS */
+        static const uint8_t r[6] = {GM_NABS, GM_NABS, GM_NABS, GM_NABS, GM_NZERO, GM_NZERO};
+        return r[e];
+    }
+    }
+}
+
+struct Replay {
+    std::unordered_map<int, Expr> ex;
+    std::vector<int> order;
+    Expr get(int L) const
+    {
+        auto it = ex.find(L);
+        if (it != ex.end()) return it->second;
+        Expr e; e.src = L; e.m[0] = e.m[1] = e.m[2] = GM_COPY;
+        return e;
+    }
+    /* dst <- t(src); t given per component class (c0, c1, rest) */
+    void write(int dst, int src, uint8_t t0, uint8_t t1, uint8_t tr)
+    {
+        Expr s = get(src), r;
+        r.src = s.src;
+        r.m[0] = compose1(t0, s.m[0]);
+        r.m[1] = compose1(t1, s.m[1]);
+        r.m[2] = compose1(tr, s.m[2]);
+        if (r.m[0] >= GM_ZERO && r.m[1] >= GM_ZERO && r.m[2] >= GM_ZERO) r.src = dst;
+        if (ex.find(dst) == ex.end()) order.push_back(dst);
+        ex[dst] = r;
+    }
+};
+
+/* local option list of set_ghost_pattern, src/mpi_vec/mpi_vec_ght.c:156-218 */
+static void local_opts(const int *go, int vt, int lo[4])
+{
+    switch (vt) {
+    case VT_ALWAYS_COPY: lo[0] = lo[1] = lo[2] = lo[3] = SALEC_GHT_OUTFLOW; break;
+    case VT_NORMAL: for (int k = 0; k < 4; ++k) lo[k] = go[k]; break;
+    case VT_VELOCITY_INC:
+        for (int k = 0; k < 4; ++k) lo[k] = go[k];
+        if (go[0] == SALEC_GHT_MIRROR) lo[0] = SALEC_GHT_REVERSE_VEL_X;
+        if (go[3] == SALEC_GHT_OUTFLOW) lo[3] = SALEC_GHT_REVERSE_VEL_ABSY;
+        break;
+    case VT_VELOCITY:
+        for (int k = 0; k < 4; ++k) lo[k] = SALEC_GHT_OUTFLOW_VERTEX;
+        if (go[0] == SALEC_GHT_MIRROR) lo[0] = SALEC_GHT_REVERSE_VEL_X;
+        break;
+    case VT_STRESS:
+        for (int k = 0; k < 4; ++k) lo[k] = SALEC_GHT_COPY_STE;
+        if (go[0] == SALEC_GHT_MIRROR) lo[0] = SALEC_GHT_REVERSE_STE_1;
+        break;
+    case VT_DENSITY:
+        lo[0] = lo[1] = lo[2] = SALEC_GHT_OUTFLOW; lo[3] = SALEC_GHT_CLEAN; break;
+    default: lo[0] = lo[1] = lo[2] = lo[3] = SALEC_GHT_OUTFLOW; break;
+    }
+}
+
+/* Replay the cell ghost table (init_field_ght_table + ghost_pad_fill,
+ * src/mpi_vec/mpi_vec_ght.c:12-356) in entry order. */
+static void replay_cells(Replay &R, int W, int H, const int *go, int vt)
+{
+    int lo[4];
+    local_opts(go, vt, lo);
+    const uint8_t C = GM_COPY, N = GM_NEG, A = GM_ABS, Z = GM_ZERO;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            bool L = x == 0, Rr = x == W - 1, B = y == 0, T = y == H - 1;
+            if (!(L || Rr || B || T)) continue;
+            int ax = x, ay = y, bx = x, by = y, cx = x, cy = y;
+            if (L) { ax = x + 1; bx = x + 2; cx = x + 3; }
+            if (Rr) { ax = x - 1; bx = x - 2; cx = x - 3; }
+            if (B) { ay = y + 1; by = y + 2; cy = y + 3; }
+            if (T) { ay = y - 1; by = y - 2; cy = y - 3; }
+            if (std::min(x, W - 1 - x) + std::min(y, H - 1 - y) <= 1) {
+                bx = std::min(std::max(bx, 2), W - 3); by = std::min(std::max(by, 2), H - 3);
+                cx = std::min(std::max(cx, 2), W - 3); cy = std::min(std::max(cy, 2), H - 3);
+            }
+            int opt = L ? lo[0] : (Rr ? lo[1] : (B ? lo[2] : lo[3]));
+            int gt = x + W * y, ra = ax + W * ay, rb = bx + W * by, rc = cx + W * cy;
+            switch (opt) {
+            case SALEC_GHT_OUTFLOW: case SALEC_GHT_COPY_STE:
+                R.write(gt, rc, C, C, C); R.write(ra, rb, C, C, C); break;
+            case SALEC_GHT_FIXED:
+                R.write(gt, rc, N, N, N); R.write(ra, rb, N, N, N); break;
+            case SALEC_GHT_ZERO:
+                R.write(gt, gt, Z, Z, Z); R.write(ra, ra, Z, Z, Z); break;
+            case SALEC_GHT_REVERSE_VEL_Y: case SALEC_GHT_REVERSE_STE_1:
+                R.write(gt, rc, C, N, C); R.write(ra, rb, C, N, C); break;
+            case SALEC_GHT_REVERSE_VEL_X:
+                R.write(gt, rc, N, C, C); R.write(ra, rb, N, C, C); break;
+            case SALEC_GHT_REVERSE_VEL_ABSY:
+                R.write(gt, rc, C, A, C); R.write(ra, rb, C, A, C); break;
+            case SALEC_GHT_OUTFLOW_VERTEX:
+                R.write(gt, rc, C, C, C); R.write(ra, rc, C, C, C); R.write(rb, rc, C, C, C); break;
+            case SALEC_GHT_CLEAN:
+                R.write(ra, ra, Z, Z, Z); R.write(rb, rb, Z, Z, Z); R.write(gt, gt, Z, Z, Z); break;
+            default: break;
+            }
+        }
+}
+
+/* Replay the vertex restriction table (init_field_rst_table,
+ * set_restrict_pattern, apply_restrict_vertex; src/mpi_vec/mpi_vec_ght.c:440-696). */
+static void replay_restrict(Replay &R, int W, int H, const int *go)
+{
+    int ro[4] = {SALEC_RST_NONE, SALEC_RST_NONE, SALEC_RST_NONE, SALEC_RST_NONE};
+    switch (go[1]) {
+    case SALEC_GHT_OUTFLOW: ro[1] = SALEC_RST_NONE; break;
+    case SALEC_GHT_FREESLIP: case SALEC_GHT_REVERSE_VEL_X: ro[1] = SALEC_RST_X_ZERO; break;
+    default: ro[1] = SALEC_RST_ZERO;
+    }
+    switch (go[2]) {
+    case SALEC_GHT_OUTFLOW: ro[2] = SALEC_RST_NONE; break;
+    case SALEC_GHT_FREESLIP: case SALEC_GHT_REVERSE_VEL_Y: ro[2] = SALEC_RST_Y_ZERO; break;
+    default: ro[2] = SALEC_RST_ZERO;
+    }
+    ro[0] = (go[0] == SALEC_GHT_REVERSE_VEL_X || go[0] == SALEC_GHT_MIRROR) ? SALEC_RST_X_ZERO : SALEC_RST_NONE;
+    const uint8_t C = GM_COPY, Z = GM_ZERO, A = GM_ABS;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            bool LP = x == 2, BP = y == 2, RP = x == W - 3, TP = y == H - 3;
+            if (!(LP || BP || RP || TP)) continue;
+            int ax = x, ay = y, bx = x, by = y;
+            if (LP) { ax = x + 1; bx = x - 1; }
+            if (RP) { ax = x - 1; bx = x + 1; }
+            if (BP) { ay = y + 1; by = y - 1; }
+            if (TP) { ay = y - 1; by = y + 1; }
+            int opt = SALEC_RST_NONE;
+            if (LP) opt = ro[0];
+            if (RP) opt = ro[1];
+            if (TP) opt = ro[3];
+            if (BP) opt = ro[2];
+            if (BP && RP && ro[2] != SALEC_RST_NONE && ro[1] != SALEC_RST_NONE) opt = SALEC_RST_ZERO;
+            int d = x + W * y, ra = ax + W * ay, rb = bx + W * by;
+            R.write(rb, ra, C, C, C);
+            switch (opt) {
+            case SALEC_RST_ZERO: R.write(d, d, Z, Z, Z); break;
+            case SALEC_RST_X_ZERO: R.write(d, ra, Z, C, C); break;
+            case SALEC_RST_Y_ZERO: R.write(d, ra, C, Z, C); break;
+            case SALEC_RST_Y_POSITIVE: R.write(d, ra, C, A, C); break;
+            case SALEC_RST_X_POSITIVE: R.write(d, ra, A, C, C); break;
+            default: R.write(d, ra, C, C, C); break;
+            }
+        }
+}
+
+struct SlotSpec { int field; int vt; };
+
+static int build_program(salec_ctx *c, Program &P, const std::vector<SlotSpec> &specs)
+{
+    std::vector<GhostOp> ops;
+    P.nslots = (int)specs.size();
+    P.two_phase = 0;
+    P.maxc = 1;
+    for (int s = 0; s < P.nslots; ++s) {
+        int f = specs[s].field, grid, nc;
+        salec_field_info(f, c->g.nmat, &grid, &nc);
+        int W = grid == SALEC_GRID_CELL ? c->g.CW : c->g.VW;
+        int H = grid == SALEC_GRID_CELL ? c->g.CH : c->g.VH;
+        P.slots.base[s] = c->F.f[f];
+        P.slots.plane[s] = grid == SALEC_GRID_CELL ? c->g.NC : c->g.NV;
+        P.slots.ncomp[s] = nc;
+        P.maxc = std::max(P.maxc, nc);
+        Replay R;
+        if (specs[s].vt == VT_RESTRICT) replay_restrict(R, W, H, c->cfg.go_list);
+        else replay_cells(R, W, H, c->cfg.go_list, specs[s].vt);
+        std::unordered_map<int, int> dsts;
+        size_t first = ops.size();
+        for (int L : R.order) {
+            Expr e = R.ex[L];
+            bool ident = (e.src == L && e.m[0] == GM_COPY && e.m[1] == GM_COPY && e.m[2] == GM_COPY);
+            if (ident) continue;
+            GhostOp o;
+            o.slot = s; o.dst = L; o.src = e.src; o.m0 = e.m[0]; o.m1 = e.m[1]; o.mr = e.m[2]; o.pad = 0;
+            ops.push_back(o);
+            dsts[L] = 1;
+        }
+        for (size_t i = first; i < ops.size(); ++i) {
+            const GhostOp &o = ops[i];
+            bool reads = !(o.m0 >= GM_ZERO && o.m1 >= GM_ZERO && o.mr >= GM_ZERO);
+            if (reads && o.src != o.dst && dsts.count(o.src)) P.two_phase = 1;
+        }
+    }
+    P.nops = (int)ops.size();
+    if (P.nops) {
+        int rc = dalloc(c, &P.d_ops, ops.size());
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(P.d_ops, ops.data(), sizeof(GhostOp) * ops.size(), hipMemcpyHostToDevice));
+        long need = (long)P.nops * P.maxc;
+        if (P.two_phase && need > c->stage_len) c->stage_len = need;
+    }
+    return SALEC_OK;
+}
+
+static void run_program(salec_ctx *c, const Program &P)
+{
+    launch_ghost(c->stream, P.slots, P.d_ops, P.nops, c->d_stage, P.two_phase, P.maxc);
+}
+
+static DevMat to_dev(const salec_material &m)
+{
+    DevMat d;
+    std::memset(&d, 0, sizeof d);
+    d.eos_type = m.eos_type; d.yfunc = m.yfunc; d.dfunc = m.dfunc; d.tfunc = m.tfunc;
+    d.till_nden = m.till_nden; d.tab_nden = m.tab_nden; d.tab_ntem = m.tab_ntem;
+    d.PolyLiq = m.PolyLiq; d.PolySol = m.PolySol;
+    d.TLRho0 = m.TLRho0; d.TLCv = m.TLCv; d.TLA = m.TLA; d.TLB = m.TLB; d.TLE0 = m.TLE0; d.TLa = m.TLa;
+    d.TLb = m.TLb; d.TLAlpha = m.TLAlpha; d.TLBeta = m.TLBeta; d.TLEiv = m.TLEiv; d.TLEcv = m.TLEcv; d.TLTref = m.TLTref;
+    d.till_stepden = m.till_stepden;
+    d.VaporDen = m.VaporDen; d.MeltDen = m.MeltDen; d.GaCoef = m.GaCoef;
+    d.Yint0 = m.Yint0; d.Yfricint = m.Yfricint; d.Ylimint = m.Ylimint; d.Ydam0 = m.Ydam0; d.Yfricdam = m.Yfricdam;
+    d.Ylimdam = m.Ylimdam; d.Yten0 = m.Yten0; d.IvanA = m.IvanA; d.IvanB = m.IvanB; d.IvanC = m.IvanC;
+    d.Pbd = m.Pbd; d.Pbp = m.Pbp; d.minPint = m.minPint; d.minPdam = m.minPdam;
+    d.JCA = m.JCA; d.JCB = m.JCB; d.JCN = m.JCN; d.JCC = m.JCC; d.JCM = m.JCM; d.JCTREF = m.JCTREF;
+    d.JCMaxPlastic = m.JCMaxPlastic; d.Asimon = m.Asimon; d.Csimon = m.Csimon; d.Tmelt0 = m.Tmelt0;
+    d.Tfrac = m.Tfrac; d.Tdelta = m.Tdelta; d.Tlidc1 = m.Tlidc1; d.Tlidc2 = m.Tlidc2; d.Tlidc3 = m.Tlidc3;
+    d.Tsolc1 = m.Tsolc1; d.Tsolc2 = m.Tsolc2; d.Tsolc3 = m.Tsolc3; d.Toff = m.Toff; d.Cvib = m.Cvib;
+    d.VibMax = m.VibMax; d.Tdamp = m.Tdamp; d.Pvlim = m.Pvlim; d.Acvis = m.Acvis; d.alpha0 = m.alpha0;
+    d.chi = m.chi; d.gamfrac = m.gamfrac; d.melt_viscosity = m.melt_viscosity;
+    return d;
+}
+
+} // namespace
+
+extern "C" {
+
+int salec_create(const salec_config *cfg, salec_ctx **out)
+{
+    if (!cfg || !out) return SALEC_EINVAL;
+    *out = nullptr;
+    if (cfg->nx < 4 || cfg->ny < 4 || cfg->nmat < 2 || cfg->nmat > SALEC_MAXMAT || !cfg->mat) return SALEC_EINVAL;
+    if (cfg->damp_type == SALEC_DAMP_PML3) return SALEC_EINVAL;   /* advect_flux_pml3 not provided */
+    for (int m = 1; m < cfg->nmat; ++m) {
+        const salec_material &mm = cfg->mat[m];
+        if (mm.pfunc != SALEC_FN_NONE) return SALEC_EINVAL;      /* porosity model not provided */
+        if (mm.eos_type == SALEC_EOS_TILLOTSON && (!mm.till_yeng || mm.till_nden < 2)) return SALEC_EINVAL;
+        if ((mm.eos_type == SALEC_EOS_ANEOS_SD || mm.eos_type == SALEC_EOS_ANEOS) &&
+            (!mm.tab_yden || !mm.tab_xtem || !mm.tab_data || mm.tab_nden < 2 || mm.tab_ntem < 2)) return SALEC_EINVAL;
+        if (mm.eos_type < 0 || mm.eos_type > SALEC_EOS_ANEOS_SD) return SALEC_EINVAL;
+        int fns[3] = {mm.yfunc, mm.dfunc, mm.tfunc};
+        for (int q = 0; q < 3; ++q) if (fns[q] < 0 || fns[q] >= SALEC_FN_COUNT) return SALEC_ENOSYM;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SALEC_EDEVICE;
+    salec_ctx *c = new salec_ctx();
+    hipGetDevice(&c->device);
+    c->cfg = *cfg;
+    c->cfg.mat = nullptr;
+    int rc = SALEC_OK;
+    Grid &g = c->g;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nmat = cfg->nmat; g.PL = 3 * cfg->nmat + 10;
+    g.CW = g.nx + 4; g.CH = g.ny + 4; g.VW = g.nx + 5; g.VH = g.ny + 5;
+    g.BXW = g.nx + 5; g.BXH = g.ny + 4; g.BYW = g.nx + 4; g.BYH = g.ny + 5;
+    g.NC = (long)g.CW * g.CH; g.NV = (long)g.VW * g.VH; g.NBX = (long)g.BXW * g.BXH; g.NBY = (long)g.BYW * g.BYH;
+    Params &P = c->P;
+    P.avis1 = cfg->avis1; P.avis2 = cfg->avis2; P.anc = cfg->anc; P.density_cutoff = cfg->density_cutoff;
+    P.velocity_max = cfg->velocity_max; P.velocity_min = cfg->velocity_min; P.acceleration_min = cfg->acceleration_min;
+    P.alpha_damp_x = cfg->alpha_damp_x; P.alpha_damp_y = cfg->alpha_damp_y; P.damp_time = cfg->damp_time;
+    P.threshold_z = cfg->threshold_z; P.flow_resistance = cfg->flow_resistance; P.flow_friction = cfg->flow_friction;
+    P.max_dt = cfg->max_dt; P.partpres = cfg->partpres; P.tensile_failure = cfg->tensile_failure;
+    P.damp_type = cfg->damp_type;
+    P.acoustic_fluid = 0;
+    for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].Toff > 0.0) P.acoustic_fluid = 1;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SALEC_EDEVICE; }
+#define TRY(x) do { rc = (x); if (rc != SALEC_OK) { salec_destroy(c); return rc; } } while (0)
+    for (int f = 0; f < SALEC_NFIELDS; ++f) {
+        int grid, nc;
+        salec_field_info(f, g.nmat, &grid, &nc);
+        TRY(dalloc(c, &c->F.f[f], (size_t)salec_grid_points(grid, g.nx, g.ny) * nc));
+    }
+    TRY(dalloc(c, &c->bx, (size_t)g.PL * g.NBX));
+    TRY(dalloc(c, &c->by, (size_t)g.PL * g.NBY));
+    TRY(dalloc(c, &c->d_time, 1));
+    c->gv_blocks = gradvel_dt_blocks(g);
+    int nb = std::max(c->gv_blocks, dampref_blocks(g));
+    TRY(dalloc(c, &c->d_part, (size_t)nb));
+    TRY(dalloc(c, &c->d_counter, 1));
+    c->prof_chunks = (g.CH + 63) / 64;
+    TRY(dalloc(c, &c->d_prof, (size_t)c->prof_chunks * g.CW));
+    TRY(dalloc(c, &c->d_nf, 1));
+    /* materials and their tables */
+    std::vector<DevMat> dm(SALEC_MAXMAT);
+    for (int m = 1; m < cfg->nmat; ++m) {
+        const salec_material &mm = cfg->mat[m];
+        dm[m] = to_dev(mm);
+        if (mm.eos_type == SALEC_EOS_TILLOTSON) {
+            double *t;
+            TRY(dalloc(c, &t, mm.till_nden));
+            TRY(hip_ok(hipMemcpy(t, mm.till_yeng, sizeof(double) * mm.till_nden, hipMemcpyHostToDevice)));
+            dm[m].till_yeng = t;
+        } else {
+            double *yd, *xt, *dd;
+            TRY(dalloc(c, &yd, mm.tab_nden));
+            TRY(dalloc(c, &xt, mm.tab_ntem));
+            TRY(dalloc(c, &dd, (size_t)mm.tab_nden * mm.tab_ntem * 3));
+            TRY(hip_ok(hipMemcpy(yd, mm.tab_yden, sizeof(double) * mm.tab_nden, hipMemcpyHostToDevice)));
+            TRY(hip_ok(hipMemcpy(xt, mm.tab_xtem, sizeof(double) * mm.tab_ntem, hipMemcpyHostToDevice)));
+            TRY(hip_ok(hipMemcpy(dd, mm.tab_data, sizeof(double) * mm.tab_nden * mm.tab_ntem * 3, hipMemcpyHostToDevice)));
+            dm[m].tab_yden = yd; dm[m].tab_xtem = xt; dm[m].tab_data = dd;
+        }
+    }
+    TRY(dalloc(c, &c->d_mats, SALEC_MAXMAT));
+    TRY(hip_ok(hipMemcpy(c->d_mats, dm.data(), sizeof(DevMat) * SALEC_MAXMAT, hipMemcpyHostToDevice)));
+    /* ghost programs at the fill points of the loop body */
+    TRY(build_program(c, c->gfA, {{SALEC_M_VOF, VT_ALWAYS_COPY}, {SALEC_E_DVEL, VT_VELOCITY_INC}, {SALEC_E_DEN, VT_DENSITY}}));
+    TRY(build_program(c, c->gfE, {{SALEC_E_VEL, VT_VELOCITY}}));
+    TRY(build_program(c, c->gfB, {{SALEC_E_PRE, VT_ALWAYS_COPY}, {SALEC_E_STE, VT_STRESS}, {SALEC_E_Q, VT_ALWAYS_COPY}}));
+    TRY(build_program(c, c->gfC, {{SALEC_M_DEN, VT_DENSITY}, {SALEC_E_VIB, VT_ALWAYS_COPY}, {SALEC_M_ENG, VT_DENSITY}}));
+    TRY(build_program(c, c->rstV, {{SALEC_V_VEL, VT_RESTRICT}}));
+    TRY(build_program(c, c->rstA, {{SALEC_V_ACC, VT_RESTRICT}}));
+    TRY(build_program(c, c->rstD, {{SALEC_V_DIS, VT_RESTRICT}}));
+    /* src/salec2d.c:78-84: e_den, e_pre, e_ste, m_eng, m_pty, e_cvs */
+    TRY(build_program(c, c->gfInit, {{SALEC_E_DEN, VT_DENSITY}, {SALEC_E_PRE, VT_ALWAYS_COPY}, {SALEC_E_STE, VT_STRESS}, {SALEC_M_ENG, VT_DENSITY}}));
+    TRY(build_program(c, c->gfInit2, {{SALEC_M_PTY, VT_ALWAYS_COPY}, {SALEC_E_CVS, VT_ALWAYS_COPY}}));
+    if (c->stage_len > 0) TRY(dalloc(c, &c->d_stage, (size_t)c->stage_len));
+    /* tracer storage */
+    c->tr_cap = std::max(0L, cfg->max_tracers);
+    if (c->tr_cap > 0) {
+        Tracers &T = c->T;
+        TRY(dalloc(c, &T.pos, 2 * c->tr_cap)); TRY(dalloc(c, &T.vel, 2 * c->tr_cap));
+        TRY(dalloc(c, &T.mpre, c->tr_cap)); TRY(dalloc(c, &T.mtem, c->tr_cap));
+        TRY(dalloc(c, &T.gx, c->tr_cap)); TRY(dalloc(c, &T.gy, c->tr_cap)); TRY(dalloc(c, &T.tag, c->tr_cap));
+        TRY(dalloc(c, &T.matid, c->tr_cap)); TRY(dalloc(c, &T.idx, c->tr_cap)); TRY(dalloc(c, &T.idy, c->tr_cap));
+        TRY(dalloc(c, &T.alive, c->tr_cap));
+        T.n = 0;
+    }
+    TRY(dalloc(c, &c->d_vx, g.VW));
+    TRY(dalloc(c, &c->d_vy, g.VH));
+    for (int k = 0; k <= PH_COUNT; ++k) TRY(hip_ok(hipEventCreate(&c->ev[k])));
+#undef TRY
+    *out = c;
+    return SALEC_OK;
+}
+
+void salec_destroy(salec_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : c->owned) hipFree(p);
+    for (int k = 0; k <= PH_COUNT; ++k) if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int salec_upload(salec_ctx *c, int field, const double *host)
+{
+    if (!c || !host) return SALEC_EINVAL;
+    int grid, nc;
+    if (salec_field_info(field, c->g.nmat, &grid, &nc)) return SALEC_EINVAL;
+    size_t n = (size_t)salec_grid_points(grid, c->g.nx, c->g.ny) * nc;
+    HIPCHK(hipMemcpyAsync(c->F.f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (field == SALEC_V_POS) {
+        /* vertex coordinate lines used by the tracers (src/tracer.c:347-361) */
+        std::vector<double> vx(c->g.VW), vy(c->g.VH);
+        for (int k = 0; k < c->g.VW; ++k) vx[k] = host[k + (long)c->g.VW * 1];
+        for (int j = 0; j < c->g.VH; ++j) vy[j] = host[c->g.NV + 1 + (long)c->g.VW * j];
+        HIPCHK(hipMemcpy(c->d_vx, vx.data(), sizeof(double) * c->g.VW, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_vy, vy.data(), sizeof(double) * c->g.VH, hipMemcpyHostToDevice));
+    }
+    return SALEC_OK;
+}
+
+int salec_download(salec_ctx *c, int field, double *host)
+{
+    if (!c || !host) return SALEC_EINVAL;
+    int grid, nc;
+    if (salec_field_info(field, c->g.nmat, &grid, &nc)) return SALEC_EINVAL;
+    size_t n = (size_t)salec_grid_points(grid, c->g.nx, c->g.ny) * nc;
+    HIPCHK(hipMemcpyAsync(host, c->F.f[field], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SALEC_OK;
+}
+
+int salec_upload_tracers(salec_ctx *c, const salec_tracers *tr)
+{
+    if (!c || !tr) return SALEC_EINVAL;
+    if (tr->n > c->tr_cap) return SALEC_EINVAL;
+    long n = tr->n;
+    Tracers &T = c->T;
+    T.n = n;
+    if (n == 0) return SALEC_OK;
+    hipStream_t s = c->stream;
+    /* pos/vel are [2][n]; device stride is n as well */
+    HIPCHK(hipMemcpyAsync(T.pos, tr->pos, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(T.vel, tr->vel, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(T.mpre, tr->mpre, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(T.mtem, tr->mtem, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    int *hs[7] = {tr->gx, tr->gy, tr->tag, tr->matid, tr->idx, tr->idy, tr->alive};
+    int *ds[7] = {T.gx, T.gy, T.tag, T.matid, T.idx, T.idy, T.alive};
+    for (int q = 0; q < 7; ++q) HIPCHK(hipMemcpyAsync(ds[q], hs[q], sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SALEC_OK;
+}
+
+int salec_download_tracers(salec_ctx *c, salec_tracers *tr)
+{
+    if (!c || !tr) return SALEC_EINVAL;
+    Tracers &T = c->T;
+    if (tr->n < T.n) return SALEC_EINVAL;
+    long n = T.n;
+    tr->n = n;
+    if (n == 0) return SALEC_OK;
+    hipStream_t s = c->stream;
+    HIPCHK(hipMemcpyAsync(tr->pos, T.pos, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(tr->vel, T.vel, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(tr->mpre, T.mpre, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(tr->mtem, T.mtem, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    int *hs[7] = {tr->gx, tr->gy, tr->tag, tr->matid, tr->idx, tr->idy, tr->alive};
+    int *ds[7] = {T.gx, T.gy, T.tag, T.matid, T.idx, T.idy, T.alive};
+    for (int q = 0; q < 7; ++q) HIPCHK(hipMemcpyAsync(hs[q], ds[q], sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SALEC_OK;
+}
+
+int salec_set_time(salec_ctx *c, double t, double dt)
+{
+    if (!c) return SALEC_EINVAL;
+    DevTime h{};
+    h.dt = dt; h.t = t; h.step = 0;
+    HIPCHK(hipMemcpyAsync(c->d_time, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SALEC_OK;
+}
+
+int salec_get_time(salec_ctx *c, double *t, double *dt, long *step)
+{
+    if (!c) return SALEC_EINVAL;
+    DevTime h{};
+    HIPCHK(hipMemcpyAsync(&h, c->d_time, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (t) *t = h.t;
+    if (dt) *dt = h.dt;
+    if (step) *step = (long)h.step;
+    return SALEC_OK;
+}
+
+/* setup tail of main(), src/salec2d.c:76-94 */
+int salec_init(salec_ctx *c, double *damp_time_io)
+{
+    if (!c) return SALEC_EINVAL;
+    hipStream_t s = c->stream;
+    launch_update_state(s, c->g, c->F, c->P, c->d_mats);
+    run_program(c, c->gfInit);
+    run_program(c, c->gfInit2);
+    int nb = dampref_blocks(c->g);
+    launch_dampref(s, c->g, c->F, c->d_part, nb);
+    std::vector<double> part(nb);
+    HIPCHK(hipMemcpyAsync(part.data(), c->d_part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double ldt = 0.;
+    for (double v : part) ldt = v > ldt ? v : ldt;
+    double damp_t = ldt * 2.0;
+    double in = damp_time_io ? *damp_time_io : c->cfg.damp_time;
+    double dtime = damp_t > in ? damp_t : in;
+    c->P.damp_time = dtime;
+    c->cfg.damp_time = dtime;
+    if (damp_time_io) *damp_time_io = dtime;
+    launch_init_v_field(s, c->g, c->F);
+    HIPCHK(hipMemsetAsync(c->F.f[SALEC_E_VEL], 0, sizeof(double) * 2 * c->g.NC, s));
+    HIPCHK(hipMemsetAsync(c->F.f[SALEC_E_STE], 0, sizeof(double) * 4 * c->g.NC, s));
+    HIPCHK(hipMemsetAsync(c->F.f[SALEC_E_DVEL], 0, sizeof(double) * 2 * c->g.NC, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    c->initialized = 1;
+    return SALEC_OK;
+}
+
+int salec_set_damp_time(salec_ctx *c, double damp_time)
+{
+    if (!c) return SALEC_EINVAL;
+    c->P.damp_time = damp_time;
+    c->cfg.damp_time = damp_time;
+    c->initialized = 1;
+    return SALEC_OK;
+}
+
+int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
+{
+    if (!c) return SALEC_EINVAL;
+    hipStream_t s = c->stream;
+    const Grid &g = c->g;
+    const Fields &F = c->F;
+    const Params &P = c->P;
+    bool prof = c->profiling != 0;
+#define MARK(ph) do { if (prof) hipEventRecord(c->ev[ph], s); } while (0)
+    MARK(PH_PROFILE);
+    launch_trace_profile(s, g, F, P, c->d_prof, c->prof_chunks);
+    launch_profile_finish(s, g, F, P, c->d_prof, c->prof_chunks);
+    MARK(PH_GF_A);      run_program(c, c->gfA);
+    MARK(PH_VERTEX);    launch_vertex_vof_vel(s, g, F, P);
+    MARK(PH_RST_VEL);   run_program(c, c->rstV);
+    MARK(PH_E_VEL);     launch_update_e_vel(s, g, F);
+    MARK(PH_GF_EVEL);   run_program(c, c->gfE);
+    MARK(PH_CHECK_V);   launch_check_v_vel(s, g, F);
+    MARK(PH_GRADVEL_DT); launch_gradvel_dt(s, g, F, P, c->d_time, c->d_part, c->d_counter, c->gv_blocks);
+    MARK(PH_STRESS);    launch_stress(s, g, F, P, c->d_mats, c->d_time);
+    MARK(PH_GF_B);      run_program(c, c->gfB);
+    MARK(PH_ACC);       launch_acc(s, g, F, P, c->d_time);
+    MARK(PH_RST_ACC);   run_program(c, c->rstA);
+    MARK(PH_VDIS);      launch_v_dis(s, g, F, c->d_time);
+    MARK(PH_RST_DIS);   run_program(c, c->rstD);
+    MARK(PH_ENERGY);    launch_energy(s, g, F, c->d_mats, c->d_time);
+    MARK(PH_TRACERS);   launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time);
+    MARK(PH_FLUX);      launch_flux(s, g, F, c->bx, c->by, 0);
+    MARK(PH_GF_C);      run_program(c, c->gfC);
+    MARK(PH_FLUX_RING); launch_flux(s, g, F, c->bx, c->by, 1);
+    MARK(PH_ADVECT);    launch_advect(s, g, F, P, c->bx, c->by);
+    MARK(PH_DAMP);      launch_damp(s, g, F, P, c->d_time);
+    MARK(PH_EJECTA);    launch_ejecta_flow(s, g, F, P, c->d_time);
+    MARK(PH_STATE);     launch_update_state(s, g, F, P, c->d_mats);
+    MARK(PH_TIME);      launch_advance_time(s, c->d_time);
+    if (prof) hipEventRecord(c->ev[PH_COUNT], s);
+#undef MARK
+    HIPCHK(hipGetLastError());
+    if (prof) {
+        HIPCHK(hipEventSynchronize(c->ev[PH_COUNT]));
+        for (int k = 0; k < PH_COUNT; ++k) hipEventElapsedTime(&c->phase_ms[k], c->ev[k], c->ev[k + 1]);
+    }
+    if (!(flags & SALEC_STEP_NOSYNC) || st) {
+        DevTime h{};
+        HIPCHK(hipMemcpyAsync(&h, c->d_time, sizeof h, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (st) { st->dt = h.dt; st->t = h.t; st->step = (long)h.step; }
+    }
+    return SALEC_OK;
+}
+
+int salec_sync(salec_ctx *c)
+{
+    if (!c) return SALEC_EINVAL;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SALEC_OK;
+}
+
+int salec_check(salec_ctx *c, int *field, long *index)
+{
+    if (!c) return SALEC_EINVAL;
+    static const int persistent[] = {SALEC_M_VOF, SALEC_M_DEN, SALEC_M_ENG, SALEC_E_STE, SALEC_E_DAM, SALEC_E_TPS,
+                                     SALEC_E_VIB, SALEC_E_DVEL, SALEC_E_DEN, SALEC_E_PRE, SALEC_E_CSD, SALEC_V_VEL};
+    for (int f : persistent) {
+        int grid, nc;
+        salec_field_info(f, c->g.nmat, &grid, &nc);
+        long n = salec_grid_points(grid, c->g.nx, c->g.ny) * nc;
+        unsigned long long init = ~0ull, res = 0;
+        HIPCHK(hipMemcpyAsync(c->d_nf, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
+        launch_nonfinite(c->stream, c->F.f[f], n, c->d_nf);
+        HIPCHK(hipMemcpyAsync(&res, c->d_nf, sizeof res, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (res != ~0ull) {
+            if (field) *field = f;
+            if (index) *index = (long)(res - 1);
+            return SALEC_ENONFINITE;
+        }
+    }
+    return SALEC_OK;
+}
+
+int salec_set_profiling(salec_ctx *c, int on)
+{
+    if (!c) return SALEC_EINVAL;
+    c->profiling = on;
+    return SALEC_OK;
+}
+
+int salec_phase_times(salec_ctx *c, int max_phases, const char **names, float *ms)
+{
+    if (!c) return SALEC_EINVAL;
+    int n = std::min(max_phases, (int)PH_COUNT);
+    for (int k = 0; k < n; ++k) {
+        if (names) names[k] = kPhaseNames[k];
+        if (ms) ms[k] = c->phase_ms[k];
+    }
+    return PH_COUNT;
+}
+
+} // extern "C"

@@ -1,0 +1,1277 @@
+/*
+ * kernels.hip -- HIP kernels of the per-timestep solver path for gfx950.
+ *
+ * Layout: structure-of-arrays FP64 planes on the reference's padded index
+ * space (include/salec.h).  Every kernel is gather-form: a thread owns one
+ * cell, vertex, ghost-table entry or tracer and writes only its own outputs,
+ * so results do not depend on scheduling.  The only scatter is the face
+ * packet written by the donor cell; each face has exactly one donor because
+ * the swept volume seen from its two cells is exactly antisymmetric
+ * (src/advect.c:1347-1354).
+ */
+#include "device_math.h"
+
+namespace salec {
+
+#define CPL(fld, c, i) (F.f[SALEC_##fld][(long)(c) * g.NC + (i)])
+#define VPL(fld, c, i) (F.f[SALEC_##fld][(long)(c) * g.NV + (i)])
+
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+/* ------------------------------------------------------ trace_profile -- */
+/* src/ejecta.c:8-63: per-column minimum of vacuum-bearing cell heights and
+ * the ejecta flag.  Pass 1: thread per (column, row chunk). */
+#define PROF_CHUNK 64
+__global__ void k_trace_profile(Grid g, Fields F, Params P, double *part, int nchunk)
+{
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    int ch = blockIdx.y;
+    if (col >= g.CW || ch >= nchunk) return;
+    double yp = P.threshold_z * 2.0;
+    int y0 = ch * PROF_CHUNK, y1 = min(g.CH, y0 + PROF_CHUNK);
+    for (int y = y0; y < y1; ++y) {
+        long c = col + (long)g.CW * y;
+        double v0 = CPL(M_VOF, 0, c), ey = CPL(E_POS, 1, c);
+        if (v0 > 0.1) yp = ey < yp ? ey : yp;
+        if (ey > P.threshold_z && v0 < TOLVOF) CPL(E_EJT, 0, c) = 1.0;
+    }
+    part[(long)ch * g.CW + col] = yp;
+}
+__global__ void k_profile_finish(Grid g, Fields F, Params P, const double *part, int nchunk)
+{
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= g.CW) return;
+    double yp = P.threshold_z * 2.0;
+    for (int ch = 0; ch < nchunk; ++ch) {
+        double v = part[(long)ch * g.CW + col];
+        yp = v < yp ? v : yp;
+    }
+    F.f[SALEC_Y_PROFILE][col] = yp;
+}
+void launch_trace_profile(hipStream_t s, const Grid &g, const Fields &F, const Params &P, double *part, int nchunk)
+{
+    dim3 grid(nblk(g.CW, 256), nchunk);
+    k_trace_profile<<<grid, 256, 0, s>>>(g, F, P, part, nchunk);
+}
+void launch_profile_finish(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *part, int nchunk)
+{
+    k_profile_finish<<<nblk(g.CW, 256), 256, 0, s>>>(g, F, P, part, nchunk);
+}
+
+/* ------------------------------------------------------------- ghosts -- */
+/* Ghost fills and vertex restrictions (src/mpi_vec/mpi_vec_ght.c:279-356,
+ * 657-696) are applied as precomputed programs: the host replays the
+ * reference's ordered table once and reduces it to "dst = mode(src at the
+ * start of the fill)" operations, which are independent of each other. */
+__device__ __forceinline__ double gmode(double x, int m)
+{
+    switch (m) {
+    case GM_COPY: return x;
+    case GM_NEG: return -1.0 * x;
+    case GM_ABS: return fabs(x);
+    case GM_NABS: return -1.0 * fabs(x);
+    case GM_ZERO: return 0.0;
+    default: return -0.0;
+    }
+}
+__global__ void k_ghost_direct(GhostSlots sl, const GhostOp *ops, int nops)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nops) return;
+    GhostOp o = ops[i];
+    double *b = sl.base[o.slot];
+    long pl = sl.plane[o.slot];
+    int nc = sl.ncomp[o.slot];
+    for (int c = 0; c < nc; ++c) {
+        int m = c == 0 ? o.m0 : (c == 1 ? o.m1 : o.mr);
+        b[c * pl + o.dst] = gmode(b[c * pl + o.src], m);
+    }
+}
+__global__ void k_ghost_gather(GhostSlots sl, const GhostOp *ops, int nops, double *stage, int maxc)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nops) return;
+    GhostOp o = ops[i];
+    const double *b = sl.base[o.slot];
+    long pl = sl.plane[o.slot];
+    int nc = sl.ncomp[o.slot];
+    for (int c = 0; c < nc; ++c) {
+        int m = c == 0 ? o.m0 : (c == 1 ? o.m1 : o.mr);
+        stage[(long)c * nops + i] = gmode(b[c * pl + o.src], m);
+    }
+    (void)maxc;
+}
+__global__ void k_ghost_scatter(GhostSlots sl, const GhostOp *ops, int nops, const double *stage)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nops) return;
+    GhostOp o = ops[i];
+    double *b = sl.base[o.slot];
+    long pl = sl.plane[o.slot];
+    int nc = sl.ncomp[o.slot];
+    for (int c = 0; c < nc; ++c) b[c * pl + o.dst] = stage[(long)c * nops + i];
+}
+void launch_ghost(hipStream_t s, const GhostSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase, int maxc)
+{
+    if (nops <= 0) return;
+    if (!two_phase) {
+        k_ghost_direct<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops);
+    } else {
+        k_ghost_gather<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, maxc);
+        k_ghost_scatter<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage);
+    }
+}
+
+/* vertex index of an active vertex (distance >= 1 from the padded edge) */
+__device__ __forceinline__ bool active_vertex(const Grid &g, long t, int &k, int &j)
+{
+    int w = g.VW - 2;
+    if (t >= (long)w * (g.VH - 2)) return false;
+    k = 1 + (int)(t % w);
+    j = 1 + (int)(t / w);
+    return true;
+}
+/* physical cell (distance >= 2) */
+__device__ __forceinline__ bool physical_cell(const Grid &g, long t, int &k, int &j)
+{
+    if (t >= (long)g.nx * g.ny) return false;
+    k = 2 + (int)(t % g.nx);
+    j = 2 + (int)(t / g.nx);
+    return true;
+}
+
+/* -------------------------------------------- update_v_vof + v_vel_inc -- */
+/* update_v_vof (src/sale2d.c:751-790) and update_v_vel_inc
+ * (src/sale2d.c:930-1048) over all active vertices, run after the m_vof,
+ * e_dvel and e_den ghost fills.  The reference runs the internal vertices
+ * before those fills; internal vertices touch only cells at distance >= 3,
+ * which the fills never write, so the results are identical. */
+__global__ void k_vertex_vof_vel(Grid g, Fields F, Params P)
+{
+    int k, j;
+    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    long v = k + (long)g.VW * j;
+    long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
+    for (int m = 0; m < g.nmat; ++m) {
+        double a = 0.0;
+        for (int i = 0; i < 4; ++i) a += 0.25 * CPL(M_VOF, m, e[i]);
+        VPL(V_VOF, m, v) = a;
+    }
+    const int voff[4] = {2, 3, 0, 1};
+    double inc0 = 0., inc1 = 0., old0 = VPL(V_VEL, 0, v), old1 = VPL(V_VEL, 1, v);
+    double vvol = 0., vmass = 0.;
+    int vc = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (CPL(E_WPT, voff[i], e[i]) > 0.) vc++;
+        double den = CPL(E_DEN, 0, e[i]);
+        if (den < TOLRHO) continue;
+        double lv = VPL(V_VOL, i + 1, v);
+        vvol += lv;
+        vmass += den * lv;
+        double pa = den * lv;
+        inc0 += pa * CPL(E_DVEL, 0, e[i]);
+        inc1 += pa * CPL(E_DVEL, 1, e[i]);
+    }
+    double v0, v1;
+    if (vmass > TOLRHO * vvol) {
+        double pa = 1.0 / vmass, pb = VPL(V_MASS, 0, v) / vmass;
+        v0 = pa * inc0 + pb * old0;
+        v1 = pa * inc1 + pb * old1;
+        VPL(V_MASS, 0, v) = vmass;
+    } else {
+        VPL(V_MASS, 0, v) = 0.;
+        v0 = 0.0; v1 = 0.0;
+    }
+    VPL(V_MASS, 1, v) = 1.0 * vc;
+    double r = 0.;
+    r += v0 * v0; r += v1 * v1;
+    double vn = sqrt(r);
+    if (vn > P.velocity_max) { double px = P.velocity_max / vn; v0 *= px; v1 *= px; }
+    if (vn < P.velocity_min * 1) { v0 = 0.0; v1 = 0.0; }
+    VPL(V_VEL, 0, v) = v0; VPL(V_VEL, 1, v) = v1;
+}
+void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const Params &P)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    k_vertex_vof_vel<<<nblk(n, 256), 256, 0, s>>>(g, F, P);
+}
+
+/* ------------------------------------------------------- update_e_vel -- */
+/* src/sale2d.c:838-927 */
+__global__ void k_update_e_vel(Grid g, Fields F)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    long c = k + (long)g.CW * j;
+    long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
+    double den = CPL(E_DEN, 0, c);
+    double e0 = 0.0, e1 = 0.0;
+    if (den >= TOLRHO) {
+        int vac = 0;
+        double w[4];
+        for (int kv = 0; kv < 4; ++kv) {
+            w[kv] = CPL(E_WPT, kv, c);
+            double vm1 = VPL(V_MASS, 1, vv[kv]);
+            if (vm1 >= 4.0 || (vm1 >= 3.0 && w[kv] < 0.)) vac++;
+        }
+        if (vac != 4) {
+            double sm = 0.;
+            for (int kv = 0; kv < 4; ++kv) {
+                if (w[kv] > 0.0) continue;
+                double m = VPL(V_MASS, 0, vv[kv]);
+                e0 += m * VPL(V_VEL, 0, vv[kv]);
+                e1 += m * VPL(V_VEL, 1, vv[kv]);
+                sm += m;
+            }
+            if (sm > 0.) { double px = 1.0 / sm; e0 *= px; e1 *= px; }
+            else { e0 = 0.0; e1 = 0.0; }
+        }
+    }
+    CPL(E_VEL, 0, c) = e0; CPL(E_VEL, 1, c) = e1;
+}
+void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F)
+{
+    k_update_e_vel<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F);
+}
+
+/* -------------------------------------------------------- check_v_vel -- */
+/* src/sale2d.c:1051-1141 */
+__global__ void k_check_v_vel(Grid g, Fields F)
+{
+    int k, j;
+    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    long v = k + (long)g.VW * j;
+    long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
+    const int voff[4] = {2, 3, 0, 1};
+    int vcell = 0, vcor = 0;
+    double d[4];
+    for (int i = 0; i < 4; ++i) {
+        d[i] = CPL(E_DEN, 0, e[i]);
+        if (d[i] < TOLRHO) vcell++;
+        if (CPL(E_WPT, voff[i], e[i]) > 0.) vcor++;
+    }
+    if (vcell >= 4 || vcor < 4) return;
+    double a0 = 0., a1 = 0., ad = 0.;
+    for (int i = 0; i < 4; ++i) {
+        if (d[i] < TOLRHO) continue;
+        ad += d[i];
+        a0 += d[i] * CPL(E_VEL, 0, e[i]);
+        a1 += d[i] * CPL(E_VEL, 1, e[i]);
+    }
+    double pa = 1.0 / ad;
+    VPL(V_VEL, 0, v) = pa * a0;
+    VPL(V_VEL, 1, v) = pa * a1;
+}
+void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    k_check_v_vel<<<nblk(n, 256), 256, 0, s>>>(g, F);
+}
+
+/* ------------------------------------------- update_e_gradvel + dt min -- */
+/* update_e_gradvel (src/sale2d.c:1144-1235) fused with the per-cell CFL
+ * bound of calculate_dt (src/sale2d.c:1959-2016).  The minimum is reduced
+ * with wave shuffles, per block, then by the last block to finish; min is
+ * exact and order-independent, so dt is bit-reproducible.
+ * dt = Min(max_dt, MIN(1.1*dt_prev, DTF*min_cells(...)))  (src/salec2d.c:167) */
+#define GV_BS 256
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, Params P, DevTime *tm, double *part, unsigned *counter)
+{
+    __shared__ double red[GV_BS / 64];
+    __shared__ bool last;
+    int k, j;
+    double cand = HUGE_VAL;  /* identity of MIN over finite values */
+    bool has = false;
+    if (physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) {
+        long c = k + (long)g.CW * j;
+        long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
+        double vx[4], vy[4];
+        for (int i = 0; i < 4; ++i) { vx[i] = VPL(V_VEL, 0, vv[i]); vy[i] = VPL(V_VEL, 1, vv[i]); }
+        double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+        for (int i = 0; i < 4; ++i) g0 += CPL(E_GRAD, i, c) * vx[i];
+        for (int i = 0; i < 4; ++i) g2 += CPL(E_GRAD, i, c) * vy[i];
+        for (int i = 0; i < 4; ++i) g1 += CPL(E_GRAD, 4 + i, c) * vx[i];
+        for (int i = 0; i < 4; ++i) g3 += CPL(E_GRAD, 4 + i, c) * vy[i];
+        for (int i = 0; i < 4; ++i) g4 += CPL(E_GRAD, 8 + i, c) * vx[i];
+        CPL(E_GRAD_VEL, 0, c) = g0; CPL(E_GRAD_VEL, 1, c) = g1; CPL(E_GRAD_VEL, 2, c) = g2;
+        CPL(E_GRAD_VEL, 3, c) = g3; CPL(E_GRAD_VEL, 4, c) = g4;
+        double div = g0 + g3 + g4;
+        CPL(E_DIV_VEL, 0, c) = div;
+        double v0 = CPL(M_VOF, 0, c), csd = CPL(E_CSD, 0, c);
+        double q;
+        if (v0 > TOLVOF || CPL(E_PRE, 0, c) <= 0.0) {
+            q = 0.;
+        } else {
+            double ea = CPL(E_VOL, 0, c) / CPL(E_POS, 0, c);
+            ea = fabs(ea);
+            double ql = P.avis1 * csd * sqrt(ea);
+            double qq = P.avis2 * div * ea;
+            q = (0. < div ? 0. : div) * CPL(E_DEN, 0, c) * (qq - ql);
+            if (q < TOLQPRE) q = 0.;
+        }
+        CPL(E_Q, 0, c) = q;
+        if (!(v0 > 1.0 - TOLVOF)) {
+            double cyl = 1.0;
+            double dr = cyl * CPL(E_POS, 2, c) / (fabs(CPL(E_VEL, 0, c)) + csd);
+            double dz = 1.0 * CPL(E_POS, 3, c) / (fabs(CPL(E_VEL, 1, c)) + csd);
+            cand = DTF * (dr < dz ? dr : dz);
+            has = true;
+        }
+    }
+    (void)has;
+    double m = wave_min(cand);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = red[0];
+        for (int w = 1; w < GV_BS / 64; ++w) b = red[w] < b ? red[w] : b;
+        part[blockIdx.x] = b;
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    /* last block: reduce the per-block minima */
+    __threadfence();
+    double r = HUGE_VAL;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        double v = __builtin_nontemporal_load(part + b);
+        r = v < r ? v : r;
+    }
+    r = wave_min(r);
+    if (lane == 0) red[wid] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = red[0];
+        for (int w = 1; w < GV_BS / 64; ++w) b = red[w] < b ? red[w] : b;
+        double rtn = 1.1 * tm->dt;
+        rtn = b < rtn ? b : rtn;       /* MIN(rtn_dt, dt_rz) over all cells */
+        double ldt = dMin(P.max_dt, rtn);
+        tm->dt = ldt;
+        *counter = 0u;
+    }
+}
+int gradvel_dt_blocks(const Grid &g) { return nblk((long)g.nx * g.ny, GV_BS); }
+void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const Params &P, DevTime *tm,
+                       double *part, unsigned *counter, int nblocks)
+{
+    k_gradvel_dt<<<nblocks, GV_BS, 0, s>>>(g, F, P, tm, part, counter);
+}
+
+/* ------------------------------------ block_vibration + update_e_stress -- */
+/* block_vibration (src/strength.c:625-695) then update_e_stress
+ * (src/strength.c:7-349, avg_strength mixing) for one physical cell. */
+__global__ void k_stress(Grid g, Fields F, Params P, const DevMat *mats, const DevTime *tm)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    const double dt = tm->dt, time = tm->t;
+    long c = k + (long)g.CW * j;
+    int n = g.nmat;
+    double v0 = CPL(M_VOF, 0, c);
+    double dam = CPL(E_DAM, 0, c);
+    double vib = CPL(E_VIB, 0, c);
+    if (P.acoustic_fluid) {
+        if (v0 > TOLVOF || dam < 0.5) {
+            vib = 0.;
+        } else {
+            double Td = -1.0, Cv = 0.0, To = -1.0, VM = 0.0;
+            for (int m = 1; m < n; ++m) {
+                if (CPL(M_VOF, m, c) < TOLVOF) continue;
+                const DevMat &r = mats[m];
+                Td = dMax(r.Tdamp, Td); Cv = dMax(r.Cvib, Cv); To = dMax(r.Toff, To); VM = dMax(r.VibMax, VM);
+            }
+            if (Td <= 0. || To <= 0.) {
+                vib = 0.;
+            } else {
+                double ev0 = CPL(E_VEL, 0, c), ev1 = CPL(E_VEL, 1, c);
+                double l = 0.;
+                l += ev0 * ev0; l += ev1 * ev1;
+                double nv = Cv * sqrt(l), nv2 = nv * nv;
+                if ((time <= To) && (nv2 >= vib) && (nv >= 1.0) && (CPL(E_Q, 0, c) > 0.))
+                    vib = dMin(nv2, VM * VM);
+                else
+                    vib = dMax(vib * (1.0 - 2.0 * dt / Td + 2.0 * (dt / Td) * (dt / Td)), 0.0);
+            }
+        }
+        CPL(E_VIB, 0, c) = vib;
+    }
+    if (v0 > TOLVOF) {
+        for (int i = 0; i < 4; ++i) CPL(E_STE, i, c) = 0.0;
+        CPL(E_STE2S, 0, c) = 0.; CPL(E_STA2S, 0, c) = 0.; CPL(E_STRENGTH, 0, c) = 0.;
+        return;
+    }
+    double ste[4];
+    for (int i = 0; i < 4; ++i) ste[i] = CPL(E_STE, i, c);
+    double vref = 0., mref = 0.;
+    for (int m = 1; m < n; ++m) {
+        double vf = CPL(M_VOF, m, c);
+        if (vf < TOLVOF) continue;
+        vref += vf * mats[m].VaporDen;
+        mref += vf * mats[m].MeltDen;
+    }
+    double mp = 1.0;
+    mp = dMax(mp, 1.0);
+    vref /= mp; mref /= mp;
+    double eden = CPL(E_DEN, 0, c);
+    double s2o = CPL(E_STE2S, 0, c);
+    double ste2s = s2o, sta2s = CPL(E_STA2S, 0, c), tps = CPL(E_TPS, 0, c);
+    double strength = CPL(E_STRENGTH, 0, c);
+    int vapor = 0, melt = 0;
+    if (eden <= vref) {
+        for (int i = 0; i < 4; ++i) ste[i] = 0.0;
+        ste2s = 0.; tps = 0.; vapor = 1; dam = 0.; strength = 0.;
+    } else if (eden <= mref) {
+        for (int i = 0; i < 4; ++i) ste[i] = 0.0;
+        ste2s = 0.; melt = 1;
+        double si[4] = {CPL(E_GRAD_VEL, 0, c), 0.5 * CPL(E_GRAD_VEL, 1, c) + 0.5 * CPL(E_GRAD_VEL, 2, c),
+                        CPL(E_GRAD_VEL, 3, c), CPL(E_GRAD_VEL, 4, c)};
+        double tr = si[0] + si[2] + si[3];
+        si[0] -= tr / 3.0; si[2] -= tr / 3.0; si[3] -= tr / 3.0;
+        sta2s = sqrt(second_inv(si));
+        tps = dMin(tps + sta2s * dt, 10.);
+        dam = 1.; strength = 0.;
+    } else {
+        double gv[5];
+        for (int i = 0; i < 5; ++i) gv[i] = CPL(E_GRAD_VEL, i, c);
+        double si[4] = {gv[0], 0.5 * gv[1] + 0.5 * gv[2], gv[3], gv[4]};
+        double tr = si[0] + si[2] + si[3];
+        si[0] -= tr / 3.0; si[2] -= tr / 3.0; si[3] -= tr / 3.0;
+        double s2w = (gv[2] - gv[1]) * dt;
+        s2w = dWind(s2w, -1., 1.);
+        double c2w = sqrt(1.0 - s2w * s2w) - 1.0;
+        double ss2 = (ste[0] - ste[2]) / 2.0;
+        double rot[4] = {ss2 * c2w + ste[1] * s2w, ste[1] * c2w - ss2 * s2w, -ss2 * c2w - ste[1] * s2w, 0.0};
+        for (int i = 0; i < 4; ++i) ste[i] += 1.0 * rot[i];
+        double pa = 2.0 * CPL(E_MU, 0, c) * dt;
+        for (int i = 0; i < 4; ++i) ste[i] += pa * si[i];
+        ste2s = sqrt(second_inv(ste));
+        sta2s = sqrt(second_inv(si));
+    }
+    double pre = CPL(E_PRE, 0, c), tem = CPL(E_TEM, 0, c), csd = CPL(E_CSD, 0, c);
+    if (vapor) {
+        dam = 0.;
+    } else if (melt) {
+        dam = 1.0;
+    } else {
+        double lss = 0.0, lts = 0.;
+        for (int m = 1; m < n; ++m) {
+            double vf = CPL(M_VOF, m, c);
+            if (vf < TOLVOF) continue;
+            const DevMat &r = mats[m];
+            double my = strength_fn(r.yfunc, pre, dam, tps, sta2s, tem, r);
+            if (P.acoustic_fluid && pre >= 0.) {
+                double vp = dMin(eden * csd * sqrt(vib), pre);
+                double mt = SimonMelt(pre, r);
+                double sf = OhnakaSoft(tem, r, mt);
+                sf = dWind(sf, 0.0, 1.0);
+                double lp = pre - vp * dam * dam * sqrt(sf);
+                double myv = strength_fn(r.yfunc, lp, dam, tps, sta2s, tem, r) + eden * r.Acvis * sta2s;
+                my = dMin(myv, my);
+            }
+            lss += vf * dMax(my, 0.0);
+            if (P.tensile_failure) lts += vf * strength_fn(r.tfunc, pre, dam, tps, sta2s, tem, r);
+        }
+        strength = lss;
+        if (lss < ste2s) {
+            sta2s = sta2s / dMax(0.5, (ste2s - s2o) / (ste2s - lss));
+            double sfac = lss / ste2s;
+            for (int i = 0; i < 4; ++i) ste[i] *= sfac;
+            ste2s = lss;
+            double dsd = 0.;
+            for (int m = 1; m < n; ++m)
+                dsd += CPL(M_VOF, m, c) * strength_fn(mats[m].dfunc, pre, dam, 0., 0., 0., mats[m]);
+            dam = dWind(dam + sta2s * dsd * dt, 0.0, 1.0);
+            tps = dMin(tps + sta2s * dt, 10.);
+        }
+        if (P.tensile_failure) {
+            double Sxx = ste[0], Szz = ste[2], Sxz = ste[1], Sth = ste[3];
+            double Ts = sqrt(Sxz * Sxz + 0.25 * (Sxx - Szz) * (Sxx - Szz));
+            double Tmax = dMax(0.5 * (Sxx + Szz) - Ts, 0.5 * (Sxx + Szz) + Ts);
+            Tmax = dMax(Tmax, Sth);
+            if ((Tmax - pre) > lts) {
+                double dinc = 0.4 * csd / dMin(CPL(E_POS, 1, c), CPL(E_POS, 2, c));
+                double dold = dam;
+                double t3 = pow(dold, 1.0 / 3.);
+                t3 += dinc;
+                dam = dWind(pow(t3, 3.0), 0., 1.);
+                double sfac = dold < 0.98 ? (1.0 - dam) / (1.0 - dold) : 0.0;
+                for (int i = 0; i < 4; ++i) ste[i] *= sfac;
+                ste2s *= sfac;
+            }
+        }
+    }
+    for (int i = 0; i < 4; ++i) CPL(E_STE, i, c) = ste[i];
+    CPL(E_STE2S, 0, c) = ste2s; CPL(E_STA2S, 0, c) = sta2s; CPL(E_TPS, 0, c) = tps;
+    CPL(E_DAM, 0, c) = dam; CPL(E_STRENGTH, 0, c) = strength;
+}
+void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm)
+{
+    k_stress<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats, tm);
+}
+
+/* -------------------------------------------------------- calculate_acc -- */
+/* calculate_acc_pml (src/velocity.c:158-340) for pml2/pml3 and
+ * calculate_acc_default (src/velocity.c:9-155) otherwise, over all active
+ * vertices after the e_pre/e_ste/e_q ghost fills (internal vertices never
+ * read the cells those fills write). */
+template <bool PML>
+__global__ void k_acc(Grid g, Fields F, Params P, const DevTime *tm)
+{
+    int k, j;
+    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    const double dt = tm->dt;
+    long v = k + (long)g.VW * j;
+    long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
+    double a0 = 0.0, a1 = 0.0, vm = 0.0, vv = 0.0;
+    double vs0 = 0., vs1 = 0., al0 = 0., al1 = 0.;
+    bool damp = false;
+    if (PML) {
+        for (int i = 0; i < 4; ++i) { vs0 += 0.25 * CPL(E_SIGMA, 0, e[i]); vs1 += 0.25 * CPL(E_SIGMA, 1, e[i]); }
+        al0 = VPL(V_ALPHA, 0, v); al1 = VPL(V_ALPHA, 1, v);
+        damp = (vs0 > 0. || vs1 > 0.);
+    }
+    for (int i = 0; i < 4; ++i) {
+        double su0 = VPL(V_SUF, 2 * i, v), su1 = VPL(V_SUF, 2 * i + 1, v), sv = VPL(V_SUBVOL, i, v);
+        double lp = CPL(E_PRE, 0, e[i]) + CPL(E_Q, 0, e[i]);
+        double s0 = CPL(E_STE, 0, e[i]), s1 = CPL(E_STE, 1, e[i]), s2 = CPL(E_STE, 2, e[i]), s3 = CPL(E_STE, 3, e[i]);
+        if (PML) {
+            double lpr = CPL(E_DAMPREF, 2, e[i]);
+            double Sxx = s0 - lp, Szz = s2 - lp, Sxz = s1, Sth = s3 - lp;
+            double f0 = 0, f1 = 0;
+            f0 += su0 * Sxx; f0 += su1 * Sxz;
+            f1 += su0 * Sxz; f1 += su1 * Szz;
+            f0 += sv * Sth;
+            a0 += -1.0 * f0; a1 += -1.0 * f1;
+            if (damp) {
+                Sxx += lpr; Szz += lpr; Sth += lpr;
+                double b0 = 0, b1 = 0;
+                b0 += su0 * (vs1 * Sxx); b0 += su1 * (vs0 * Sxz);
+                b1 += su0 * (vs1 * Sxz); b1 += su1 * (vs0 * Szz);
+                al0 += b0 * dt;
+                al1 += b1 * dt;
+                al0 += (vs0 + vs1) * sv * Sth * dt;
+            }
+        } else {
+            double fp0 = 0., fp1 = 0.;
+            fp0 += lp * su0; fp1 += lp * su1;
+            fp0 += sv * lp;
+            double f0 = 0, f1 = 0;
+            f0 += su0 * s0; f0 += su1 * s1;
+            f1 += su0 * s1; f1 += su1 * s2;
+            f0 += sv * s3;
+            a0 += -1.0 * f0; a1 += -1.0 * f1;
+            a0 += 1.0 * fp0; a1 += 1.0 * fp1;
+        }
+        double lv = VPL(V_VOL, 1 + i, v);
+        vm += lv * CPL(E_DEN, 0, e[i]);
+        vv += lv;
+    }
+    if (PML) {
+        if (damp) {
+            al0 -= P.alpha_damp_x * al0 * dt;
+            al1 -= P.alpha_damp_y * al1 * dt;
+            a0 += -1.0 * al0; a1 += -1.0 * al1;
+        }
+        VPL(V_ALPHA, 0, v) = al0; VPL(V_ALPHA, 1, v) = al1;
+    }
+    int xd = min(k, g.VW - 1 - k), yd = min(j, g.VH - 1 - j);
+    double n0 = 0., n1 = 0.;
+    if (!(xd <= 1 || yd <= 1)) {
+        double L0 = 0, L1 = 0;
+        int q = 0;
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk, ++q) {
+                long w = (k + dk) + (long)g.VW * (j + dj);
+                double wt = VPL(V_LAPLACE, q, v);
+                L0 += wt * VPL(V_VEL, 0, w);
+                L1 += wt * VPL(V_VEL, 1, w);
+            }
+        double sq = sqrt(fabs(VPL(V_SUBVOL, 0, v)) + fabs(VPL(V_SUBVOL, 2, v)));
+        n0 = 5.0e3 * sq * L0;
+        n1 = 5.0e3 * sq * L1;
+    }
+    if (VPL(V_MASS, 1, v) >= 1.) { n0 = 0.0; n1 = 0.0; }
+    if (vm > TOLRHO * vv) {
+        double px = 1.0 / vm;
+        a0 *= px; a1 *= px;
+        a0 += 1.0 * VPL(V_BF, 0, v); a1 += 1.0 * VPL(V_BF, 1, v);
+        a0 += P.anc * n0; a1 += P.anc * n1;
+    } else if (PML) {
+        a0 = VPL(V_BF, 0, v); a1 = VPL(V_BF, 1, v);
+    } else {
+        a0 = 0.0; a1 = 0.0;
+    }
+    if (fabs(a0) < P.acceleration_min * 1) a0 = 0.;
+    if (fabs(a1) < P.acceleration_min * 1) a1 = 0.;
+    VPL(V_ACC, 0, v) = a0; VPL(V_ACC, 1, v) = a1;
+}
+void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3)
+        k_acc<true><<<nblk(n, 256), 256, 0, s>>>(g, F, P, tm);
+    else
+        k_acc<false><<<nblk(n, 256), 256, 0, s>>>(g, F, P, tm);
+}
+
+/* ------------------------------------------------------ calculate_v_dis -- */
+/* src/sale2d.c:1238-1268 */
+__global__ void k_v_dis(Grid g, Fields F, const DevTime *tm)
+{
+    int k, j;
+    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    const double dt = tm->dt;
+    long v = k + (long)g.VW * j;
+    double pb = 0.5 * dt * dt;
+    for (int c = 0; c < 2; ++c) {
+        double u = VPL(V_VEL, c, v), a = VPL(V_ACC, c, v);
+        VPL(V_DIS, c, v) = dt * u + pb * a;
+        VPL(V_VEL, c, v) = u + dt * a;
+    }
+}
+void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const DevTime *tm)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    k_v_dis<<<nblk(n, 256), 256, 0, s>>>(g, F, tm);
+}
+
+/* -------------------------------------------------------- update_energy -- */
+/* src/sale2d.c:564-712 (porosity model off) */
+__global__ void k_energy(Grid g, Fields F, const DevMat *mats, const DevTime *tm)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    const double dt = tm->dt;
+    long c = k + (long)g.CW * j;
+    int n = g.nmat;
+    if (CPL(M_VOF, 0, c) > TOLVOF || CPL(E_DEN, 0, c) < TOLRHO) return;
+    double lp = CPL(E_PRE, 0, c) + CPL(E_Q, 0, c);
+    double div = CPL(E_DIV_VEL, 0, c);
+    double dEp = -lp * div;
+    double dEs = CPL(E_STE, 0, c) * CPL(E_GRAD_VEL, 0, c)
+               + CPL(E_STE, 1, c) * (CPL(E_GRAD_VEL, 1, c) + CPL(E_GRAD_VEL, 2, c))
+               + CPL(E_STE, 2, c) * CPL(E_GRAD_VEL, 3, c) + CPL(E_STE, 3, c) * CPL(E_GRAD_VEL, 4, c);
+    double eg = 0.;
+    for (int m = 1; m < n; ++m) {
+        double vf = CPL(M_VOF, m, c);
+        if (vf > TOLVOF) eg += vf * mats[m].gamfrac / CPL(M_DEN, m, c);
+    }
+    double dcvs = div - eg * (dEp + dEs);
+    for (int m = 1; m < n; ++m) {
+        if (CPL(M_VOF, m, c) <= TOLVOF || CPL(M_DEN, m, c) <= TOLRHO) {
+            CPL(M_ENG, m, c) = 0.;
+        } else {
+            double f = 1.0;
+            CPL(M_ENG, m, c) = dMax(0., CPL(M_ENG, m, c) + (dEp + dEs) * dt * f);
+        }
+    }
+    CPL(E_CVS, 0, c) = dWind(CPL(E_CVS, 0, c) + dcvs * dt, -10.0, 10.0);
+}
+void launch_energy(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm)
+{
+    k_energy<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, mats, tm);
+}
+
+/* ------------------------------------------------------------- tracers -- */
+/* fl_update_node, src/tracer.c:52-175; a tracer that leaves the physical
+ * cells is deleted (its neighbour rank is absent on a single domain). */
+__global__ void k_tracers(Grid g, Fields F, Params P, Tracers T, const double *vx, const double *vy, const DevTime *tm)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= T.n || !T.alive[i]) return;
+    const double dt = tm->dt;
+    int k = T.idx[i], j = T.idy[i];
+    long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
+    long c = k + (long)g.CW * j;
+    double p0 = T.pos[i], p1 = T.pos[T.n + i];
+    double u0 = T.vel[i], u1 = T.vel[T.n + i];
+    double r0 = 1.0 * p0 + -1.0 * VPL(V_POS, 0, vv[0]);
+    double r1 = 1.0 * p1 + -1.0 * VPL(V_POS, 1, vv[0]);
+    r0 = r0 / (vx[k + 1] - vx[k]) * 2.0 - 1.0;
+    r1 = r1 / (vy[j + 1] - vy[j]) * 2.0 - 1.0;
+    if (CPL(M_VOF, 0, c) > TOLVOF) {
+        double b0 = interp2d(VPL(V_ACC, 0, vv[0]), VPL(V_ACC, 0, vv[1]), VPL(V_ACC, 0, vv[2]), VPL(V_ACC, 0, vv[3]), r0, r1);
+        double b1 = interp2d(VPL(V_ACC, 1, vv[0]), VPL(V_ACC, 1, vv[1]), VPL(V_ACC, 1, vv[2]), VPL(V_ACC, 1, vv[3]), r0, r1);
+        double pb = 0.5 * dt * dt;
+        double d0 = dt * u0 + pb * b0, d1 = dt * u1 + pb * b1;
+        u0 += dt * b0; u1 += dt * b1;
+        p0 += 1.0 * d0; p1 += 1.0 * d1;
+    } else {
+        double d0 = interp2d(VPL(V_DIS, 0, vv[0]), VPL(V_DIS, 0, vv[1]), VPL(V_DIS, 0, vv[2]), VPL(V_DIS, 0, vv[3]), r0, r1);
+        double d1 = interp2d(VPL(V_DIS, 1, vv[0]), VPL(V_DIS, 1, vv[1]), VPL(V_DIS, 1, vv[2]), VPL(V_DIS, 1, vv[3]), r0, r1);
+        double t0 = interp2d(VPL(V_VEL, 0, vv[0]), VPL(V_VEL, 0, vv[1]), VPL(V_VEL, 0, vv[2]), VPL(V_VEL, 0, vv[3]), r0, r1);
+        double t1 = interp2d(VPL(V_VEL, 1, vv[0]), VPL(V_VEL, 1, vv[1]), VPL(V_VEL, 1, vv[2]), VPL(V_VEL, 1, vv[3]), r0, r1);
+        p0 += 1.0 * d0; p1 += 1.0 * d1;
+        u0 = 1.0 * t0; u1 = 1.0 * t1;
+    }
+    int mid = T.matid[i];
+    if (mid > 0 && p1 > P.threshold_z) T.matid[i] = -mid;
+    double ep = CPL(E_PRE, 0, c), et = CPL(E_TEM, 0, c);
+    double mp = (double)T.mpre[i], mt = (double)T.mtem[i];
+    T.mpre[i] = (float)(mp > ep ? mp : ep);
+    T.mtem[i] = (float)(mt > et ? mt : et);
+    int xo = (p0 < vx[k]) ? -1 : ((p0 < vx[k + 1] && p0 >= vx[k]) ? 0 : 1);
+    int yo = (p1 < vy[j]) ? -1 : ((p1 < vy[j + 1] && p1 >= vy[j]) ? 0 : 1);
+    k += xo; j += yo;
+    T.pos[i] = p0; T.pos[T.n + i] = p1;
+    T.vel[i] = u0; T.vel[T.n + i] = u1;
+    T.idx[i] = k; T.idy[i] = j;
+    if (k < 2 || k > g.CW - 3 || j < 2 || j > g.CH - 3) T.alive[i] = 0;
+}
+void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const Tracers &T,
+                    const double *vx, const double *vy, const DevTime *tm)
+{
+    if (T.n <= 0) return;
+    k_tracers<<<nblk(T.n, 256), 256, 0, s>>>(g, F, P, T, vx, vy, tm);
+}
+
+/* ------------------------------------------------ calculate_flux_mixed -- */
+/* src/advect.c:861-1136 with _calculate_flux_set_bphi (:1138-1224).  The
+ * donor writes the packet of each of its outflow faces (swept volume < 0). */
+__device__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by, int k, int j)
+{
+    long c = k + (long)g.CW * j;
+    int n = g.nmat;
+    long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
+    double px[4], py[4], dx[4], dy[4];
+    for (int i = 0; i < 4; ++i) {
+        px[i] = VPL(V_POS, 0, vv[i]); py[i] = VPL(V_POS, 1, vv[i]);
+        dx[i] = VPL(V_DIS, 0, vv[i]); dy[i] = VPL(V_DIS, 1, vv[i]);
+    }
+    double lv = flux_volcyl(px[0], py[0], px[3], py[3], dx[0], dy[0], dx[3], dy[3]);
+    double rv = flux_volcyl(px[2], py[2], px[1], py[1], dx[2], dy[2], dx[1], dy[1]);
+    double bv = flux_volcyl(px[1], py[1], px[0], py[0], dx[1], dy[1], dx[0], dy[0]);
+    double uv = flux_volcyl(px[3], py[3], px[2], py[2], dx[3], dy[3], dx[2], dy[2]);
+    double lbru[4] = {lv, bv, rv, uv};
+    double ev[SALEC_MAXMAT];
+    for (int m = 0; m < n; ++m) ev[m] = CPL(M_VOF, m, c);
+    double evol = CPL(E_VOL, 0, c);
+    double fv[SALEC_MAXMAT][4];
+    int pm = 0;
+    if (ev[0] < TOLVOF)
+        for (int q = 1; q < n; ++q) if (ev[q] > ev[pm]) pm = q;
+    double mlv = lv < 0 ? lv : 0, mrv = rv < 0 ? rv : 0, mbv = bv < 0 ? bv : 0, muv = uv < 0 ? uv : 0;
+    if (ev[pm] > 1.0 - TOLVOF) {
+        for (int q = 0; q < n; ++q) {
+            if (q == pm) { fv[q][0] = mlv; fv[q][2] = -1.0 * mrv; fv[q][1] = mbv; fv[q][3] = -1.0 * muv; }
+            else { fv[q][0] = fv[q][1] = fv[q][2] = fv[q][3] = 0.; }
+        }
+        double w = (pm == 0) ? 1.0 : -0.25;
+        for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = w;
+    } else {
+        double gr[8];
+        for (int i = 0; i < 8; ++i) gr[i] = CPL(E_GRAD, i, c);
+        double g0 = 0, g1 = 0;
+        for (int i = 0; i < 4; ++i) g0 += VPL(V_VOF, pm, vv[i]) * gr[i];
+        for (int i = 0; i < 4; ++i) g1 += VPL(V_VOF, pm, vv[i]) * gr[4 + i];
+        vnorm2(g0, g1);
+        double ca = CPL(E_POS, 2, c) * fabs(g0), cb = CPL(E_POS, 3, c) * fabs(g1);
+        if (ca < cb) { double t = ca; ca = cb; cb = t; }
+        double dstar = cut_length(ev[pm], ca, cb);
+        double w[4];
+        for (int i = 0; i < 4; ++i) { double z = 0; z += g0 * px[i]; z += g1 * py[i]; w[i] = z; }
+        double mw = w[0] > w[1] ? w[0] : w[1];
+        double mw2 = w[2] > w[3] ? w[2] : w[3];
+        mw = mw > mw2 ? mw : mw2;
+        for (int i = 0; i < 4; ++i) w[i] = w[i] - mw + dstar;
+        for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = (pm == 0) ? w[i] : -0.25;
+        double fp[4];
+        fp[0] = geo_ratio(w[3], w[0]);
+        fp[2] = geo_ratio(w[2], w[1]);
+        fp[1] = geo_ratio(w[0], w[1]);
+        fp[3] = geo_ratio(w[3], w[2]);
+        double cc = (lv < 0.0 ? lv : 0.0) + mrv + mbv + muv;
+        cc = fabs(cc) / evol;
+        long nb[4] = {k - 1 + (long)g.CW * j, k + (long)g.CW * (j - 1), k + 1 + (long)g.CW * j, k + (long)g.CW * (j + 1)};
+        bool nbok[4] = {k - 1 >= 0, j - 1 >= 0, k + 1 < g.CW, j + 1 < g.CH};
+        double bn[4][2];
+        bn[0][0] = F.f[SALEC_BX_NORMAL][k + (long)g.BXW * j]; bn[0][1] = F.f[SALEC_BX_NORMAL][g.NBX + k + (long)g.BXW * j];
+        bn[2][0] = F.f[SALEC_BX_NORMAL][k + 1 + (long)g.BXW * j]; bn[2][1] = F.f[SALEC_BX_NORMAL][g.NBX + k + 1 + (long)g.BXW * j];
+        bn[1][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * j]; bn[1][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * j];
+        bn[3][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * (j + 1)]; bn[3][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * (j + 1)];
+        double fnp[SALEC_MAXMAT][4];
+        double snp[4] = {0., 0., 0., 0.};
+        for (int m = 0; m < n; ++m) fnp[m][0] = fnp[m][1] = fnp[m][2] = fnp[m][3] = 0.;
+        for (int m = 1; m < n; ++m) {
+            if (pm == m) continue;
+            double h0 = 0, h1 = 0;
+            for (int i = 0; i < 4; ++i) h0 += VPL(V_VOF, m, vv[i]) * gr[i];
+            for (int i = 0; i < 4; ++i) h1 += VPL(V_VOF, m, vv[i]) * gr[4 + i];
+            vnorm2(h0, h1);
+            double nbv[4];
+            for (int i = 0; i < 4; ++i) nbv[i] = nbok[i] ? CPL(M_VOF, m, nb[i]) : ev[m];
+            double up[4] = {nbv[0], nbv[1], nbv[2], nbv[3]};
+            double ac[4] = {nbv[2], nbv[3], nbv[0], nbv[1]};
+            for (int i = 0; i < 4; ++i) {
+                fnp[m][i] = alg_ratio(up[i], ev[m], ac[i], cc, bn[i][0], bn[i][1], h0, h1);
+                snp[i] += fnp[m][i];
+            }
+        }
+        for (int i = 0; i < 4; ++i) {
+            if (snp[i] > TOLVOF) { for (int m = 1; m < n; ++m) fnp[m][i] = fnp[m][i] / snp[i]; }
+            else { for (int m = 1; m < n; ++m) fnp[m][i] = ev[m] / (1.0 - ev[pm]); }
+        }
+        for (int q = 0; q < n; ++q) {
+            if (q == pm) {
+                fv[q][0] = mlv * fp[0]; fv[q][2] = -1.0 * mrv * fp[2];
+                fv[q][1] = mbv * fp[1]; fv[q][3] = -1.0 * muv * fp[3];
+            } else {
+                fv[q][0] = mlv * (1.0 - fp[0]) * fnp[q][0];
+                fv[q][2] = -1.0 * mrv * (1.0 - fp[2]) * fnp[q][2];
+                fv[q][1] = mbv * (1.0 - fp[1]) * fnp[q][1];
+                fv[q][3] = -1.0 * muv * (1.0 - fp[3]) * fnp[q][3];
+            }
+        }
+    }
+    for (int q = 1; q < n; ++q) {
+        double z = 0;
+        z += fv[q][0] * 1.0; z += fv[q][1] * 1.0; z += fv[q][2] * -1.0; z += fv[q][3] * -1.0;
+        double cvv = -1.0 * z;
+        if (cvv > ev[q] * evol) {
+            double sc = ev[q] * evol / cvv;
+            for (int i = 0; i < 4; ++i) fv[q][i] *= sc;
+        }
+    }
+    long fi[4] = {k + (long)g.BXW * j, k + (long)g.BYW * j, k + 1 + (long)g.BXW * j, k + (long)g.BYW * (j + 1)};
+    for (int i = 0; i < 4; ++i) {
+        if (lbru[i] >= 0.) continue;
+        double *ph = (i == 0 || i == 2) ? bx : by;
+        long st = (i == 0 || i == 2) ? g.NBX : g.NBY;
+        long o = fi[i];
+        for (int q = 0; q < n; ++q) {
+            ph[(long)(q) * st + o] = fv[q][i];
+            ph[(long)(q + n) * st + o] = CPL(M_DEN, q, c);
+            ph[(long)(q + 2 * n) * st + o] = CPL(M_ENG, q, c);
+        }
+        double fm = 1.0;
+        ph[(long)(3 * n) * st + o] = CPL(E_VEL, 0, c);
+        ph[(long)(3 * n + 1) * st + o] = CPL(E_VEL, 1, c);
+        for (int q = 0; q < 4; ++q) ph[(long)(3 * n + 2 + q) * st + o] = fm * CPL(E_STE, q, c);
+        ph[(long)(3 * n + 6) * st + o] = fm * CPL(E_DAM, 0, c);
+        ph[(long)(3 * n + 7) * st + o] = fm * CPL(E_TPS, 0, c);
+        ph[(long)(3 * n + 8) * st + o] = fm * CPL(E_VIB, 0, c);
+        ph[(long)(3 * n + 9) * st + o] = fm * CPL(E_EJT, 0, c);
+    }
+}
+__global__ void __launch_bounds__(128) k_flux_physical(Grid g, Fields F, double *bx, double *by)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    flux_cell(g, F, bx, by, k, j);
+}
+/* ghost ring (distance < 2): rows 0,1,CH-2,CH-1 and columns 0,1,CW-2,CW-1 */
+__global__ void __launch_bounds__(128) k_flux_ring(Grid g, Fields F, double *bx, double *by)
+{
+    long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long nrow = 4L * g.CW, ncol = 4L * g.ny;
+    int k, j;
+    if (t < nrow) {
+        int r = (int)(t / g.CW);
+        k = (int)(t % g.CW);
+        j = r < 2 ? r : g.CH - 4 + r;
+    } else if (t < nrow + ncol) {
+        long r = t - nrow;
+        j = 2 + (int)(r / 4);
+        int cc = (int)(r % 4);
+        k = cc < 2 ? cc : g.CW - 4 + cc;
+    } else {
+        return;
+    }
+    flux_cell(g, F, bx, by, k, j);
+}
+void launch_flux(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by, int ring)
+{
+    if (!ring) k_flux_physical<<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
+    else k_flux_ring<<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
+}
+
+/* --------------------------------------------------------- advect_flux -- */
+/* advect_flux_default, src/advect.c:26-302 (porosity model off).  A face
+ * holds a packet from this step iff its swept volume is non-zero: the cell
+ * on the outflow side wrote it; faces with zero volume were zeroed by
+ * clean_flux in the reference and contribute exactly nothing. */
+__global__ void __launch_bounds__(128) k_advect(Grid g, Fields F, Params P, const double *bx, const double *by)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    long c = k + (long)g.CW * j;
+    int n = g.nmat, PL = g.PL;
+    long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
+    double px[4], py[4], dx[4], dy[4];
+    for (int i = 0; i < 4; ++i) {
+        px[i] = VPL(V_POS, 0, vv[i]); py[i] = VPL(V_POS, 1, vv[i]);
+        dx[i] = VPL(V_DIS, 0, vv[i]); dy[i] = VPL(V_DIS, 1, vv[i]);
+    }
+    double sv[4];
+    sv[0] = flux_volcyl(px[0], py[0], px[3], py[3], dx[0], dy[0], dx[3], dy[3]);
+    sv[1] = flux_volcyl(px[1], py[1], px[0], py[0], dx[1], dy[1], dx[0], dy[0]);
+    sv[2] = flux_volcyl(px[2], py[2], px[1], py[1], dx[2], dy[2], dx[1], dy[1]);
+    sv[3] = flux_volcyl(px[3], py[3], px[2], py[2], dx[3], dy[3], dx[2], dy[2]);
+    long fi[4] = {k + (long)g.BXW * j, k + (long)g.BYW * j, k + 1 + (long)g.BXW * j, k + (long)g.BYW * (j + 1)};
+    const double dir[4] = {1.0, 1.0, -1.0, -1.0};
+    double dphi[3 * SALEC_MAXMAT + 10];
+    for (int q = 0; q < PL; ++q) dphi[q] = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        if (!(sv[i] < 0. || sv[i] > 0.)) continue;
+        const double *ph = (i == 0 || i == 2) ? bx : by;
+        long st = (i == 0 || i == 2) ? g.NBX : g.NBY;
+        long o = fi[i];
+        double tail[10];
+        for (int q = 0; q < 10; ++q) tail[q] = ph[(long)(3 * n + q) * st + o];
+        for (int m = 0; m < n; ++m) {
+            double mv = dir[i] * ph[(long)m * st + o];
+            double dm = ph[(long)(m + n) * st + o];
+            dphi[m] += mv;
+            dphi[m + n] += mv * dm;
+            dphi[m + 2 * n] += mv * ph[(long)(m + 2 * n) * st + o];
+            double pa = mv * dm;
+            for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * tail[q];
+        }
+    }
+    double evol = CPL(E_VOL, 0, c);
+    double emass = evol * CPL(E_DEN, 0, c);
+    double sphi[3 * SALEC_MAXMAT + 10];
+    double ev[SALEC_MAXMAT], md[SALEC_MAXMAT], me[SALEC_MAXMAT];
+    for (int m = 0; m < n; ++m) {
+        ev[m] = CPL(M_VOF, m, c); md[m] = CPL(M_DEN, m, c); me[m] = CPL(M_ENG, m, c);
+        sphi[m] = 0.0 + ev[m] * evol;
+        sphi[m + n] = 0.0 + ev[m] * evol * md[m];
+        sphi[m + 2 * n] = 0.0 + ev[m] * evol * me[m];
+    }
+    sphi[3 * n] = 0.0; sphi[3 * n + 1] = 0.0;
+    for (int q = 0; q < 4; ++q) sphi[3 * n + 2 + q] = 0.0 + emass * CPL(E_STE, q, c);
+    sphi[3 * n + 6] = emass * CPL(E_DAM, 0, c);
+    sphi[3 * n + 7] = emass * CPL(E_TPS, 0, c);
+    sphi[3 * n + 8] = emass * CPL(E_VIB, 0, c);
+    sphi[3 * n + 9] = emass * CPL(E_EJT, 0, c);
+    for (int q = 0; q < PL; ++q) sphi[q] += 1.0 * dphi[q];
+    double Z = 0.0;
+    emass = 0.;
+    for (int m = 0; m < n; ++m) {
+        if (sphi[m] < 0.) { sphi[m] = 0.; sphi[m + n] = 0.; }
+        if (m == 0) continue;
+        Z += sphi[m];
+        emass += sphi[m + n];
+    }
+    if (Z > evol * (1.0 - TOLVOF) || (Z > 0. && sphi[0] < evol * TOLVOF)) {
+        double s1 = 1.0 / Z;
+        for (int m = 0; m < n; ++m) ev[m] = s1 * sphi[m];
+        ev[0] = 0.0;
+    } else if (Z < evol * TOLVOF) {
+        for (int m = 0; m < n; ++m) ev[m] = 0.0;
+        ev[0] = 1.0;
+    } else if (P.partpres) {
+        double s1 = 1.0 / (Z + sphi[0]);
+        for (int m = 0; m < n; ++m) ev[m] = s1 * sphi[m];
+    } else {
+        sphi[0] = evol - Z;
+        double s1 = 1.0 / evol;
+        for (int m = 0; m < n; ++m) ev[m] = s1 * sphi[m];
+    }
+    for (int m = 1; m < n; ++m) {
+        if (ev[m] < TOLVOF || sphi[m + n] < TOLRHO * TOLVOF * evol) {
+            me[m] = 0.0; md[m] = 0.0; ev[m] = 0.0;
+        } else {
+            me[m] = sphi[m + 2 * n] / (ev[m] * evol);
+            md[m] = sphi[m + n] / (ev[m] * evol);
+        }
+    }
+    double dv0, dv1, ste[4], dam, tps, vib, ejt;
+    if (emass > TOLRHO * evol) {
+        double s1 = 1.0 / emass;
+        dv0 = s1 * sphi[3 * n]; dv1 = s1 * sphi[3 * n + 1];
+        for (int q = 0; q < 4; ++q) ste[q] = s1 * sphi[3 * n + 2 + q];
+        dam = sphi[3 * n + 6] / emass; tps = sphi[3 * n + 7] / emass;
+        vib = sphi[3 * n + 8] / emass; ejt = sphi[3 * n + 9] / emass;
+    } else {
+        dv0 = dv1 = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
+        dam = tps = vib = ejt = 0.;
+    }
+    double eden = emass / evol;
+    tps = dWind(tps, 0.0, 10.0);
+    dam = dWind(dam, 0.0, 1.0);
+    ejt = dWind(ejt, 0.0, 1.0);
+    if (eden < P.density_cutoff || eden < 10.0 * P.density_cutoff * (1.0 - ev[0])) {
+        for (int m = 0; m < n; ++m) { ev[m] = 0.0; md[m] = 0.0; me[m] = 0.0; }
+        ev[0] = 1.0;
+        dv0 = dv1 = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
+        dam = tps = vib = ejt = 0.;
+    }
+    for (int m = 0; m < n; ++m) { CPL(M_VOF, m, c) = ev[m]; CPL(M_DEN, m, c) = md[m]; CPL(M_ENG, m, c) = me[m]; }
+    CPL(E_DVEL, 0, c) = dv0; CPL(E_DVEL, 1, c) = dv1;
+    for (int q = 0; q < 4; ++q) CPL(E_STE, q, c) = ste[q];
+    CPL(E_DAM, 0, c) = dam; CPL(E_TPS, 0, c) = tps; CPL(E_VIB, 0, c) = vib; CPL(E_EJT, 0, c) = ejt;
+    CPL(E_DEN, 0, c) = eden;
+}
+void launch_advect(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *bx, const double *by)
+{
+    k_advect<<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, P, bx, by);
+}
+
+/* ---------------------------------------------------------- damp_on_cell -- */
+/* damp_on_cell_pml2 (src/velocity.c:487-588), _pml (:378-483), _sbc
+ * (:688-767); all cells including the ghost ring; inactive once t >= damp_time */
+__global__ void k_damp(Grid g, Fields F, Params P, const DevTime *tm)
+{
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= g.NC) return;
+    const double dt = tm->dt, t = tm->t;
+    if (P.damp_time <= t) return;
+    int n = g.nmat;
+    if (CPL(M_VOF, 0, c) > TOLVOF) return;
+    double sx = CPL(E_SIGMA, 0, c), sy = CPL(E_SIGMA, 1, c);
+    double adx = P.alpha_damp_x, ady = P.alpha_damp_y;
+    if (P.damp_type == SALEC_DAMP_PML2) {
+        if (!(sy > 0.0 || sx > 0.0)) return;
+        CPL(E_DVEL, 0, c) += -1.0 * (sx + sy) * CPL(E_VEL, 0, c) * dt;
+        CPL(E_DVEL, 1, c) += -1.0 * (sx + sy) * CPL(E_VEL, 1, c) * dt;
+        double g0 = CPL(E_GRAD_VEL, 0, c), g1 = CPL(E_GRAD_VEL, 1, c), g2 = CPL(E_GRAD_VEL, 2, c);
+        double g3 = CPL(E_GRAD_VEL, 3, c), g4 = CPL(E_GRAD_VEL, 4, c);
+        double a0 = CPL(E_ALPHA, 0, c), a1 = CPL(E_ALPHA, 1, c);
+        a0 += (sy * g0 + sy * g4 - ady * a0) * dt + (sx * g3 + sx * g4 - adx * a0) * dt;
+        double lp = CPL(E_PRE, 0, c) + CPL(E_Q, 0, c) - CPL(E_DAMPREF, 2, c);
+        double sd = (CPL(E_STE, 0, c) - lp) * sy * g0 + CPL(E_STE, 1, c) * (sy * g1 + sx * g2)
+                  + (CPL(E_STE, 2, c) - lp) * sx * g3 + (sy + sx) * CPL(E_STE, 3, c) * g4;
+        a1 += -1.0 * sd * dt - 1.0 * (adx + ady) * a1 * dt;
+        CPL(E_ALPHA, 0, c) = a0; CPL(E_ALPHA, 1, c) = a1;
+        for (int m = 1; m < n; ++m) {
+            if (CPL(M_VOF, m, c) < 1.0 - TOLVOF) continue;
+            double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
+            double dd = (-(sx + sy) * (CPL(M_DEN, m, c) - r0) - r0 * a0) * dt;
+            double de = (-(sx + sy) * (CPL(M_ENG, m, c) - r1) - a1) * dt;
+            CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
+            CPL(M_ENG, m, c) += de / CPL(M_PTY, m, c);
+            break;
+        }
+    } else if (P.damp_type == SALEC_DAMP_PML) {
+        if (sy > 0.0) {
+            double a1 = CPL(E_ALPHA, 1, c);
+            a1 += (sy * CPL(E_GRAD_VEL, 0, c) + sy * CPL(E_GRAD_VEL, 4, c) - ady * a1) * dt;
+            CPL(E_ALPHA, 1, c) = a1;
+            CPL(E_DVEL, 1, c) += -1.0 * sy * CPL(E_VEL, 1, c) * dt;
+            for (int m = 1; m < n; ++m) {
+                if (CPL(M_VOF, m, c) < 1.0 - TOLVOF) continue;
+                double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
+                double dd = (-sy * (CPL(M_DEN, m, c) - r0) - r0 * a1) * dt;
+                double de = (-sy * (CPL(M_ENG, m, c) - r1) - CPL(E_PRE, 0, c) * a1) * dt;
+                CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
+                CPL(M_ENG, m, c) += de / CPL(M_PTY, m, c);
+                break;
+            }
+        }
+        if (sx > 0.0) {
+            double a0 = CPL(E_ALPHA, 0, c);
+            a0 += (sx * CPL(E_GRAD_VEL, 3, c) + sx * CPL(E_GRAD_VEL, 4, c) - adx * a0) * dt;
+            CPL(E_ALPHA, 0, c) = a0;
+            CPL(E_DVEL, 0, c) += -1.0 * sx * CPL(E_VEL, 0, c) * dt;
+            double as = a0 + CPL(E_ALPHA, 1, c);
+            for (int m = 1; m < n; ++m) {
+                if (CPL(M_VOF, m, c) > 1.0 - TOLVOF) {
+                    double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
+                    double dd = (-sx * (CPL(M_DEN, m, c) - r0) - r0 * as) * dt;
+                    double de = (-sx * (CPL(M_ENG, m, c) - r1) - CPL(E_PRE, 0, c) * as) * dt;
+                    CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
+                    CPL(M_ENG, m, c) += de / CPL(M_PTY, m, c);
+                    break;
+                }
+            }
+        }
+    } else if (P.damp_type == SALEC_DAMP_SBC) {
+        if (sy < 1.0)
+            for (int m = 1; m < n; ++m) {
+                if (CPL(M_VOF, m, c) < TOLVOF) continue;
+                double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
+                double dd = CPL(M_DEN, m, c) - r0, de = CPL(M_ENG, m, c) - r1;
+                CPL(M_DEN, m, c) += dd * sy + r0;
+                CPL(M_ENG, m, c) += de * sy + r1;
+            }
+        if (sx < 1.0)
+            for (int m = 1; m < n; ++m) {
+                if (CPL(M_VOF, m, c) < TOLVOF) continue;
+                double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
+                double dd = CPL(M_DEN, m, c) - r0, de = CPL(M_ENG, m, c) - r1;
+                CPL(M_DEN, m, c) = dd * sx + r0;
+                CPL(M_ENG, m, c) = de * sx * sx + r1;
+            }
+    }
+}
+void launch_damp(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
+{
+    if (P.damp_type != SALEC_DAMP_PML && P.damp_type != SALEC_DAMP_PML2 && P.damp_type != SALEC_DAMP_SBC) return;
+    k_damp<<<nblk(g.NC, 256), 256, 0, s>>>(g, F, P, tm);
+}
+
+/* ---------------------------------------------------------- ejecta_flow -- */
+/* src/ejecta.c:65-134 */
+__global__ void k_ejecta_flow(Grid g, Fields F, Params P, const DevTime *tm)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    const double dt = tm->dt;
+    long c = k + (long)g.CW * j;
+    long v0 = k - 1 + (long)g.VW * (j - 1), v1 = k + (long)g.VW * (j - 1);
+    double gy = 0.;
+    gy += 0.5 * VPL(V_BF, 1, v0);
+    gy += 0.5 * VPL(V_BF, 1, v1);
+    if (CPL(E_POS, 1, c) <= F.f[SALEC_Y_PROFILE][k] && CPL(E_EJT, 0, c) > 0.1) {
+        if (P.flow_resistance > 0.) {
+            double pa = -P.flow_resistance * dt;
+            CPL(E_DVEL, 0, c) += pa * CPL(E_VEL, 0, c);
+            CPL(E_DVEL, 1, c) += pa * CPL(E_VEL, 1, c);
+        }
+        if (P.flow_friction > 0.) {
+            for (int q = 0; q < 4; ++q) CPL(E_STE, q, c) = 0.0;
+            if (CPL(E_VEL, 0, c) >= 0.) CPL(E_DVEL, 0, c) += -1.0 * P.flow_friction * fabs(gy) * dt;
+            else CPL(E_DVEL, 0, c) += 1.0 * P.flow_friction * fabs(gy) * dt;
+        }
+    }
+}
+void launch_ejecta_flow(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
+{
+    if (P.flow_resistance <= 0. && P.flow_friction <= 0.) return;
+    k_ejecta_flow<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, tm);
+}
+
+/* --------------------------------------------------------- update_state -- */
+/* src/sale2d.c:1635-1742 */
+__global__ void k_update_state(Grid g, Fields F, Params P, const DevMat *mats)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    long c = k + (long)g.CW * j;
+    int n = g.nmat;
+    double dam = CPL(E_DAM, 0, c);
+    double pre = 0.0, csd = 0.0, tem = 0.0, den = 0.0, mu = 0.0;
+    for (int m = 1; m < n; ++m) {
+        double vf = CPL(M_VOF, m, c), md = CPL(M_DEN, m, c);
+        double mp = 0., mc = 1., mt = 0.;
+        if (!(vf < TOLVOF)) {
+            const DevMat &r = mats[m];
+            double pmin = (1.0 - dam) * r.minPint + dam * r.minPdam;
+            eos_state_capped(r, md, CPL(M_ENG, m, c), pmin, CPL(M_PTY, m, c), &mt, &mp, &mc);
+        }
+        pre += vf * mp;
+        csd += vf / dMax(1.0, mc);
+        tem += vf * mt;
+        den += vf * md;
+        mu += vf / dMax(1.0, md * mc * mc * mats[m].GaCoef);
+    }
+    if (mu > 0.) mu = 1.0 / mu;
+    if (csd > 0.) csd = 1.0 / csd;
+    double v0 = CPL(M_VOF, 0, c);
+    if ((P.partpres == 0 && v0 > TOLVOF) || (P.partpres == 1 && v0 > 1.0 - TOLVOF)) {
+        pre = 0.; csd = 1.0; mu = 0.;
+    } else if (P.partpres == 1) {
+        csd = csd * (1.0 - v0); tem = tem / (1.0 - v0); mu = mu * (1.0 - v0);
+    }
+    CPL(E_PRE, 0, c) = pre; CPL(E_CSD, 0, c) = csd; CPL(E_TEM, 0, c) = tem; CPL(E_DEN, 0, c) = den; CPL(E_MU, 0, c) = mu;
+}
+void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats)
+{
+    k_update_state<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats);
+}
+
+/* update_cycle_control, src/sale2d.c:71-80 */
+__global__ void k_advance_time(DevTime *tm)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) { tm->t += tm->dt; tm->step += 1; }
+}
+void launch_advance_time(hipStream_t s, DevTime *tm) { k_advance_time<<<1, 64, 0, s>>>(tm); }
+
+/* ------------------------------------------------------- setup tail -- */
+/* record_avg_value, src/sale2d.c:1562-1627: dampref for every cell and the
+ * latest wave-arrival time over damping cells (max, reduced per block). */
+#define DR_BS 256
+__global__ void __launch_bounds__(DR_BS) k_dampref(Grid g, Fields F, double *part)
+{
+    __shared__ double red[DR_BS / 64];
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double ldt = 0.;
+    if (c < g.NC) {
+        CPL(E_DAMPREF, 0, c) = CPL(E_DEN, 0, c);
+        double e0 = CPL(E_POS, 0, c), e1 = CPL(E_POS, 1, c);
+        double l = 0.;
+        l += e0 * e0; l += e1 * e1;
+        double ld = sqrt(l);
+        double csd = CPL(E_CSD, 0, c);
+        if ((CPL(E_SIGMA, 0, c) > 0. || CPL(E_SIGMA, 1, c) > 0.) && csd > 2.0) {
+            double v = ld / csd;
+            ldt = ldt > v ? ldt : v;
+        }
+        double e = 0.0;
+        for (int m = 1; m < g.nmat; ++m) e += CPL(M_VOF, m, c) * CPL(M_ENG, m, c);
+        CPL(E_DAMPREF, 1, c) = e;
+        CPL(E_DAMPREF, 2, c) = CPL(E_PRE, 0, c);
+    }
+    for (int off = 32; off > 0; off >>= 1) { double o = __shfl_xor(ldt, off, 64); ldt = o > ldt ? o : ldt; }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ldt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = red[0];
+        for (int w = 1; w < DR_BS / 64; ++w) b = red[w] > b ? red[w] : b;
+        part[blockIdx.x] = b;
+    }
+}
+int dampref_blocks(const Grid &g) { return nblk(g.NC, DR_BS); }
+void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part, int nblocks)
+{
+    k_dampref<<<nblocks, DR_BS, 0, s>>>(g, F, part);
+}
+/* init_v_field, src/sale2d.c:1897-1956 */
+__global__ void k_init_v_field(Grid g, Fields F)
+{
+    int k, j;
+    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    long v = k + (long)g.VW * j;
+    long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
+    double u0 = 0.0, u1 = 0.0, em = 0., evv = 0.;
+    for (int i = 0; i < 4; i++) {
+        double lv = VPL(V_VOL, i + 1, v), d = CPL(E_DEN, 0, e[i]);
+        double pa = d * lv;
+        u0 += pa * CPL(E_VEL, 0, e[i]);
+        u1 += pa * CPL(E_VEL, 1, e[i]);
+        em += d * lv;
+        evv += lv;
+    }
+    if (em > TOLRHO * evv) {
+        double px = 1.0 / em;
+        u0 *= px; u1 *= px;
+        VPL(V_MASS, 0, v) = em; VPL(V_MASS, 1, v) = evv;
+    } else {
+        u0 = u1 = 0.0;
+        VPL(V_MASS, 0, v) = 0.; VPL(V_MASS, 1, v) = 0.;
+    }
+    VPL(V_VEL, 0, v) = u0; VPL(V_VEL, 1, v) = u1;
+}
+void launch_init_v_field(hipStream_t s, const Grid &g, const Fields &F)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    k_init_v_field<<<nblk(n, 256), 256, 0, s>>>(g, F);
+}
+
+/* first non-finite element (flat index + 1, 0 if none) */
+__global__ void k_nonfinite(const double *p, long n, unsigned long long *out)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = p[i];
+    if (!isfinite(x)) atomicMin(out, (unsigned long long)(i + 1));
+}
+void launch_nonfinite(hipStream_t s, const double *p, long n, unsigned long long *out)
+{
+    if (n <= 0) return;
+    k_nonfinite<<<nblk(n, 256), 256, 0, s>>>(p, n, out);
+}
+
+} // namespace salec

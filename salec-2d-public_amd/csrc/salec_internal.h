@@ -1,0 +1,87 @@
+/*
+ * salec_internal.h -- types shared by the C-ABI layer (capi.cpp) and the
+ * HIP kernels (kernels.hip).  Not part of the public ABI.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/salec.h"
+
+namespace salec {
+
+/* material parameters as the kernels read them (state_reference + tables) */
+struct DevMat {
+    int eos_type, yfunc, dfunc, tfunc;
+    int till_nden, tab_nden, tab_ntem, PolyLiq, PolySol, pad_;
+    double TLRho0, TLCv, TLA, TLB, TLE0, TLa, TLb, TLAlpha, TLBeta, TLEiv, TLEcv, TLTref;
+    double till_stepden;
+    const double *till_yeng, *tab_yden, *tab_xtem, *tab_data;
+    double VaporDen, MeltDen, GaCoef;
+    double Yint0, Yfricint, Ylimint, Ydam0, Yfricdam, Ylimdam, Yten0;
+    double IvanA, IvanB, IvanC, Pbd, Pbp, minPint, minPdam;
+    double JCA, JCB, JCN, JCC, JCM, JCTREF, JCMaxPlastic;
+    double Asimon, Csimon, Tmelt0, Tfrac, Tdelta;
+    double Tlidc1, Tlidc2, Tlidc3, Tsolc1, Tsolc2, Tsolc3;
+    double Toff, Cvib, VibMax, Tdamp, Pvlim, Acvis;
+    double alpha0, chi, gamfrac, melt_viscosity;
+};
+
+struct Grid {
+    int nx, ny, nmat, PL;           /* PL = face packet length 3*nmat+10    */
+    int CW, CH, VW, VH, BXW, BXH, BYW, BYH;
+    long NC, NV, NBX, NBY;
+};
+
+struct Params {
+    double avis1, avis2, anc, density_cutoff, velocity_max, velocity_min, acceleration_min;
+    double alpha_damp_x, alpha_damp_y, damp_time, threshold_z, flow_resistance, flow_friction, max_dt;
+    int partpres, tensile_failure, acoustic_fluid, damp_type;
+};
+
+struct Fields { double *f[SALEC_NFIELDS]; };
+
+struct DevTime { double dt; double t; long long step; double pad; };
+
+/* ghost / restriction program: dst <- mode(src), applied to every component
+ * of a field slot; modes: 0 x, 1 -x, 2 |x|, 3 -|x|, 4 +0, 5 -0 */
+enum { GM_COPY = 0, GM_NEG, GM_ABS, GM_NABS, GM_ZERO, GM_NZERO };
+struct GhostOp { int slot; int dst; int src; uint8_t m0, m1, mr, pad; };
+struct GhostSlots { double *base[4]; long plane[4]; int ncomp[4]; };
+
+struct Tracers {
+    long n;
+    double *pos, *vel;
+    float *mpre, *mtem;
+    int *gx, *gy, *tag, *matid, *idx, *idy, *alive;
+};
+
+/* launchers (kernels.hip) */
+void launch_trace_profile(hipStream_t s, const Grid &g, const Fields &F, const Params &P, double *part, int nchunk);
+void launch_profile_finish(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *part, int nchunk);
+void launch_ghost(hipStream_t s, const GhostSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase, int maxc);
+void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const Params &P);
+void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F);
+void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F);
+void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const Params &P, DevTime *tm,
+                       double *part, unsigned *counter, int nblocks);
+int  gradvel_dt_blocks(const Grid &g);
+void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm);
+void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
+void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const DevTime *tm);
+void launch_energy(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm);
+void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const Tracers &T,
+                    const double *vx, const double *vy, const DevTime *tm);
+void launch_flux(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by, int ring);
+void launch_advect(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *bx, const double *by);
+void launch_damp(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
+void launch_ejecta_flow(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
+void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats);
+void launch_advance_time(hipStream_t s, DevTime *tm);
+/* setup tail (src/salec2d.c:76-94) */
+void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part, int nblocks);
+int  dampref_blocks(const Grid &g);
+void launch_init_v_field(hipStream_t s, const Grid &g, const Fields &F);
+/* non-finite scan: writes first bad flat index per field into out */
+void launch_nonfinite(hipStream_t s, const double *p, long n, unsigned long long *out);
+
+} // namespace salec

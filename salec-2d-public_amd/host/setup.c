@@ -1,0 +1,1115 @@
+/*
+ * setup.c -- host-side initialisation of a run (see setup.h for the map to
+ * the reference's files).  Plain C, compiled with -ffp-contract=off.
+ */
+#include "setup.h"
+#include "inp.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <strings.h>
+
+#define TOLVOF 1.0e-6
+#define NPAD 2
+
+static double Wind(double x, double l, double r) { return x < l ? l : (x > r ? r : x); }
+static double Max(double a, double b) { return a > b ? a : b; }
+static double Min(double a, double b) { return a < b ? a : b; }
+static double Sqrt_trunc(double x) { return x < 1.0e-2 ? -sqrt(fabs(x)) : sqrt(x); }
+
+struct salec_host {
+    salec_config cfg;
+    salec_material mat[SALEC_MAXMAT];
+    char name[SALEC_MAXMAT][80];
+    double *till_yeng[SALEC_MAXMAT];
+    double *tab_yden[SALEC_MAXMAT], *tab_xtem[SALEC_MAXMAT], *tab_data[SALEC_MAXMAT];
+    /* Tillotson constants used to invert (T,P) -> (rho,e) at setup */
+    double inv_p[SALEC_MAXMAT][12];
+    double *inv_yeng[SALEC_MAXMAT];
+    int inv_nden[SALEC_MAXMAT];
+    /* mesh */
+    int nx, ny;
+    double dx, dy, ox, oy, x_ext, y_ext;
+    int nx_ext[2], ny_ext[2];
+    int damp_x, damp_y, damp_x_left, damp_y_top;
+    double lambda_x, lambda_y, damp_t_in;
+    double anc_depth, anc_width;
+    char anc_pattern[64];
+    double gravity[2];
+    int tracer_strip;
+    /* cycle */
+    double time0, time1, maxdt, dt0;
+    int maxstep;
+    /* projectile */
+    int p_mat;
+    double p_radius, p_center[2], p_vel[2], p_dam, p_pre, p_tem;
+    /* target */
+    int t_plane, n_layer, t_mat[10], tem_type, n_profile;
+    double t_depth[10], max_depth, surf_pre, surf_tem, surf_dtdz, r_planet;
+    /* fields */
+    double *f[SALEC_NFIELDS];
+    double x0, y0;
+};
+
+/* ------------------------------------------------------- Tillotson (host) */
+typedef struct { const double *p; const double *yeng; int nden; double step; } till_t;
+enum { RHO0, CV, TA, TB, E0, La, Lb, ALPHA, BETA_T, EIV, ECV, TREF };
+
+/* TillCold/TillHot/TillTran/TillPres, src/eos/eos_state.c:322-438 */
+static void till_pres(const double *t, double E, double R, double *P, double *Cs)
+{
+    double Eta = R / t[RHO0], Mu = Eta - 1.0;
+    double Phi0 = E / (t[E0] * Eta * Eta), Phi1 = 1.0 / (Phi0 + 1.0);
+    double Xi = t[RHO0] / R - 1.0;
+    double B = Eta < 1.0 ? t[TA] : t[TB];
+    double Pc = (t[La] + t[Lb] * Phi1) * E * R + t[TA] * Mu + B * Mu * Mu;
+    double cc1 = (1.0 / t[RHO0]) * (t[TA] + 2.0 * B * Mu) + E * (t[La] + t[Lb] * Phi1 * (3.0 - 2.0 * Phi1));
+    if ((R >= t[RHO0]) || (E < t[EIV])) {
+        *P = Pc;
+        *Cs = Sqrt_trunc(cc1 + (t[La] + t[Lb] * Phi1 * Phi1) * (*P) / R);
+    } else {
+        double A2 = 0.0, B1 = 0.0;
+        if (t[ALPHA] * Xi * Xi < 100.0) A2 = exp(-1.0 * t[ALPHA] * Xi * Xi);
+        if (t[BETA_T] * Xi < 100.0) B1 = exp(-1.0 * t[BETA_T] * Xi);
+        double Ph = t[La] * R * E + (t[Lb] * E * R * Phi1 + t[TA] * Mu * B1) * A2;
+        double ch1 = t[La] * E + t[Lb] * E * Phi1 * (3.0 + 2.0 * t[ALPHA] * Xi / Eta - 2.0 * Phi1) * A2
+                   + t[TA] / t[RHO0] * (1.0 - (Xi / Eta) * (t[BETA_T] + 2.0 * t[ALPHA] * Xi)) * A2 * B1;
+        if (E > t[ECV]) {
+            *P = Ph;
+            *Cs = Sqrt_trunc(ch1 + (t[La] + t[Lb] * Phi1 * Phi1 * A2) * (*P) / R);
+        } else {
+            double ET = t[ECV] - t[EIV], dEr = E - t[EIV], dEl = t[ECV] - E;
+            *P = (dEl * Pc + dEr * Ph) / ET;
+            double c2c = cc1 + (t[La] + t[Lb] * Phi1 * Phi1) * (*P) / R;
+            double c2h = ch1 + (t[La] + t[Lb] * Phi1 * Phi1 * A2) * (*P) / R;
+            double s1 = (dEl * c2c + dEr * c2h) / ET;
+            double s2 = (Ph - Pc) / ET * (*P) / (R * R);
+            *Cs = Sqrt_trunc(s1 + s2);
+        }
+    }
+    if (*P <= 1.0e-2) *Cs = Sqrt_trunc(t[TA] / t[RHO0]);
+}
+
+/* TillColdEnergy (RK4), src/eos/eos_state.c:463-517 */
+static double *till_cold_energy(const double *t, int nDen)
+{
+    double MaxDen = t[RHO0] * 10.0, Step = MaxDen / (nDen - 1);
+    double *x = (double *)malloc(sizeof(double) * nDen), *y = (double *)malloc(sizeof(double) * nDen);
+    if (!x || !y) { free(x); free(y); return NULL; }
+    for (int k = 0; k < nDen; k++) { x[k] = k * Step; y[k] = 0.0; }
+    int IM = (int)floor(t[RHO0] / MaxDen * nDen);
+    x[IM] = t[RHO0]; y[IM] = 0.0;
+    double P, C;
+#define F(E, R) (till_pres(t, (E), (R), &P, &C), P / ((R) * (R)))
+    for (int k = IM + 1; k < nDen; k++) {
+        double h = x[k] - x[k - 1];
+        double K1 = F(y[k - 1], x[k - 1]);
+        double K2 = F(y[k - 1] + 0.50 * K1 * h, x[k - 1] + 0.50 * h);
+        double K3 = F(y[k - 1] + 0.50 * K2 * h, x[k - 1] + 0.50 * h);
+        double K4 = F(y[k - 1] + K3 * h, x[k - 1] + h);
+        y[k] = y[k - 1] + (K1 + 2.0 * K2 + 2.0 * K3 + K4) * h / 6.0;
+    }
+    for (int k = IM - 1; k >= 1; k--) {
+        double h = x[k] - x[k + 1];
+        double K1 = F(y[k + 1], x[k + 1]);
+        double K2 = F(y[k + 1] + 0.50 * K1 * h, x[k + 1] + 0.50 * h);
+        double K3 = F(y[k + 1] + 0.50 * K2 * h, x[k + 1] + 0.50 * h);
+        double K4 = F(y[k + 1] + K3 * h, x[k + 1] + h);
+        y[k] = y[k + 1] + (K1 + 2.0 * K2 + 2.0 * K3 + K4) * h / 6.0;
+    }
+#undef F
+    y[0] = y[1];
+    free(x);
+    return y;
+}
+
+/* TillTemp, src/eos/eos_state.c:519-526 */
+static double till_temp(const double *t, const double *yeng, int nden, double E, double R)
+{
+    double step = t[RHO0] * 10.0 / (nden - 1);
+    int DL = (int)Wind(floor(R / step), 0, nden - 2);
+    double Dxl = DL + 1.0 - R / step, Dxr = 1.0 - Dxl;
+    double rE = Dxl * yeng[DL] + Dxr * yeng[DL + 1];
+    return Max((E - rE) / t[CV] + t[TREF], 0.0);
+}
+
+/* TillEOSInterpolateTP, src/eos/eos_state.c:528-579: which = -1 density,
+ * 0 energy density */
+static double till_interp_tp(const double *t, const double *yeng, int nden, double T, double Pt, int which)
+{
+    double step = t[RHO0] * 10.0 / (nden - 1);
+    double ThE = (T - t[TREF]) * t[CV];
+    int IM = (int)floor(t[RHO0] / (t[RHO0] * 10.0) * nden);
+#define XD(k) ((k) == IM ? t[RHO0] : (k) * step)
+    int DI = 0, DJ = nden - 1;
+    double P, C;
+    while (DI + 1 < DJ) {
+        int DM = (DI + DJ) / 2;
+        till_pres(t, yeng[DM] + ThE, XD(DM), &P, &C);
+        if (P > Pt) DJ = DM; else DI = DM;
+    }
+    double LI = XD(DI), RJ = XD(DJ), LM = 0, LE = 0;
+    for (int k = 0; k < 50; k++) {
+        double err = (RJ - LI) / RJ;
+        LM = (LI + RJ) / 2.0;
+        if (err < 1e-12) break;
+        double Dxl = (XD(DJ) - LM) / step, Dxr = 1.0 - Dxl;
+        LE = Dxl * yeng[DI] + Dxr * yeng[DJ] + ThE;
+        till_pres(t, LE, LM, &P, &C);
+        if (P > Pt) RJ = LM; else LI = LM;
+    }
+#undef XD
+    return which == -1 ? LM : LE * LM;
+}
+
+/* ------------------------------------------------------------ EOS files */
+static int load_tillotson(salec_host *h, int m, const char *path)
+{
+    inp_file *f = inp_open(path, 1);
+    if (!f) return SALEC_EINVAL;
+    double *p = h->inv_p[m];
+    p[RHO0] = inp_d(f, "Tillostson.Rho0", 2.7e3);
+    p[CV] = inp_d(f, "ColdEnergy.Cv", 8.96e2);
+    p[TA] = inp_d(f, "Tillostson.BulkA", 7.52e10);
+    p[TB] = inp_d(f, "Tillostson.BulkB", 6.50e10);
+    p[E0] = inp_d(f, "Tillostson.E0", 5.0e6);
+    p[La] = inp_d(f, "Tillostson.Ta", 0.50);
+    p[Lb] = inp_d(f, "Tillostson.Tb", 1.63);
+    p[ALPHA] = inp_d(f, "Tillostson.Alpha", 5.0);
+    p[BETA_T] = inp_d(f, "Tillostson.Beta", 5.0);
+    p[EIV] = inp_d(f, "Tillostson.Eiv", 3.00e6);
+    p[ECV] = inp_d(f, "Tillostson.Ecv", 1.39e7);
+    p[TREF] = inp_d(f, "ColdEnergy.Tref", 293.0);
+    int nden = inp_i(f, "ColdEnergy.RhoStep", 1501);
+    inp_close(f);
+    h->inv_nden[m] = nden;
+    h->inv_yeng[m] = till_cold_energy(p, nden);
+    if (!h->inv_yeng[m]) return SALEC_ENOMEM;
+    salec_material *M = &h->mat[m];
+    M->eos_type = SALEC_EOS_TILLOTSON;
+    M->TLRho0 = p[RHO0]; M->TLCv = p[CV]; M->TLA = p[TA]; M->TLB = p[TB]; M->TLE0 = p[E0];
+    M->TLa = p[La]; M->TLb = p[Lb]; M->TLAlpha = p[ALPHA]; M->TLBeta = p[BETA_T];
+    M->TLEiv = p[EIV]; M->TLEcv = p[ECV]; M->TLTref = p[TREF];
+    M->till_nden = nden;
+    M->till_stepden = p[RHO0] * 10.0 / (nden - 1);
+    h->till_yeng[m] = h->inv_yeng[m];
+    M->till_yeng = h->till_yeng[m];
+    /* TillInitStateRef, src/eos/eos_state.c:581-594 */
+    M->MeltDen = 0.85 * p[RHO0];
+    M->VaporDen = 1.0e-4 * p[RHO0] * 10.0;
+    M->den0 = p[RHO0];
+    M->gamfrac = (p[Lb] + p[La]) / (p[TA] / p[RHO0]);
+    return SALEC_OK;
+}
+
+/* LoadANEOSSieDen layout, src/eos/aneos_sieden.c:11-73; header line 2 of
+ * tables written by salec_write_sd_table carries the Tillotson constants. */
+static int load_sd_table(salec_host *h, int m, const char *path)
+{
+    FILE *fp = fopen(path, "r");
+    if (!fp) return SALEC_EINVAL;
+    char line[4096];
+    int have_inv = 0;
+    for (int l = 0; l < 3; ++l) {
+        if (!fgets(line, sizeof line, fp)) { fclose(fp); return SALEC_EINVAL; }
+        double *p = h->inv_p[m];
+        int rs = 0;
+        if (sscanf(line, "# tillotson %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %d",
+                   p + 0, p + 1, p + 2, p + 3, p + 4, p + 5, p + 6, p + 7, p + 8, p + 9, p + 10, p + 11, &rs) == 13) {
+            have_inv = 1;
+            h->inv_nden[m] = rs;
+        }
+    }
+    int nden, nene;
+    double pn, tn;
+    if (fscanf(fp, "%d %d", &nden, &nene) != 2 || fscanf(fp, "%le %le", &pn, &tn) != 2) { fclose(fp); return SALEC_EINVAL; }
+    double *xt = (double *)malloc(sizeof(double) * nene), *yd = (double *)malloc(sizeof(double) * nden);
+    double *dd = (double *)malloc(sizeof(double) * 3 * (size_t)nden * nene);
+    if (!xt || !yd || !dd) { fclose(fp); free(xt); free(yd); free(dd); return SALEC_ENOMEM; }
+    int ok = 1;
+    for (int i = 0; i < nene && ok; i++) ok = fscanf(fp, "%le", xt + i) == 1;
+    for (int j = 0; j < nden && ok; j++) ok = fscanf(fp, "%le", yd + j) == 1;
+    for (int j = 0; j < nden && ok; j++)
+        for (int i = 0; i < nene && ok; i++) {
+            double *q = dd + ((size_t)i * nden + j) * 3;
+            ok = fscanf(fp, "%le %le %le", q, q + 1, q + 2) == 3;
+        }
+    fclose(fp);
+    if (!ok) { free(xt); free(yd); free(dd); return SALEC_EINVAL; }
+    salec_material *M = &h->mat[m];
+    M->eos_type = SALEC_EOS_ANEOS_SD;
+    M->tab_nden = nden; M->tab_ntem = nene;
+    h->tab_yden[m] = yd; h->tab_xtem[m] = xt; h->tab_data[m] = dd;
+    M->tab_yden = yd; M->tab_xtem = xt; M->tab_data = dd;
+    if (!have_inv) return SALEC_EINVAL;  /* setup inversion needs Tillotson constants */
+    h->inv_yeng[m] = till_cold_energy(h->inv_p[m], h->inv_nden[m]);
+    if (!h->inv_yeng[m]) return SALEC_ENOMEM;
+    /* Dnorm-based thresholds of ANEOSSieDenInitStateRef (src/eos/aneos_sieden.c:239-240);
+     * Dnorm is the reference density of the Tillotson constants the table was filled from. */
+    double Dn = h->inv_p[m][RHO0];
+    M->MeltDen = 0.85 * Dn;
+    M->VaporDen = 0.04 * Dn;
+    M->den0 = Dn;
+    M->gamfrac = (h->inv_p[m][Lb] + h->inv_p[m][La]) / (h->inv_p[m][TA] / Dn);
+    return SALEC_OK;
+}
+
+int salec_write_tillotson(const char *path, const double p[12], int rhostep)
+{
+    FILE *fp = fopen(path, "w");
+    if (!fp) return SALEC_EINVAL;
+    fprintf(fp, "[Tillostson]\n");
+    fprintf(fp, "Rho0 = %.17g\nBulkA = %.17g\nBulkB = %.17g\nE0 = %.17g\nTa = %.17g\nTb = %.17g\n",
+            p[RHO0], p[TA], p[TB], p[E0], p[La], p[Lb]);
+    fprintf(fp, "Alpha = %.17g\nBeta = %.17g\nEiv = %.17g\nEcv = %.17g\n", p[ALPHA], p[BETA_T], p[EIV], p[ECV]);
+    fprintf(fp, "[ColdEnergy]\nCv = %.17g\nTref = %.17g\nRhoStep = %d\n", p[CV], p[TREF], rhostep);
+    fclose(fp);
+    return SALEC_OK;
+}
+
+int salec_write_sd_table(const char *path, const double p[12], int rhostep, int nden, int neng,
+                         double rho_min, double rho_max, double e_min, double e_max)
+{
+    double *yeng = till_cold_energy(p, rhostep);
+    if (!yeng) return SALEC_ENOMEM;
+    FILE *fp = fopen(path, "w");
+    if (!fp) { free(yeng); return SALEC_EINVAL; }
+    fprintf(fp, "# ANEOS (rho,e) table filled from the Tillotson EOS\n");
+    fprintf(fp, "# tillotson %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %d\n",
+            p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10], p[11], rhostep);
+    fprintf(fp, "# columns: temperature pressure sound-speed\n");
+    fprintf(fp, "%d %d\n%.17e %.17e\n", nden, neng, 1.0e5, 293.0);
+    double *xe = (double *)malloc(sizeof(double) * neng), *yd = (double *)malloc(sizeof(double) * nden);
+    /* energy axis: a node at e = 0, then log-spaced [e_min, e_max]; the
+     * lookup extrapolates asymmetrically below its first node
+     * (src/eos/aneos_sieden.c:275-279), so the axis must start at zero */
+    xe[0] = 0.0;
+    for (int i = 1; i < neng; ++i) xe[i] = e_min * pow(e_max / e_min, (double)(i - 1) / (neng - 2));
+    for (int j = 0; j < nden; ++j) yd[j] = rho_min * pow(rho_max / rho_min, (double)j / (nden - 1));
+    for (int i = 0; i < neng; ++i) fprintf(fp, "%.17e\n", xe[i]);
+    for (int j = 0; j < nden; ++j) fprintf(fp, "%.17e\n", yd[j]);
+    for (int j = 0; j < nden; ++j)
+        for (int i = 0; i < neng; ++i) {
+            double P, C;
+            till_pres(p, xe[i], yd[j], &P, &C);
+            double T = till_temp(p, yeng, rhostep, xe[i], yd[j]);
+            fprintf(fp, "%.17e %.17e %.17e\n", T, P, C);
+        }
+    for (int j = 0; j < nden; ++j)
+        for (int i = 0; i < neng; ++i) fprintf(fp, "0.0 0.0\n");
+    fclose(fp);
+    free(xe); free(yd); free(yeng);
+    return SALEC_OK;
+}
+
+/* ----------------------------------------------------------------- load */
+static int fn_by_name(const char *s)
+{
+    static const char *names[SALEC_FN_COUNT] = {"SimpleRock", "SimpleShear", "JohnsonCook1", "JohnsonCook2",
+        "RockStrengthT", "IceStrength", "Collins", "Ivanov", "SimpleTensile", "None", "Wunnemann",
+        "VonMises1", "VonMises2"};
+    for (int k = 0; k < SALEC_FN_COUNT; ++k)
+        if (strcasecmp(s, names[k]) == 0) return k;
+    return -1;
+}
+
+static int boundary_str2opt(const char *s, int is_lr)
+{
+    /* src/sale2d_back.c:405-428 */
+    if (strcasecmp(s, "outflow") == 0) return SALEC_GHT_OUTFLOW;
+    if (strcasecmp(s, "fixed") == 0) return SALEC_GHT_FIXED;
+    if (strcasecmp(s, "freeslip") == 0) return is_lr ? SALEC_GHT_REVERSE_VEL_X : SALEC_GHT_REVERSE_VEL_Y;
+    return SALEC_GHT_ZERO;
+}
+
+#define ERR(...) do { if (err) snprintf(err, errlen, __VA_ARGS__); salec_host_free(h); return rc; } while (0)
+
+int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out, char *err, int errlen)
+{
+    int rc = SALEC_EINVAL;
+    salec_host *h = (salec_host *)calloc(1, sizeof(salec_host));
+    if (!h) return SALEC_ENOMEM;
+    inp_file *f = inp_open(inp_path, 0);
+    if (!f) ERR("cannot open %s", inp_path);
+    char buf[160];
+    /* mesh, src/sale2d_back.c:71-97 */
+    h->dx = inp_d(f, "mesh.dx", 1.); h->dy = inp_d(f, "mesh.dy", 1.);
+    h->oy = inp_d(f, "mesh.oy", 0.7); h->ox = inp_d(f, "mesh.ox", 0.0);
+    h->nx = inp_i(f, "mesh.npx", 32); h->ny = inp_i(f, "mesh.npy", 32);
+    inp_s(f, "mesh.ext", buf, sizeof buf, "off");
+    if (strcasecmp(buf, "off") == 0) {
+        h->x_ext = h->y_ext = -1.0;
+    } else {
+        h->x_ext = inp_dk(f, "mesh.ex", 0, 1.05); h->nx_ext[0] = inp_ik(f, "mesh.ex", 1, 0); h->nx_ext[1] = inp_ik(f, "mesh.ex", 2, 0);
+        h->y_ext = inp_dk(f, "mesh.ey", 0, 1.05); h->ny_ext[0] = inp_ik(f, "mesh.ey", 1, 0); h->ny_ext[1] = inp_ik(f, "mesh.ey", 2, 0);
+    }
+    salec_config *c = &h->cfg;
+    c->nx = h->nx; c->ny = h->ny;
+    /* numerical, src/sale2d_back.c:100-165 */
+    c->avis2 = inp_d(f, "numerical.AVISQ", 1.2);
+    c->avis1 = inp_d(f, "numerical.AVISL", 0.5);
+    c->anc = inp_d(f, "numerical.ANC", 0.1);
+    h->anc_depth = inp_d(f, "numerical.ANCD", 2.);
+    h->anc_width = inp_d(f, "numerical.ANCW", 3.);
+    inp_s(f, "numerical.ANCPAT", h->anc_pattern, sizeof h->anc_pattern, "pml");
+    double ej_thr = inp_d(f, "ejecta.threshold", 2.);
+    c->flow_resistance = inp_d(f, "ejecta.resistance", -1.0);
+    c->flow_friction = inp_d(f, "ejecta.friction", 0.0);
+    c->density_cutoff = inp_d(f, "numerical.MinDensity", 1.0e-2);
+    double vmax = inp_d(f, "numerical.MaxVelocity", -1.63);
+    double vmin = inp_d(f, "numerical.MinVelocity", -1e-8);
+    double amin = inp_d(f, "numerical.MinAcceleration", -1e-3);
+    c->partpres = inp_i(f, "numerical.partpres", 0);
+    c->tensile_failure = inp_i(f, "numerical.tensile_failure", 0);
+    inp_s(f, "numerical.damping", buf, sizeof buf, "off");
+    h->damp_x = inp_i(f, "numerical.DampX", 0); h->damp_y = inp_i(f, "numerical.DampY", 0);
+    h->lambda_x = inp_d(f, "numerical.LambdaX", 10); h->lambda_y = inp_d(f, "numerical.LambdaY", 10);
+    h->damp_t_in = inp_d(f, "numerical.damp_time", -1.0);
+    if (h->damp_t_in < 0.) h->damp_t_in = inp_d(f, "cycle.time1", 100.0);
+    h->damp_x_left = inp_i(f, "numerical.DampX_Left", 0);
+    h->damp_y_top = inp_i(f, "numerical.DampY_Right", 0);
+    if (!strcasecmp(buf, "on") || !strcasecmp(buf, "pml")) c->damp_type = SALEC_DAMP_PML;
+    else if (!strcasecmp(buf, "pml2")) c->damp_type = SALEC_DAMP_PML2;
+    else if (!strcasecmp(buf, "pml3")) c->damp_type = SALEC_DAMP_PML3;
+    else if (!strcasecmp(buf, "sbc")) { c->damp_type = SALEC_DAMP_SBC; h->lambda_x = Wind(h->lambda_x, 0.0, 0.5); h->lambda_y = Wind(h->lambda_y, 0.0, 0.5); }
+    else if (!strcasecmp(buf, "ctlg") || !strcasecmp(buf, "rock")) c->damp_type = SALEC_DAMP_CTLG;
+    else { c->damp_type = SALEC_DAMP_NONE; h->damp_x = h->damp_y = 0; h->lambda_x = h->lambda_y = 0.; h->damp_t_in = -1.0; h->damp_x_left = h->damp_y_top = 0; }
+    /* cycle, src/sale2d.c:6-13 */
+    h->time0 = inp_d(f, "cycle.time0", 0.0); h->time1 = inp_d(f, "cycle.time1", 100.0);
+    h->maxdt = inp_d(f, "cycle.maxdt", 1.0); h->dt0 = inp_d(f, "cycle.dt0", 0.0);
+    h->maxstep = inp_i(f, "cycle.maxstep", 200);
+    c->max_dt = h->maxdt;
+    /* materials */
+    c->nmat = inp_i(f, "material.nm", 2);
+    if (c->nmat < 2 || c->nmat > SALEC_MAXMAT) ERR("material.nm out of range");
+    for (int m = 0; m < c->nmat; ++m) {
+        inp_sk(f, "material.name", m, h->name[m], sizeof h->name[m], "vacuum_");
+        char post[64], path[1024];
+        inp_sk(f, "material.postfix", m, post, sizeof post, "null");
+        h->mat[m].eos_type = SALEC_EOS_NONE;
+        if (m == 0) continue;
+        if (!strcasecmp(post, "tillotson")) {
+            snprintf(path, sizeof path, "%s/%s.tillotson", eos_dir, h->name[m]);
+            if ((rc = load_tillotson(h, m, path)) != SALEC_OK) ERR("cannot load %s", path);
+        } else if (!strcasecmp(post, "aneos.sd")) {
+            snprintf(path, sizeof path, "%s/%s.aneos.sd", eos_dir, h->name[m]);
+            if ((rc = load_sd_table(h, m, path)) != SALEC_OK) ERR("cannot load %s", path);
+        } else {
+            rc = SALEC_EINVAL;
+            ERR("unsupported EOS postfix '%s' for %s", post, h->name[m]);
+        }
+    }
+    double pr_radius = inp_d(f, "projectile.radiu", 1.0);
+    /* load_state_reference, src/sale2d_back.c:250-402 */
+    for (int m = 1; m < c->nmat; ++m) {
+        salec_material *r = &h->mat[m];
+        double nu = inp_dk(f, "material.poisson", m, 0.25);
+        r->GaCoef = 1.5 * (1.0 - 2.0 * nu) / (1.0 + nu);
+        r->Yint0 = inp_dk(f, "material.yint0", m, 1.0);
+        r->Yfricint = inp_dk(f, "material.yintfri", m, 1.0);
+        r->Ylimint = inp_dk(f, "material.yintlim", m, 2.50e9);
+        r->Ydam0 = inp_dk(f, "material.ydam0", m, 1.0);
+        r->Yfricdam = inp_dk(f, "material.ydamfri", m, 1.0);
+        r->Ylimdam = inp_dk(f, "material.ydamlim", m, 2.50e9);
+        char yf[64], tf[64], df[64], pf[64];
+        inp_sk(f, "material.yshear", m, yf, sizeof yf, "SimpleRock");
+        r->Yten0 = inp_dk(f, "material.yten0", m, 3.0e8);
+        inp_sk(f, "material.ytens", m, tf, sizeof tf, "SimpleTensile");
+        r->IvanA = inp_dk(f, "material.IvanA", m, 1.0e-4);
+        r->IvanB = inp_dk(f, "material.IvanB", m, 1.0e-11);
+        r->IvanC = inp_dk(f, "material.IvanC", m, 3.0e8);
+        r->Pbd = inp_dk(f, "material.Pbd", m, 2.7e9);
+        r->Pbp = inp_dk(f, "material.Pbp", m, 4.1e9);
+        inp_sk(f, "material.damage", m, df, sizeof df, "SimpleShear");
+        r->JCA = inp_dk(f, "material.JcA", m, 4.90e7); r->JCB = inp_dk(f, "material.JcB", m, 1.57e8);
+        r->JCN = inp_dk(f, "material.JcN", m, 1.67e-1); r->JCC = inp_dk(f, "material.JcC", m, 1.60e-2);
+        r->JCM = inp_dk(f, "material.JcM", m, 1.70); r->JCTREF = inp_dk(f, "material.JcTref", m, 8.00e2);
+        r->JCMinP = inp_dk(f, "material.JcminP", m, -2.44e9);
+        r->JCMaxPlastic = inp_dk(f, "material.JCMaxPlastic", m, 10.0);
+        if (!strcasecmp(yf, "JohnsonCook2") || !strcasecmp(yf, "JohnsonCook1") ||
+            !strcasecmp(yf, "VonMises1") || !strcasecmp(yf, "VonMises2")) {
+            r->minPdam = r->JCMinP; r->minPint = r->JCMinP;
+        } else if (strstr(yf, "Rock") != NULL || !strcasecmp(yf, "IceStrength")) {
+            r->minPint = r->Yint0 / r->Yfricint * (r->Yint0 / r->Ylimint - 1.0);
+            r->minPdam = -1.0 * r->Ydam0 / r->Yfricdam;
+        } else {
+            r->minPdam = 0.; r->minPint = 0.;
+        }
+        r->Asimon = 1.0 / inp_dk(f, "material.SimonA", m, 6.00e9);
+        r->Csimon = 1.0 / inp_dk(f, "material.SimonC", m, 3.00);
+        r->Tmelt0 = inp_dk(f, "material.SimonT0", m, 933.0);
+        r->Tfrac = inp_dk(f, "material.OhnakaXi", m, 1.2);
+        r->Tdelta = inp_dk(f, "material.Tdelta", m, 0.0);
+        r->PolyLiq = inp_ik(f, "material.PolyLiq", m, 0);
+        r->Tlidc1 = inp_dk(f, "material.Tlidc1", m, 0.0); r->Tlidc2 = inp_dk(f, "material.Tlidc2", m, 0.0);
+        r->Tlidc3 = inp_dk(f, "material.Tlidc3", m, 0.0);
+        r->PolySol = inp_ik(f, "material.PolySol", m, 0);
+        r->Tsolc1 = inp_dk(f, "material.Tsolc1", m, 0.0); r->Tsolc2 = inp_dk(f, "material.Tsolc2", m, 0.0);
+        r->Tsolc3 = inp_dk(f, "material.Tsolc3", m, 0.0);
+        double bs = inp_dk(f, "material.BlockSize", m, -1.0);
+        if (bs < 0.) bs = pr_radius;
+        double ge = inp_dk(f, "material.GammaEta", m, 8.0e-3), gb = inp_dk(f, "material.GammaBeta", m, 1.15e2);
+        r->Toff = inp_dk(f, "material.Toff", m, 16.0);
+        r->Cvib = inp_dk(f, "material.Cvib", m, 0.1);
+        r->VibMax = inp_dk(f, "material.VibMax", m, 200.0);
+        r->Pvlim = inp_dk(f, "material.Pvlim", m, 2.50e10);
+        r->Acvis = ge * bs * 5000.0;
+        r->Tdamp = gb * bs / 5000.0;
+        r->melt_viscosity = inp_dk(f, "material.viscosityM", m, 0.0);
+        inp_sk(f, "material.Porosity", m, pf, sizeof pf, "None");
+        r->alpha0 = 1.0; r->chi = 0.;
+        r->yfunc = fn_by_name(yf); r->tfunc = fn_by_name(tf); r->dfunc = fn_by_name(df); r->pfunc = fn_by_name(pf);
+        /* str2func accepts registry indices > 0 only (src/strength.c:590) */
+        if (r->yfunc <= 0 || r->tfunc <= 0 || r->dfunc <= 0 || r->pfunc <= 0) {
+            rc = SALEC_ENOSYM;
+            ERR("undefined strength/damage/tensile/porosity function for material %d", m);
+        }
+        if (r->pfunc != SALEC_FN_NONE) { rc = SALEC_EINVAL; ERR("porosity model '%s' is not supported", pf); }
+    }
+    /* velocity / acceleration limits, src/sale2d.c:448-472 */
+    h->p_vel[0] = inp_dk(f, "projectile.velocity", 0, 0.0);
+    h->p_vel[1] = inp_dk(f, "projectile.velocity", 1, -1000.0);
+    h->gravity[1] = inp_dk(f, "condition.gravity", 1, -1.62);
+    h->gravity[0] = inp_dk(f, "condition.gravity", 0, 0.0);
+    double pv = sqrt(h->p_vel[0] * h->p_vel[0] + h->p_vel[1] * h->p_vel[1]);
+    double gv = sqrt(h->gravity[0] * h->gravity[0] + h->gravity[1] * h->gravity[1]);
+    c->velocity_max = vmax > 0. ? vmax : fabs(vmax) * pv;
+    c->velocity_min = vmin > 0. ? vmin : fabs(vmin) * pv;
+    c->acceleration_min = amin > 0. ? amin : fabs(amin) * gv;
+    /* boundary, src/sale2d_back.c:219-238 */
+    static const char *bkey[4] = {"boundary.left", "boundary.right", "boundary.bottom", "boundary.top"};
+    static const char *bdef[4] = {"freeslip", "outflow", "freeslip", "outflow"};
+    for (int k = 0; k < 4; ++k) {
+        inp_s(f, bkey[k], buf, sizeof buf, bdef[k]);
+        c->go_list[k] = boundary_str2opt(buf, k < 2);
+    }
+    if (h->ox < 1e-6) c->go_list[0] = SALEC_GHT_MIRROR;
+    h->tracer_strip = inp_i(f, "numerical.tarcer_strip", 2);
+    /* projectile, src/sale2d_back.c:29-62 */
+    inp_s(f, "projectile.material", buf, sizeof buf, "unknown");
+    h->p_mat = -1;
+    for (int m = 0; m < c->nmat; ++m) if (!strcasecmp(buf, h->name[m])) { h->p_mat = m; break; }
+    if (h->p_mat < 0) ERR("cannot get material id of projectile [%s]", buf);
+    h->p_radius = pr_radius;
+    h->p_center[0] = inp_dk(f, "projectile.center", 0, 0.0);
+    h->p_center[1] = inp_dk(f, "projectile.center", 1, 300.0);
+    h->p_dam = inp_dk(f, "projectile.damage", 1, 1.0);
+    h->p_pre = inp_dk(f, "projectile.pressure", 1, 1.0);
+    h->p_tem = inp_dk(f, "projectile.temperature", 1, 293.0);
+    c->threshold_z = ej_thr * h->p_radius;
+    /* target, src/target_profile.c:345-483 (plane; const/surface/conductive) */
+    h->n_profile = 3001;
+    h->n_layer = inp_i(f, "target.number", 1);
+    inp_s(f, "target.type", buf, sizeof buf, "plane");
+    if (strcasecmp(buf, "plane") != 0) ERR("target.type '%s' is not supported", buf);
+    h->t_plane = 1;
+    for (int l = 0; l < h->n_layer && l < 10; ++l) {
+        h->t_depth[l] = inp_dk(f, "target.depth", l, 1.);
+        char tn[80];
+        inp_sk(f, "target.material", l, tn, sizeof tn, "unknown");
+        h->t_mat[l] = -1;
+        for (int m = 0; m < c->nmat; ++m) if (!strcasecmp(tn, h->name[m])) { h->t_mat[l] = m; break; }
+        if (h->t_mat[l] < 0) ERR("cannot get material id of %s in target layer", tn);
+    }
+    h->max_depth = inp_d(f, "target.max_depth", 1.);
+    inp_sk(f, "target.temperature", 0, buf, sizeof buf, "const");
+    h->surf_pre = inp_d(f, "target.surf_pressure", 1.0e-5);
+    h->surf_tem = inp_d(f, "target.surf_temperature", -1.0);
+    if (h->surf_tem < 0.) h->surf_tem = inp_dk(f, "target.temperature", 1, 293.0);
+    if (!strcasecmp(buf, "const") || !strcasecmp(buf, "surface")) h->tem_type = 0;
+    else if (!strcasecmp(buf, "conductive")) { h->tem_type = 1; h->surf_dtdz = inp_d(f, "target.surf_dtdz", 0.0); h->r_planet = inp_d(f, "target.r_planet", 1738.0e3); }
+    else ERR("target temperature profile '%s' is not supported", buf);
+    if (inp_i(f, "target.addition", 0) > 0 || inp_has(f, "addition.number")) ERR("target additions are not supported");
+    inp_close(f);
+    f = NULL;
+    c->mat = h->mat;
+    c->damp_time = h->damp_t_in;
+    *out = h;
+    return SALEC_OK;
+}
+#undef ERR
+
+void salec_host_free(salec_host *h)
+{
+    if (!h) return;
+    for (int m = 0; m < SALEC_MAXMAT; ++m) {
+        if (h->inv_yeng[m] != h->till_yeng[m]) free(h->till_yeng[m]);
+        free(h->inv_yeng[m]);
+        free(h->tab_yden[m]); free(h->tab_xtem[m]); free(h->tab_data[m]);
+    }
+    for (int k = 0; k < SALEC_NFIELDS; ++k) free(h->f[k]);
+    free(h);
+}
+
+const salec_config *salec_host_config(const salec_host *h) { return &h->cfg; }
+double *salec_host_field(salec_host *h, int field) { return (field >= 0 && field < SALEC_NFIELDS) ? h->f[field] : NULL; }
+void salec_host_set_damp_time(salec_host *h, double t) { h->cfg.damp_time = t; }
+void salec_host_cycle(const salec_host *h, double out[6])
+{
+    out[0] = h->time0; out[1] = h->time1; out[2] = h->maxdt; out[3] = h->dt0; out[4] = h->maxstep; out[5] = h->damp_t_in;
+}
+
+/* ------------------------------------------------------------- geometry */
+/* Sk, Spacing2, src/construct.c:379-411 */
+static double Sk(int k, double ext) { return ext * (pow(ext, k) - 1.0) / (ext - 1.0); }
+static double Spacing2(int k, double ext, int eL, int eR, int dL, int dR, int N)
+{
+    double Xk = 0.;
+    if (k < dL) Xk = pow(ext, eL) * k;
+    else if (k < dL + eL) Xk = Sk(eL, ext) - Sk(dL + eL - k, ext) + pow(ext, eL) * dL;
+    else if (k < N - (dR + eR)) Xk = Sk(eL, ext) + pow(ext, eL) * dL + (k - (dL + eL));
+    else if (k < N - dR) Xk = Sk(eL, ext) + pow(ext, eL) * dL + (N - (dL + eL) - (dR + eR)) + Sk(k - (N - dR - eR), ext);
+    else Xk = Sk(eL, ext) + pow(ext, eL) * dL + (N - (dL + eL) - (dR + eR)) + Sk(eR, ext) + pow(ext, eR) * (k - (N - dR));
+    return Xk;
+}
+
+/* bilinear shape functions IpB = {lb, rb, ru, lu} and derivatives,
+ * src/geometry/interpolate_fn.c:22-83,249-258 */
+static double ipb(int k, const double *x)
+{
+    switch (k) {
+    case 0: return 0.25 * (1 - x[0]) * (1 - x[1]);
+    case 1: return 0.25 * (1 + x[0]) * (1 - x[1]);
+    case 2: return 0.25 * (1 + x[0]) * (1 + x[1]);
+    default: return 0.25 * (1 - x[0]) * (1 + x[1]);
+    }
+}
+static double ipb_d(int k, int dim, const double *x)
+{
+    if (dim == 0) {
+        switch (k) { case 0: return -0.25 * (1 - x[1]); case 1: return 0.25 * (1 - x[1]);
+                     case 2: return 0.25 * (1 + x[1]); default: return -0.25 * (1 + x[1]); }
+    }
+    switch (k) { case 0: return -0.25 * (1 - x[0]); case 1: return -0.25 * (1 + x[0]);
+                 case 2: return 0.25 * (1 + x[0]); default: return 0.25 * (1 - x[0]); }
+}
+static double interp4(const double *Xi, const double *xl)
+{
+    double r = 0;
+    for (int k = 0; k < 4; ++k) r += ipb(k, xl) * Xi[k];
+    return r;
+}
+#define GB 0.57735026918962576451
+static const double GIPS2d[4][2] = {{-GB, -GB}, {-GB, GB}, {GB, -GB}, {GB, GB}};
+/* jacobi2d, src/geometry/interpolate.c:52-60 */
+static double jacobi2d(const double *Xi, const double *xl)
+{
+    double xs[4] = {Xi[0], Xi[2], Xi[4], Xi[6]}, ys[4] = {Xi[1], Xi[3], Xi[5], Xi[7]};
+    double J[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) J[0] += ipb_d(k, 0, xl) * xs[k];
+    for (int k = 0; k < 4; ++k) J[1] += ipb_d(k, 1, xl) * xs[k];
+    for (int k = 0; k < 4; ++k) J[2] += ipb_d(k, 0, xl) * ys[k];
+    for (int k = 0; k < 4; ++k) J[3] += ipb_d(k, 1, xl) * ys[k];
+    return J[0] * J[3] - J[1] * J[2];
+}
+/* Integrate2d / Integrate2dcyl / Integrate2dcyl_Fdx, src/geometry/interpolate.c:62-185 */
+static double integrate2d(const double *Xi, const double *Yi)
+{
+    double r = 0;
+    for (int j = 0; j < 4; ++j) r += 1.0 * interp4(Yi, GIPS2d[j]) * jacobi2d(Xi, GIPS2d[j]);
+    return r;
+}
+static double integrate2dcyl(const double *Xi, const double *Yi)
+{
+    double r = 0;
+    for (int j = 0; j < 4; ++j) {
+        double xs[4] = {Xi[0], Xi[2], Xi[4], Xi[6]};
+        double lr = fabs(interp4(xs, GIPS2d[j]));
+        r += 1.0 * interp4(Yi, GIPS2d[j]) * jacobi2d(Xi, GIPS2d[j]) * lr;
+    }
+    return r;
+}
+static double integrate2dcyl_fdx(const double *Xi, const double *Fi, int dim)
+{
+    double lf[8];
+    memcpy(lf, Xi, sizeof lf);
+    for (int k = 0; k < 4; ++k) lf[k * 2 + dim] = Fi[k];
+    double r = 0;
+    for (int j = 0; j < 4; ++j) {
+        double xs[4] = {Xi[0], Xi[2], Xi[4], Xi[6]};
+        double lr = interp4(xs, GIPS2d[j]);
+        r += 1.0 * jacobi2d(lf, GIPS2d[j]) * fabs(lr);
+    }
+    return r;
+}
+/* generate2dcyl_suf_and_vol, src/construct.c:480-523 */
+static void gen_suf_vol(const double *pos, double *suf, double *vsub, double *vcyl)
+{
+    static const double lxl[4][4][2] = {
+        {{-1.0, -1.0}, {0.0, -1.0}, {0.0, 0.0}, {-1.0, 0.0}},
+        {{0.0, -1.0}, {1.0, -1.0}, {1.0, 0.0}, {0.0, 0.0}},
+        {{0.0, 0.0}, {1.0, 0.0}, {1.0, 1.0}, {0.0, 1.0}},
+        {{-1.0, 0.0}, {0.0, 0.0}, {0.0, 1.0}, {-1.0, 1.0}}};
+    static const double ones[4] = {1., 1., 1., 1.};
+    double xs[4] = {pos[0], pos[2], pos[4], pos[6]}, ys[4] = {pos[1], pos[3], pos[5], pos[7]};
+    double sv[4], sr[4];
+    double ser = pos[0] + pos[2] + pos[4] + pos[6];
+    for (int k = 0; k < 4; ++k) {
+        double xg[8];
+        for (int j = 0; j < 4; ++j) { xg[2 * j] = interp4(xs, lxl[k][j]); xg[2 * j + 1] = interp4(ys, lxl[k][j]); }
+        sr[k] = integrate2dcyl(xg, ones);
+        sv[k] = integrate2d(xg, ones);
+        if (ser < 0.0) sv[k] *= -1.0;
+        /* k+1 left rotations of the polygon, then Polygon2dcyl over 3 points */
+        for (int r = 0; r < k + 1; ++r) {
+            double t0 = xg[0], t1 = xg[1];
+            for (int q = 0; q < 3; ++q) { xg[2 * q] = xg[2 * q + 2]; xg[2 * q + 1] = xg[2 * q + 3]; }
+            xg[6] = t0; xg[7] = t1;
+        }
+        double p[2] = {0., 0.};
+        for (int q = 1; q < 3; ++q) {
+            const double *a = xg + 2 * (q - 1), *b = xg + 2 * q;
+            double r0 = fabs((b[0] + a[0]) / 2.0);
+            p[0] += 1.0 * (-1.0 * (b[1] - a[1]) * r0);
+            p[1] += 1.0 * ((b[0] - a[0]) * r0);
+        }
+        suf[2 * k] = p[0]; suf[2 * k + 1] = p[1];
+    }
+    for (int k = 0; k < 4; ++k) sr[k] = fabs(sr[k]);
+    *vcyl = fabs(sr[0]) + fabs(sr[1]) + fabs(sr[2]) + fabs(sr[3]);
+    for (int k = 0; k < 4; ++k) { vsub[k] = sv[k]; vsub[4 + k] = sr[k]; }
+}
+/* generate2dcyl_grad, src/construct.c:525-556 */
+static void gen_grad(const double *pos, double *g)
+{
+    static const double ones[4] = {1., 1., 1., 1.};
+    double tv = integrate2dcyl(pos, ones);
+    for (int j = 0; j < 4; ++j) {
+        double cj[4] = {0., 0., 0., 0.};
+        cj[j] = 1.;
+        for (int k = 0; k < 2; ++k) g[4 * k + j] = integrate2dcyl_fdx(pos, cj, k);
+    }
+    double ser = pos[0] + pos[2] + pos[4] + pos[6];
+    for (int j = 0; j < 4; ++j) {
+        double cj[4] = {0., 0., 0., 0.};
+        cj[j] = 1.;
+        g[8 + j] = integrate2d(pos, cj);
+        if (ser < 0.) g[8 + j] *= -1.0;
+    }
+    double px = 1.0 / tv;
+    for (int k = 0; k < 12; ++k) g[k] *= px;
+}
+/* generate2d_center, src/construct.c:558-582 */
+static void gen_center(const double *pos, double *c)
+{
+    c[0] = c[1] = c[2] = c[3] = 0.0;
+    for (int k = 0; k < 4; ++k) { c[0] += 0.25 * pos[2 * k]; c[1] += 0.25 * pos[2 * k + 1]; }
+    for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 4; ++j) {
+            double ax = fabs(pos[2 * k] - pos[2 * j]), ay = fabs(pos[2 * k + 1] - pos[2 * j + 1]);
+            c[2] = ax > c[2] ? ax : c[2];
+            c[3] = ay > c[3] ? ay : c[3];
+        }
+}
+
+/* 9-node Lagrange element and Laplace2d9, src/geometry/interpolate_2d9.c:32-424 */
+static const double XL9[9][2] = {{-1., -1.}, {0., -1.}, {1., -1.}, {-1., 0.}, {0., 0.}, {1., 0.}, {-1., 1.}, {0., 1.}, {1., 1.}};
+static const int XLI9[9][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {3, 4, 5}, {4, 5, 3}, {5, 3, 4}, {6, 7, 8}, {7, 8, 6}, {8, 6, 7}};
+static const int XLJ9[9][3] = {{0, 3, 6}, {1, 4, 7}, {2, 5, 8}, {3, 0, 6}, {4, 7, 1}, {5, 8, 2}, {6, 0, 3}, {7, 1, 4}, {8, 2, 5}};
+#define LG(x, x0, x1) (((x) - (x1)) / ((x0) - (x1)))
+#define LGX(x, x0, x1) (1.0 / ((x0) - (x1)))
+/* kind: 0 value, 1 d/dxi, 2 d/deta, 3 xixi, 4 xieta, 5 etaeta */
+static double ip9(int k, int kind, const double *x)
+{
+    double xi = x[0], eta = x[1];
+    double a0 = XL9[XLI9[k][0]][0], a1 = XL9[XLI9[k][1]][0], a2 = XL9[XLI9[k][2]][0];
+    double b0 = XL9[XLJ9[k][0]][1], b1 = XL9[XLJ9[k][1]][1], b2 = XL9[XLJ9[k][2]][1];
+    double xf = LG(xi, a0, a1) * LG(xi, a0, a2);
+    double yf = LG(eta, b0, b1) * LG(eta, b0, b2);
+    double xf_x = LGX(xi, a0, a1) * LG(xi, a0, a2) + LG(xi, a0, a1) * LGX(xi, a0, a2);
+    double yf_x = LGX(eta, b0, b1) * LG(eta, b0, b2) + LG(eta, b0, b1) * LGX(eta, b0, b2);
+    switch (kind) {
+    case 0: return xf * yf;
+    case 1: return xf_x * yf;
+    case 2: return xf * yf_x;
+    case 3: return (2.0 * LGX(xi, a0, a1) * LGX(xi, a0, a2)) * yf;
+    case 4: return xf_x * yf_x;
+    default: return xf * (2.0 * LGX(eta, b0, b1) * LGX(eta, b0, b2));
+    }
+}
+static void jac9(const double *Xi, const double *xl, double J[4])
+{
+    J[0] = J[1] = J[2] = J[3] = 0.;
+    for (int k = 0; k < 9; ++k) J[0] += ip9(k, 1, xl) * Xi[2 * k];
+    for (int k = 0; k < 9; ++k) J[1] += ip9(k, 2, xl) * Xi[2 * k];
+    for (int k = 0; k < 9; ++k) J[2] += ip9(k, 1, xl) * Xi[2 * k + 1];
+    for (int k = 0; k < 9; ++k) J[3] += ip9(k, 2, xl) * Xi[2 * k + 1];
+}
+static double fd2x9(const double *Xi, const double *Yi, const double *xl, int dim)
+{
+    double J[4];
+    jac9(Xi, xl, J);
+    double det = J[0] * J[3] - J[1] * J[2];
+    double iv[4];
+    iv[0] = J[3] / det; iv[3] = J[0] / det; iv[1] = -J[2] / det; iv[2] = -J[1] / det;
+    double a = 0, b = 0, c = 0;
+    for (int k = 0; k < 9; ++k) a += ip9(k, 3, xl) * Yi[k];
+    for (int k = 0; k < 9; ++k) b += ip9(k, 4, xl) * Yi[k];
+    for (int k = 0; k < 9; ++k) c += ip9(k, 5, xl) * Yi[k];
+    if (dim == 0) return a * iv[0] * iv[0] + 2.0 * b * iv[0] * iv[2] + c * iv[2] * iv[2];
+    return a * iv[1] * iv[1] + 2.0 * b * iv[1] * iv[3] + c * iv[3] * iv[3];
+}
+static double laplace2d9(const double *Xi, const double *Yi, double gamma)
+{
+#define S35 0.77459666924148337703585307995648
+    static const double gp[9][2] = {{-S35, -S35}, {-S35, 0.}, {-S35, S35}, {0., -S35}, {0., 0.}, {0., S35}, {S35, -S35}, {S35, 0.}, {S35, S35}};
+    static const double gw[9] = {(5.0 / 9.0) * (5.0 / 9.0), (5.0 / 9.0) * (8.0 / 9.0), (5.0 / 9.0) * (5.0 / 9.0),
+                                 (8.0 / 9.0) * (5.0 / 9.0), (8.0 / 9.0) * (8.0 / 9.0), (8.0 / 9.0) * (5.0 / 9.0),
+                                 (5.0 / 9.0) * (5.0 / 9.0), (5.0 / 9.0) * (8.0 / 9.0), (5.0 / 9.0) * (5.0 / 9.0)};
+    double rv = 0., ra = 0.;
+    for (int k = 0; k < 9; ++k) {
+        double xl[2] = {gp[k][0] * gamma, gp[k][1] * gamma};
+        double dxx = fd2x9(Xi, Yi, xl, 0), dyy = fd2x9(Xi, Yi, xl, 2);
+        double J[4];
+        jac9(Xi, xl, J);
+        double jd = J[0] * J[3] - J[1] * J[2];
+        rv += (gamma * gamma) * gw[k] * (dxx + dyy) * jd;
+        ra += (gamma * gamma) * gw[k] * 1.0 * jd;
+    }
+    return rv / ra;
+}
+
+static double hanning(double x) { return 0.5 * (1.0 - cos(2.0 * M_PI * x)); }
+
+/* ------------------------------------------------------------ setup run */
+static int alloc_fields(salec_host *h)
+{
+    for (int k = 0; k < SALEC_NFIELDS; ++k) {
+        int grid, nc;
+        salec_field_info(k, h->cfg.nmat, &grid, &nc);
+        long n = salec_grid_points(grid, h->nx, h->ny) * nc;
+        h->f[k] = (double *)calloc((size_t)n, sizeof(double));
+        if (!h->f[k]) return SALEC_ENOMEM;
+    }
+    return SALEC_OK;
+}
+
+/* init_mesh_pos, src/construct.c:11-377 (single domain) */
+static void init_mesh(salec_host *h)
+{
+    int CW = h->nx + 4, CH = h->ny + 4, VW = h->nx + 5, VH = h->ny + 5;
+    long NC = (long)CW * CH, NV = (long)VW * VH, NBX = (long)(h->nx + 5) * (h->ny + 4), NBY = (long)(h->nx + 4) * (h->ny + 5);
+    int Ny = h->ny + 2 * NPAD, Nx = h->nx + 2 * NPAD;
+    int YeL = h->ny_ext[0], YeR = h->ny_ext[1], XeL = h->nx_ext[0], XeR = h->nx_ext[1];
+    double Yext = h->y_ext, Xext = h->x_ext;
+    int XdR = h->damp_x, XdL = h->damp_x_left, YdL = h->damp_y, YdR = h->damp_y_top;
+    double Ly = Spacing2(h->ny + NPAD, Yext, YeL, YeR, YdL, YdR, Ny) - Spacing2(NPAD, Yext, YeL, YeR, YdL, YdR, Ny);
+    int y0i;
+    for (y0i = 0; y0i < Ny + 1; ++y0i)
+        if (Spacing2(y0i + NPAD, Yext, YeL, YeR, YdL, YdR, Ny) >= Ly * h->oy) break;
+    double y0 = -Spacing2(y0i + NPAD, Yext, YeL, YeR, YdL, YdR, Ny) * h->dy;
+    double Lx = Spacing2(h->nx + NPAD, Xext, XeL, XeR, XdL, XdR, Nx) - Spacing2(NPAD, Xext, XeL, XeR, XdL, XdR, Nx);
+    int x0i;
+    for (x0i = 0; x0i < Nx + 1; ++x0i)
+        if (Spacing2(x0i, Xext, XeL, XeR, XdL, XdR, Nx) >= Lx * h->ox) break;
+    double x0 = -Spacing2(x0i + NPAD, Xext, XeL, XeR, XdL, XdR, Nx) * h->dx;
+    h->x0 = x0; h->y0 = y0;
+    double *vpos = h->f[SALEC_V_POS];
+    for (int j = 0; j < VH; ++j)
+        for (int k = 0; k < VW; ++k) {
+            vpos[k + (long)VW * j] = Spacing2(k, Xext, XeL, XeR, XdL, XdR, Nx) * h->dx + x0;
+            vpos[NV + k + (long)VW * j] = Spacing2(j, Yext, YeL, YeR, YdL, YdR, Ny) * h->dy + y0;
+        }
+    double *esuf = (double *)calloc((size_t)8 * NC, sizeof(double));
+    double *esub = (double *)calloc((size_t)8 * NC, sizeof(double));
+    double dly = Spacing2(h->damp_y, Yext, YeL, YeR, YdL, YdR, Ny) * h->dy;
+    double dlx = Spacing2(Nx, Xext, XeL, XeR, XdL, XdR, Nx) * h->dx - Spacing2(Nx - h->damp_x, Xext, XeL, XeR, XdL, XdR, Nx) * h->dx;
+    double csd = 5.0e3, pml_b = 2.3;
+    for (int j = 0; j < CH; ++j)
+        for (int k = 0; k < CW; ++k) {
+            long c = k + (long)CW * j;
+            long v[4] = {k + (long)VW * j, k + 1 + (long)VW * j, k + 1 + (long)VW * (j + 1), k + (long)VW * (j + 1)};
+            double pos[8];
+            for (int q = 0; q < 4; ++q) { pos[2 * q] = vpos[v[q]]; pos[2 * q + 1] = vpos[NV + v[q]]; }
+            double suf[8], sub[8], vol, gr[12], cen[4];
+            gen_suf_vol(pos, suf, sub, &vol);
+            gen_grad(pos, gr);
+            gen_center(pos, cen);
+            for (int q = 0; q < 8; ++q) { esuf[q * NC + c] = suf[q]; esub[q * NC + c] = sub[q]; }
+            h->f[SALEC_E_VOL][c] = vol;
+            for (int q = 0; q < 12; ++q) h->f[SALEC_E_GRAD][q * NC + c] = gr[q];
+            for (int q = 0; q < 4; ++q) h->f[SALEC_E_POS][q * NC + c] = cen[q];
+            double *es = h->f[SALEC_E_SIGMA];
+            es[c] = 0.; es[NC + c] = 0.;
+            double ddy = cen[1] - y0;
+            double ddx = Spacing2(Nx, Xext, XeL, XeR, XdL, XdR, Nx) * h->dx - (cen[0] - x0);
+            switch (h->cfg.damp_type) {
+            case SALEC_DAMP_PML: case SALEC_DAMP_PML2: case SALEC_DAMP_PML3:
+                if (ddy <= dly) es[NC + c] = pow(1.0 - ddy / dly, 2.0) / (2.0 * dly) * (3.0 * csd) * h->lambda_y;
+                if (ddx <= dlx) es[c] = pow(1.0 - ddx / dlx, 2.0) / (2.0 * dlx) * (3.0 * csd) * h->lambda_x;
+                h->cfg.alpha_damp_x = (csd * 2.0) / dlx * pml_b;
+                h->cfg.alpha_damp_y = (csd * 2.0) / dly * pml_b;
+                break;
+            case SALEC_DAMP_SBC:
+                es[c] = es[NC + c] = 1.0;
+                if (ddy <= dly) es[NC + c] = hanning(0.5 * ((1.0 - h->lambda_y) * ddy / dly + h->lambda_y));
+                if (ddx <= dlx) es[c] = hanning(0.5 * ((1.0 - h->lambda_x) * ddx / dlx + h->lambda_x));
+                es[c] = Min(es[c], es[NC + c]);
+                break;
+            default:
+                if (ddy <= dly) es[NC + c] = 1.0;
+                if (ddx <= dlx) es[c] = 1.0;
+                break;
+            }
+        }
+    static const int voff[4] = {2, 3, 0, 1};
+    for (int j = 0; j < VH; ++j)
+        for (int k = 0; k < VW; ++k) {
+            int xd = k < VW - 1 - k ? k : VW - 1 - k, yd = j < VH - 1 - j ? j : VH - 1 - j;
+            if (xd <= 0 || yd <= 0) continue;
+            long v = k + (long)VW * j;
+            long e[4] = {k - 1 + (long)CW * (j - 1), k + (long)CW * (j - 1), k + (long)CW * j, k - 1 + (long)CW * j};
+            double *vv = h->f[SALEC_V_VOL];
+            vv[v] = 0.;
+            for (int i = 0; i < 4; ++i) {
+                vv[v] += esub[(4 + voff[i]) * NC + e[i]];
+                h->f[SALEC_V_SUBVOL][i * NV + v] = esub[voff[i] * NC + e[i]];
+                vv[(i + 1) * NV + v] = esub[(voff[i] + 4) * NC + e[i]];
+                h->f[SALEC_V_SUF][(2 * i) * NV + v] = esuf[(voff[i] * 2) * NC + e[i]];
+                h->f[SALEC_V_SUF][(2 * i + 1) * NV + v] = esuf[(voff[i] * 2 + 1) * NC + e[i]];
+            }
+            double xi[18], yi[9];
+            int q = 0;
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int dk = -1; dk <= 1; ++dk, ++q) {
+                    long w = (k + dk) + (long)VW * (j + dj);
+                    xi[2 * q] = vpos[w]; xi[2 * q + 1] = vpos[NV + w];
+                }
+            for (int p = 0; p < 9; ++p) {
+                for (int r = 0; r < 9; ++r) yi[r] = 0.;
+                yi[p] = 1.;
+                h->f[SALEC_V_LAPLACE][p * NV + v] = laplace2d9(xi, yi, 0.5);
+            }
+        }
+    free(esuf); free(esub);
+    /* face normals, src/construct.c:311-354 */
+    int BXW = h->nx + 5, BXH = h->ny + 4, BYW = h->nx + 4, BYH = h->ny + 5;
+    for (int j = 0; j < BXH; ++j)
+        for (int k = 0; k < BXW; ++k) {
+            int xd = k < BXW - 1 - k ? k : BXW - 1 - k, yd = j < BXH - 1 - j ? j : BXH - 1 - j;
+            if (xd <= 0 || yd <= 0) continue;
+            long u = k + (long)CW * j, b = k - 1 + (long)CW * j;
+            double n[2] = {h->f[SALEC_E_POS][u] - h->f[SALEC_E_POS][b], h->f[SALEC_E_POS][NC + u] - h->f[SALEC_E_POS][NC + b]};
+            double l = sqrt(0. + n[0] * n[0] + n[1] * n[1]);
+            for (int q = 0; q < 2; ++q) { n[q] /= l; if (isnan(n[q])) n[q] = 1.0; }
+            h->f[SALEC_BX_NORMAL][k + (long)BXW * j] = n[0];
+            h->f[SALEC_BX_NORMAL][NBX + k + (long)BXW * j] = n[1];
+        }
+    for (int j = 0; j < BYH; ++j)
+        for (int k = 0; k < BYW; ++k) {
+            int xd = k < BYW - 1 - k ? k : BYW - 1 - k, yd = j < BYH - 1 - j ? j : BYH - 1 - j;
+            if (xd <= 0 || yd <= 0) continue;
+            long r = k + (long)CW * j, l_ = k + (long)CW * (j - 1);
+            double n[2] = {h->f[SALEC_E_POS][r] - h->f[SALEC_E_POS][l_], h->f[SALEC_E_POS][NC + r] - h->f[SALEC_E_POS][NC + l_]};
+            double l = sqrt(0. + n[0] * n[0] + n[1] * n[1]);
+            for (int q = 0; q < 2; ++q) { n[q] /= l; if (isnan(n[q])) n[q] = 1.0; }
+            h->f[SALEC_BY_NORMAL][k + (long)BYW * j] = n[0];
+            h->f[SALEC_BY_NORMAL][NBY + k + (long)BYW * j] = n[1];
+        }
+}
+
+/* build_target_profile for a plane target, src/target_profile.c:86-343,1161-1196 */
+typedef struct { int n; double dz; double *pre, *rad, *den, *tem, *eng; int *mat; } tprof;
+
+static int build_profile(salec_host *h, tprof *P)
+{
+    P->dz = Min(h->dy * 0.5, h->max_depth / h->n_profile);
+    P->n = (int)ceil(h->max_depth / P->dz);
+    int n = P->n;
+    P->pre = (double *)calloc(n, sizeof(double)); P->rad = (double *)calloc(n, sizeof(double));
+    P->den = (double *)calloc(n, sizeof(double)); P->tem = (double *)calloc(n, sizeof(double));
+    P->eng = (double *)calloc(n, sizeof(double)); P->mat = (int *)calloc(n, sizeof(int));
+    if (!P->pre || !P->rad || !P->den || !P->tem || !P->eng || !P->mat) return SALEC_ENOMEM;
+    /* SetTargetProfile_Mat */
+    int cur = n - 1;
+    for (int l = 0; l < h->n_layer; l++) {
+        int ti = (int)floor(h->t_depth[l] / P->dz);
+        for (int j = 0; j < ti && cur > 0; ++j) P->mat[cur--] = h->t_mat[l];
+    }
+    for (; cur >= 0; --cur) P->mat[cur] = h->t_mat[h->n_layer - 1];
+    /* SetTargetProfile_Tem */
+    for (int k = 0; k < n; ++k) {
+        P->pre[k] = h->surf_pre;
+        P->den[k] = h->mat[P->mat[k]].den0;
+        P->rad[k] = h->max_depth * k / (n - 1);
+    }
+    if (h->tem_type == 0) {
+        for (int k = 0; k < n; ++k) P->tem[k] = h->surf_tem;
+    } else {
+        for (int k = n - 1; k >= 0; --k) {
+            double kd = P->rad[n - 1] - P->rad[k];
+            if (n == k + 1) { P->tem[k] = h->surf_tem; continue; }
+            double g = h->surf_dtdz * (1.0 + kd / h->r_planet);
+            P->tem[k] = P->tem[k + 1] + g * (P->rad[k + 1] - P->rad[k]);
+        }
+    }
+    double gz = h->gravity[1];
+    /* IntegrateGravity + CorrectionDensity1, iterated to the fixed point */
+    double *prev = (double *)malloc(sizeof(double) * n);
+    for (int it = 0; it < 2000; ++it) {
+        for (int k = n - 2; k >= 0; --k) {
+            double drg = 0.25 * (P->den[k + 1] + P->den[k]) * (gz + gz);
+            P->pre[k] = P->pre[k + 1] + drg * (P->rad[k] - P->rad[k + 1]);
+        }
+        memcpy(prev, P->den, sizeof(double) * n);
+        for (int k = 0; k < n; ++k) {
+            const salec_material *r = &h->mat[P->mat[k]];
+            double mt = r->Tmelt0 * pow(r->Asimon * P->pre[k] + 1.0, r->Csimon);
+            if (r->PolySol > 0) { double p = P->pre[k] * 1.0e-9; mt = r->Tsolc1 + r->Tsolc2 * p + r->Tsolc3 * p * p; }
+            double lt = Min(P->tem[k], mt);
+            P->tem[k] = lt;
+            int m = P->mat[k];
+            P->eng[k] = till_interp_tp(h->inv_p[m], h->inv_yeng[m], h->inv_nden[m], lt, P->pre[k], 0);
+            P->den[k] = till_interp_tp(h->inv_p[m], h->inv_yeng[m], h->inv_nden[m], lt, P->pre[k], -1);
+        }
+        if (memcmp(prev, P->den, sizeof(double) * n) == 0) break;
+    }
+    free(prev);
+    for (int k = n - 2; k >= 0; --k) {
+        double drg = 0.25 * (P->den[k + 1] + P->den[k]) * (gz + gz);
+        P->pre[k] = P->pre[k + 1] + drg * (P->rad[k] - P->rad[k + 1]);
+    }
+    return SALEC_OK;
+}
+
+int salec_host_setup(salec_host *h)
+{
+    int rc = alloc_fields(h);
+    if (rc) return rc;
+    init_mesh(h);
+    tprof P;
+    memset(&P, 0, sizeof P);
+    if ((rc = build_profile(h, &P)) != SALEC_OK) return rc;
+    int nmat = h->cfg.nmat;
+    int CW = h->nx + 4, CH = h->ny + 4, VW = h->nx + 5, VH = h->ny + 5;
+    long NC = (long)CW * CH, NV = (long)VW * VH;
+    /* init_material_plane_distribution, src/target_profile.c:588-713 */
+    for (int j = 0; j < CH; ++j)
+        for (int k = 0; k < CW; ++k) {
+            long c = k + (long)CW * j;
+            double *ep = h->f[SALEC_E_POS];
+            double ex = ep[c], ey = ep[NC + c];
+            for (int m = 0; m < nmat; ++m) {
+                h->f[SALEC_M_PTY][m * NC + c] = 1.0;
+                h->f[SALEC_M_VOF][m * NC + c] = 0.0;
+                h->f[SALEC_M_DEN][m * NC + c] = 0.0;
+                h->f[SALEC_M_ENG][m * NC + c] = 0.0;
+            }
+            h->f[SALEC_E_VEL][c] = 0.; h->f[SALEC_E_VEL][NC + c] = 0.;
+            h->f[SALEC_E_TPS][c] = 0.; h->f[SALEC_E_VIB][c] = 0.; h->f[SALEC_E_CVS][c] = 0.;
+            double dxp = ex - h->p_center[0], dyp = ey - h->p_center[1];
+            double pd = sqrt(0. + dxp * dxp + dyp * dyp);
+            double *wpt = h->f[SALEC_E_WPT];
+            if (pd <= h->p_radius) {
+                int pm = h->p_mat;
+                h->f[SALEC_M_VOF][pm * NC + c] = 1.0;
+                h->f[SALEC_E_PRE][c] = h->p_pre; h->f[SALEC_E_TEM][c] = h->p_tem; h->f[SALEC_E_DAM][c] = h->p_dam;
+                double pty = h->mat[pm].alpha0;
+                h->f[SALEC_M_DEN][pm * NC + c] = till_interp_tp(h->inv_p[pm], h->inv_yeng[pm], h->inv_nden[pm], h->p_tem, h->p_pre * pty, -1) / pty;
+                h->f[SALEC_M_ENG][pm * NC + c] = till_interp_tp(h->inv_p[pm], h->inv_yeng[pm], h->inv_nden[pm], h->p_tem, h->p_pre * pty, 0) / pty;
+                h->f[SALEC_E_VEL][c] = h->p_vel[0]; h->f[SALEC_E_VEL][NC + c] = h->p_vel[1];
+                for (int q = 0; q < 4; ++q) wpt[q * NC + c] = -0.25;
+                h->f[SALEC_M_PTY][pm * NC + c] = pty;
+            } else if (ey <= 0.) {
+                int li = (int)floor(fabs(ey) / h->max_depth * (P.n - 1)) + 1;
+                li = P.n - 1 - li;
+                if (li > P.n - 1) li = P.n - 1;
+                if (li < 0) li = 0;
+                int lm = P.mat[li];
+                h->f[SALEC_E_PRE][c] = P.pre[li]; h->f[SALEC_E_TEM][c] = P.tem[li]; h->f[SALEC_E_DAM][c] = 0.;
+                h->f[SALEC_M_VOF][lm * NC + c] = 1.0;
+                h->f[SALEC_M_DEN][lm * NC + c] = P.den[li];
+                h->f[SALEC_M_ENG][lm * NC + c] = P.eng[li];
+                h->f[SALEC_M_PTY][lm * NC + c] = h->mat[lm].alpha0;
+                for (int q = 0; q < 4; ++q) wpt[q * NC + c] = -0.25;
+            } else {
+                h->f[SALEC_M_VOF][c] = 1.0;
+                for (int q = 0; q < 4; ++q) wpt[q * NC + c] = 1.0;
+            }
+        }
+    /* gravity and ANC taper, src/target_profile.c:715-831 */
+    for (int j = 0; j < VH; ++j)
+        for (int k = 0; k < VW; ++k) {
+            long v = k + (long)VW * j;
+            h->f[SALEC_V_BF][v] = h->gravity[0];
+            h->f[SALEC_V_BF][NV + v] = h->gravity[1];
+            if (j <= NPAD) h->f[SALEC_V_BF][NV + v] = 0.;
+            double ar = 0.;
+            if (!strcasecmp(h->anc_pattern, "pml")) {
+                if (h->damp_y >= 1 && j <= 2 * h->damp_y) {
+                    double lx = 0.5 * j / h->damp_y;
+                    ar = 2.0 * Max(ar, 2.0 * hanning(0.5 * (1.0 - lx)));
+                }
+                if (h->damp_x >= 1 && k >= VW - 2 * h->damp_x) {
+                    double lx = 0.5 * (VW - 2 * h->damp_x - k) / h->damp_x;
+                    ar = 2.0 * Max(ar, 2.0 * hanning(0.5 * lx));
+                }
+            } else if (!strcasecmp(h->anc_pattern, "uniform")) {
+                ar = 1.0;
+            } else {
+                ar = 0.;
+            }
+            for (int q = 0; q < 9; ++q) h->f[SALEC_V_LAPLACE][q * NV + v] *= ar;
+        }
+    free(P.pre); free(P.rad); free(P.den); free(P.tem); free(P.eng); free(P.mat);
+    return SALEC_OK;
+}
+
+/* fl_tracer_init, src/tracer.c:380-450 */
+static int tracer_pass(salec_host *h, salec_tracers *tr, int fill)
+{
+    int strip = h->tracer_strip < 0 ? -h->tracer_strip : h->tracer_strip;
+    if (strip == 0) strip = 1;
+    int CW = h->nx + 4, CH = h->ny + 4, VW = h->nx + 5, VH = h->ny + 5;
+    long NC = (long)CW * CH, NV = (long)VW * VH;
+    long cnt = 0;
+    int nmat = h->cfg.nmat;
+    for (int j = 0; j < CH; ++j)
+        for (int k = 0; k < CW; ++k) {
+            int xd = k < CW - 1 - k ? k : CW - 1 - k, yd = j < CH - 1 - j ? j : CH - 1 - j;
+            if (xd < 2 || yd < 2) continue;
+            if (k % strip != 0 || j % strip != 0) continue;
+            long c = k + (long)CW * j;
+            int mid = 0;
+            for (int m = 1; m < nmat; ++m)
+                if (h->f[SALEC_M_VOF][mid * NC + c] < h->f[SALEC_M_VOF][m * NC + c]) mid = m;
+            if (mid == 0) continue;
+            if (fill) {
+                if (cnt >= tr->n) return SALEC_EINVAL;
+                long i = cnt;
+                tr->pos[i] = h->f[SALEC_E_POS][c];
+                tr->pos[tr->n + i] = h->f[SALEC_E_POS][NC + c];
+                long vv[4] = {k + (long)VW * j, k + 1 + (long)VW * j, k + 1 + (long)VW * (j + 1), k + (long)VW * (j + 1)};
+                for (int d = 0; d < 2; ++d) {
+                    double r = 0, x0[2] = {0., 0.};
+                    for (int q = 0; q < 4; ++q) r += ipb(q, x0) * h->f[SALEC_V_VEL][d * NV + vv[q]];
+                    tr->vel[d * tr->n + i] = r;
+                }
+                tr->mpre[i] = 0.f; tr->mtem[i] = 0.f;
+                tr->gx[i] = k; tr->gy[i] = j;
+                tr->tag[i] = k + CW * j;
+                tr->matid[i] = mid;
+                tr->idx[i] = k; tr->idy[i] = j;
+                tr->alive[i] = 1;
+            }
+            cnt++;
+        }
+    return (int)cnt;
+}
+
+long salec_host_count_tracers(const salec_host *h) { return tracer_pass((salec_host *)h, NULL, 0); }
+
+int salec_host_init_tracers(salec_host *h, salec_tracers *tr)
+{
+    long cap = tr->n;
+    int n = tracer_pass(h, tr, 1);
+    if (n < 0) return n;
+    /* compact the second components if fewer than capacity were written */
+    if (n < cap) {
+        memmove(tr->pos + n, tr->pos + cap, sizeof(double) * n);
+        memmove(tr->vel + n, tr->vel + cap, sizeof(double) * n);
+    }
+    tr->n = n;
+    return SALEC_OK;
+}

@@ -1,0 +1,58 @@
+/*
+ * setup.h -- host-side setup of a SALEc-2D run, the C host around the
+ * device step.  Restates the reference's initialisation path:
+ *   load_mesh_info / load_state_reference      (src/sale2d_back.c:64-402)
+ *   EOS loaders, RK4 cold-energy table          (src/eos/eos_state.c:463-662,
+ *                                                src/eos/aneos_sieden.c:11-73)
+ *   init_mesh_pos and geometry generators       (src/construct.c:11-594)
+ *   plane target profile, projectile, gravity,
+ *   ANC taper                                   (src/target_profile.c:86-343,
+ *                                                588-834, 1161-1196)
+ *   fl_tracer_init                              (src/tracer.c:380-450)
+ * The produced host fields are uploaded with salec_upload(); the remaining
+ * initialisation (src/salec2d.c:76-94) runs on the device in salec_init().
+ */
+#ifndef SALEC_HOST_SETUP_H
+#define SALEC_HOST_SETUP_H
+#include "../../include/salec.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct salec_host salec_host;
+
+/* Parse an .inp file and the EOS files it names under eos_dir.
+ * Returns SALEC_OK or a negative code; err receives a message. */
+int  salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out, char *err, int errlen);
+void salec_host_free(salec_host *h);
+
+/* Allocate host fields and fill geometry + initial material distribution. */
+int  salec_host_setup(salec_host *h);
+
+const salec_config *salec_host_config(const salec_host *h);
+double *salec_host_field(salec_host *h, int field);
+/* cycle.{time0,time1,maxdt,dt0,maxstep}, numerical damp time (input) */
+void salec_host_cycle(const salec_host *h, double out[6]);
+/* Number of tracers fl_tracer_init would create (strip from the .inp). */
+long salec_host_count_tracers(const salec_host *h);
+/* Fill tracer arrays (capacity tr->n on entry, count on return); uses the
+ * host V_VEL field, so download it after salec_init first. */
+int  salec_host_init_tracers(salec_host *h, salec_tracers *tr);
+/* Set the damp time returned by salec_init into the host config copy. */
+void salec_host_set_damp_time(salec_host *h, double t);
+
+/* EOS file writers used to build the input sets: Tillotson parameter file
+ * (keys of LoadTillEOS, src/eos/eos_state.c:635-662) and a .aneos.sd table
+ * (layout of LoadANEOSSieDen, src/eos/aneos_sieden.c:11-73) filled from
+ * Tillotson on a log-spaced density x specific-energy grid.  The table's
+ * three header lines carry the Tillotson constants for setup inversion. */
+int  salec_write_tillotson(const char *path, const double p[12], int rhostep);
+int  salec_write_sd_table(const char *path, const double p[12], int rhostep,
+                          int nden, int neng, double rho_min, double rho_max,
+                          double e_min, double e_max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

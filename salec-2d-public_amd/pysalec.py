@@ -1,0 +1,430 @@
+"""ctypes bindings to libsalec.so (include/salec.h) and the input-set writers.
+
+This module is glue for tests, the smoke check and bench.py: it loads the
+in-tree shared library, mirrors the C structs, and writes the .inp / EOS files
+of the benchmark configurations.  The solver itself runs in libsalec.so; a
+missing library is an error, never a fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsalec.so")
+
+FIELDS = [
+    ("M_VOF", 0, -1), ("M_DEN", 0, -1), ("M_ENG", 0, -1), ("M_PTY", 0, -1), ("E_STE", 0, 4),
+    ("E_DAM", 0, 1), ("E_TPS", 0, 1), ("E_VIB", 0, 1), ("E_EJT", 0, 1), ("E_STE2S", 0, 1),
+    ("E_STA2S", 0, 1), ("E_STRENGTH", 0, 1), ("E_DVEL", 0, 2), ("E_ALPHA", 0, 2), ("E_WPT", 0, 4),
+    ("E_DEN", 0, 1), ("E_PRE", 0, 1), ("E_CSD", 0, 1), ("E_TEM", 0, 1), ("E_MU", 0, 1),
+    ("E_CVS", 0, 1), ("E_VEL", 0, 2), ("E_GRAD_VEL", 0, 5), ("E_DIV_VEL", 0, 1), ("E_Q", 0, 1),
+    ("E_POS", 0, 4), ("E_VOL", 0, 1), ("E_GRAD", 0, 12), ("E_SIGMA", 0, 2), ("E_DAMPREF", 0, 3),
+    ("V_VEL", 1, 2), ("V_MASS", 1, 2), ("V_ALPHA", 1, 2), ("V_VOF", 1, -1), ("V_ACC", 1, 2),
+    ("V_DIS", 1, 2), ("V_POS", 1, 2), ("V_BF", 1, 2), ("V_VOL", 1, 5), ("V_SUBVOL", 1, 4),
+    ("V_SUF", 1, 8), ("V_LAPLACE", 1, 9), ("BX_NORMAL", 2, 2), ("BY_NORMAL", 3, 2), ("Y_PROFILE", 4, 1),
+]
+FIELD_ID = {name: i for i, (name, _, _) in enumerate(FIELDS)}
+NFIELDS = len(FIELDS)
+
+# persistent state carried from step to step (reference section 2.2)
+PERSISTENT = ["M_VOF", "M_DEN", "M_ENG", "E_STE", "E_DAM", "E_TPS", "E_VIB", "E_EJT", "E_STE2S",
+              "E_DVEL", "E_ALPHA", "E_WPT", "E_DEN", "E_PRE", "E_CSD", "E_TEM", "E_MU", "E_CVS",
+              "V_VEL", "V_MASS", "V_ALPHA"]
+TRANSIENT = ["E_VEL", "E_GRAD_VEL", "E_DIV_VEL", "E_Q", "E_STA2S", "E_STRENGTH", "V_VOF", "V_ACC",
+             "V_DIS", "Y_PROFILE"]
+
+SALEC_OK = 0
+DAMP = {"pml": 0, "pml2": 1, "pml3": 2, "sbc": 3, "ctlg": 4, "none": 5}
+
+
+def grid_dims(grid, nx, ny):
+    return {0: (ny + 4, nx + 4), 1: (ny + 5, nx + 5), 2: (ny + 4, nx + 5), 3: (ny + 5, nx + 4), 4: (1, nx + 4)}[grid]
+
+
+def field_shape(name, nmat, nx, ny):
+    _, grid, nc = FIELDS[FIELD_ID[name]]
+    h, w = grid_dims(grid, nx, ny)
+    return (nmat if nc < 0 else nc, h, w)
+
+
+class Material(C.Structure):
+    _fields_ = [("eos_type", C.c_int)] + [(n, C.c_double) for n in (
+        "TLRho0", "TLCv", "TLA", "TLB", "TLE0", "TLa", "TLb", "TLAlpha", "TLBeta", "TLEiv", "TLEcv", "TLTref")] + [
+        ("till_nden", C.c_int), ("till_stepden", C.c_double), ("till_yeng", C.POINTER(C.c_double)),
+        ("tab_nden", C.c_int), ("tab_ntem", C.c_int), ("tab_yden", C.POINTER(C.c_double)),
+        ("tab_xtem", C.POINTER(C.c_double)), ("tab_data", C.POINTER(C.c_double))] + [(n, C.c_double) for n in (
+        "VaporDen", "MeltDen", "GaCoef", "Yint0", "Yfricint", "Ylimint", "Ydam0", "Yfricdam", "Ylimdam", "Yten0",
+        "IvanA", "IvanB", "IvanC", "Pbd", "Pbp", "minPint", "minPdam", "JCA", "JCB", "JCN", "JCC", "JCM", "JCTREF",
+        "JCMaxPlastic", "JCMinP", "Asimon", "Csimon", "Tmelt0", "Tfrac", "Tdelta")] + [
+        ("PolyLiq", C.c_int), ("PolySol", C.c_int)] + [(n, C.c_double) for n in (
+        "Tlidc1", "Tlidc2", "Tlidc3", "Tsolc1", "Tsolc2", "Tsolc3", "Toff", "Cvib", "VibMax", "Tdamp", "Pvlim",
+        "Acvis", "alpha0", "chi", "gamfrac", "den0", "melt_viscosity")] + [
+        ("yfunc", C.c_int), ("dfunc", C.c_int), ("tfunc", C.c_int), ("pfunc", C.c_int)]
+
+
+class Config(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nmat", C.c_int), ("mat", C.POINTER(Material))] + [
+        (n, C.c_double) for n in ("avis1", "avis2", "anc", "density_cutoff", "velocity_max", "velocity_min",
+                                  "acceleration_min")] + [
+        ("partpres", C.c_int), ("tensile_failure", C.c_int), ("damp_type", C.c_int)] + [
+        (n, C.c_double) for n in ("alpha_damp_x", "alpha_damp_y", "damp_time", "threshold_z", "flow_resistance",
+                                  "flow_friction", "max_dt")] + [
+        ("go_list", C.c_int * 4), ("max_tracers", C.c_long)]
+
+
+class Tracers(C.Structure):
+    _fields_ = [("n", C.c_long), ("pos", C.POINTER(C.c_double)), ("vel", C.POINTER(C.c_double)),
+                ("mpre", C.POINTER(C.c_float)), ("mtem", C.POINTER(C.c_float))] + [
+        (n, C.POINTER(C.c_int)) for n in ("gx", "gy", "tag", "matid", "idx", "idy", "alive")]
+
+
+class Stats(C.Structure):
+    _fields_ = [("dt", C.c_double), ("t", C.c_double), ("step", C.c_long)]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+_LIB = None
+
+
+def lib():
+    """Load libsalec.so from the package directory (raises if absent)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsalec.so is not built: run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.salec_host_load.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
+    L.salec_host_setup.argtypes = [vp]
+    L.salec_host_free.argtypes = [vp]
+    L.salec_host_config.argtypes = [vp]
+    L.salec_host_config.restype = C.POINTER(Config)
+    L.salec_host_field.argtypes = [vp, C.c_int]
+    L.salec_host_field.restype = C.POINTER(C.c_double)
+    L.salec_host_cycle.argtypes = [vp, C.POINTER(C.c_double)]
+    L.salec_host_count_tracers.argtypes = [vp]
+    L.salec_host_count_tracers.restype = C.c_long
+    L.salec_host_init_tracers.argtypes = [vp, C.POINTER(Tracers)]
+    L.salec_host_set_damp_time.argtypes = [vp, C.c_double]
+    L.salec_write_tillotson.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int]
+    L.salec_write_sd_table.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int,
+                                       C.c_double, C.c_double, C.c_double, C.c_double]
+    L.salec_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.salec_destroy.argtypes = [vp]
+    L.salec_upload.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.salec_download.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.salec_upload_tracers.argtypes = [vp, C.POINTER(Tracers)]
+    L.salec_download_tracers.argtypes = [vp, C.POINTER(Tracers)]
+    L.salec_set_time.argtypes = [vp, C.c_double, C.c_double]
+    L.salec_get_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_long)]
+    L.salec_init.argtypes = [vp, C.POINTER(C.c_double)]
+    L.salec_set_damp_time.argtypes = [vp, C.c_double]
+    L.salec_step.argtypes = [vp, C.c_uint, C.POINTER(Stats)]
+    L.salec_sync.argtypes = [vp]
+    L.salec_check.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)]
+    L.salec_set_profiling.argtypes = [vp, C.c_int]
+    L.salec_phase_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_float)]
+    L.salec_strerror.argtypes = [C.c_int]
+    L.salec_strerror.restype = C.c_char_p
+    L.salec_field_name.argtypes = [C.c_int]
+    L.salec_field_name.restype = C.c_char_p
+    L.salec_grid_points.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.salec_grid_points.restype = C.c_long
+    for i, (name, _, _) in enumerate(FIELDS):
+        got = L.salec_field_name(i).decode()
+        if got != name:
+            raise RuntimeError("field table mismatch at %d: %s != %s" % (i, got, name))
+    _LIB = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != SALEC_OK:
+        raise RuntimeError("%s failed: %s (%d)" % (what, lib().salec_strerror(rc).decode(), rc))
+
+
+class HostRun:
+    """An .inp-driven run set up on the host (salec_host_*)."""
+
+    def __init__(self, inp_path, eos_dir):
+        L = lib()
+        self.h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = L.salec_host_load(inp_path.encode(), eos_dir.encode(), C.byref(self.h), err, 512)
+        if rc != SALEC_OK:
+            raise RuntimeError("salec_host_load: %s" % err.value.decode())
+        check(L.salec_host_setup(self.h), "salec_host_setup")
+        self.cfg = L.salec_host_config(self.h).contents
+        self.nx, self.ny, self.nmat = self.cfg.nx, self.cfg.ny, self.cfg.nmat
+        cyc = (C.c_double * 6)()
+        L.salec_host_cycle(self.h, cyc)
+        self.time0, self.time1, self.maxdt, self.dt0, self.maxstep, self.damp_t_in = list(cyc)
+
+    def field(self, name):
+        """numpy view of a host field (ncomp, H, W)."""
+        shp = field_shape(name, self.nmat, self.nx, self.ny)
+        p = lib().salec_host_field(self.h, FIELD_ID[name])
+        return np.ctypeslib.as_array(p, shape=shp)
+
+    def state(self):
+        return {name: self.field(name).copy() for name, _, _ in FIELDS}
+
+    def tracers(self, v_vel=None):
+        """Tracers of fl_tracer_init (needs V_VEL after initialisation)."""
+        L = lib()
+        if v_vel is not None:
+            self.field("V_VEL")[...] = v_vel
+        n = L.salec_host_count_tracers(self.h)
+        tr = TracerSet(n)
+        check(L.salec_host_init_tracers(self.h, C.byref(tr.struct())), "salec_host_init_tracers")
+        return tr
+
+    def close(self):
+        if self.h:
+            lib().salec_host_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TracerSet:
+    def __init__(self, n):
+        self.n = int(n)
+        self.pos = np.zeros((2, max(n, 1)))
+        self.vel = np.zeros((2, max(n, 1)))
+        self.mpre = np.zeros(max(n, 1), np.float32)
+        self.mtem = np.zeros(max(n, 1), np.float32)
+        for k in ("gx", "gy", "tag", "matid", "idx", "idy", "alive"):
+            setattr(self, k, np.zeros(max(n, 1), np.int32))
+        self._s = None
+
+    def struct(self):
+        s = Tracers()
+        s.n = self.n
+        s.pos, s.vel = _dp(self.pos), _dp(self.vel)
+        s.mpre, s.mtem = _fp(self.mpre), _fp(self.mtem)
+        for k in ("gx", "gy", "tag", "matid", "idx", "idy", "alive"):
+            setattr(s, k, _ip(getattr(self, k)))
+        self._s = s
+        return s
+
+    def copy(self):
+        t = TracerSet(self.n)
+        for k in ("pos", "vel", "mpre", "mtem", "gx", "gy", "tag", "matid", "idx", "idy", "alive"):
+            getattr(t, k)[...] = getattr(self, k)
+        return t
+
+
+class Device:
+    """A device context (salec_create) holding one run's state on the GPU."""
+
+    def __init__(self, cfg, max_tracers=0):
+        L = lib()
+        cfg.max_tracers = int(max_tracers)
+        self.cfg = cfg
+        self.nx, self.ny, self.nmat = cfg.nx, cfg.ny, cfg.nmat
+        self.ctx = C.c_void_p()
+        check(L.salec_create(C.byref(cfg), C.byref(self.ctx)), "salec_create")
+
+    def upload_state(self, state):
+        for name, arr in state.items():
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            check(lib().salec_upload(self.ctx, FIELD_ID[name], _dp(a)), "salec_upload " + name)
+
+    def download(self, name):
+        a = np.empty(field_shape(name, self.nmat, self.nx, self.ny))
+        check(lib().salec_download(self.ctx, FIELD_ID[name], _dp(a)), "salec_download " + name)
+        return a
+
+    def state(self, names=None):
+        return {n: self.download(n) for n in (names or [f[0] for f in FIELDS])}
+
+    def init(self, damp_time_in):
+        d = C.c_double(damp_time_in)
+        check(lib().salec_init(self.ctx, C.byref(d)), "salec_init")
+        return d.value
+
+    def set_damp_time(self, t):
+        check(lib().salec_set_damp_time(self.ctx, t), "salec_set_damp_time")
+
+    def set_time(self, t, dt):
+        check(lib().salec_set_time(self.ctx, t, dt), "salec_set_time")
+
+    def get_time(self):
+        t, dt, st = C.c_double(), C.c_double(), C.c_long()
+        check(lib().salec_get_time(self.ctx, C.byref(t), C.byref(dt), C.byref(st)), "salec_get_time")
+        return t.value, dt.value, st.value
+
+    def step(self, sync=True):
+        s = Stats()
+        check(lib().salec_step(self.ctx, 0 if sync else 1, C.byref(s) if sync else None), "salec_step")
+        return s
+
+    def sync(self):
+        check(lib().salec_sync(self.ctx), "salec_sync")
+
+    def upload_tracers(self, tr):
+        check(lib().salec_upload_tracers(self.ctx, C.byref(tr.struct())), "salec_upload_tracers")
+
+    def download_tracers(self, n):
+        tr = TracerSet(n)
+        s = tr.struct()
+        check(lib().salec_download_tracers(self.ctx, C.byref(s)), "salec_download_tracers")
+        tr.n = s.n
+        return tr
+
+    def set_profiling(self, on):
+        check(lib().salec_set_profiling(self.ctx, 1 if on else 0), "salec_set_profiling")
+
+    def phase_times(self):
+        names = (C.c_char_p * 64)()
+        ms = (C.c_float * 64)()
+        n = lib().salec_phase_times(self.ctx, 64, names, ms)
+        return [(names[k].decode(), ms[k]) for k in range(n)]
+
+    def check_finite(self):
+        f, i = C.c_int(), C.c_long()
+        rc = lib().salec_check(self.ctx, C.byref(f), C.byref(i))
+        return rc, FIELDS[f.value][0] if rc != SALEC_OK else None, i.value
+
+    def close(self):
+        if self.ctx:
+            lib().salec_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------- inputs ---
+# Tillotson constants: Rho0, Cv, A, B, E0, a, b, alpha, beta, Eiv, Ecv, Tref
+GRANITE = [2680.0, 8.96e2, 1.8e10, 1.8e10, 1.6e7, 0.5, 1.3, 5.0, 5.0, 3.5e6, 1.8e7, 293.0]
+
+
+def tillotson(rho0=None, **kw):
+    p = list(GRANITE)
+    if rho0 is not None:
+        p[0] = rho0
+    return p
+
+
+def write_eos(eos_dir, name, params, table=None):
+    L = lib()
+    arr = (C.c_double * 12)(*params)
+    os.makedirs(eos_dir, exist_ok=True)
+    if table is None:
+        check(L.salec_write_tillotson(os.path.join(eos_dir, name + ".tillotson").encode(), arr, 1501),
+              "salec_write_tillotson")
+    else:
+        nd, ne, rmin, rmax, emin, emax = table
+        check(L.salec_write_sd_table(os.path.join(eos_dir, name + ".aneos.sd").encode(), arr, 1501, nd, ne,
+                                     rmin, rmax, emin, emax), "salec_write_sd_table")
+
+
+def _inp_lines(sections):
+    out = []
+    for sec, kv in sections:
+        out.append("[%s]" % sec)
+        for k, v in kv:
+            out.append("    %s = %s" % (k, v))
+        out.append("")
+    return "\n".join(out) + "\n"
+
+
+def write_config(path_dir, which, nx=None, ny=None, table_n=(256, 256), tracer_strip=None, proj_cells=None,
+                 reg_cells=None, velocity=None, damp_cells=None, **over):
+    """Write the .inp and EOS files of a benchmark configuration.
+
+    which = "c0": plain hydro, one granite Tillotson material, no strength,
+                  256 x 384 cells of 10 m, projectile radius 10 cells at 6 km/s.
+    which = "c1": crater with strength: regolith (60 cells, rho0 1800) over
+                  basalt (rho0 2860), both 256 x 256 log-spaced (rho, e) tables
+                  filled from Tillotson, ROCK + Ivanov damage, 1000 x 1500
+                  cells, projectile radius 20 cells at 12 km/s, ~1M tracers.
+    Cells are 10 m.  nx, ny shrink the mesh for tests; the projectile radius
+    and regolith thickness (in cells) shrink with nx unless given.
+    """
+    os.makedirs(path_dir, exist_ok=True)
+    eos_dir = os.path.join(path_dir, "eos")
+    if which == "c0":
+        nx = nx or 256
+        ny = ny or 384
+        dx = 10.0
+        rc = proj_cells if proj_cells is not None else max(3.0, 10.0 * nx / 256.0)
+        vel = velocity if velocity is not None else 6000.0
+        write_eos(eos_dir, "granite", tillotson())
+        mats = [("nm", 2), ("name", "[vacuum_, granite]"), ("postfix", "[null, tillotson]"),
+                ("yshear", "[, None]"), ("damage", "[, None]"), ("ytens", "[, None]"),
+                ("Porosity", "[, None]"), ("Toff", "[, -1.0]")]
+        proj = [("material", "granite"), ("radiu", rc * dx), ("center", "[0.0, %g]" % (rc * dx)),
+                ("velocity", "[0.0, %g]" % -vel)]
+        target = [("type", "plane"), ("number", 1), ("material", "[granite]"), ("depth", "[%g]" % (ny * dx * 2)),
+                  ("max_depth", ny * dx * 1.5), ("temperature", "[const, 293.0]")]
+        strip = 2 if tracer_strip is None else tracer_strip
+        cycle = [("dt0", 1.0e-5), ("maxdt", 1.0e-2), ("time0", 0.0), ("time1", 100.0), ("maxstep", 1999)]
+    elif which == "c1":
+        nx = nx or 1000
+        ny = ny or 1500
+        dx = 10.0
+        rc = proj_cells if proj_cells is not None else max(3.0, 20.0 * nx / 1000.0)
+        reg = reg_cells if reg_cells is not None else max(3.0, 60.0 * nx / 1000.0)
+        vel = velocity if velocity is not None else 12000.0
+        tbl = (table_n[0], table_n[1], 1.0e-2, 1.2e4, 1.0e-1, 1.0e9)
+        write_eos(eos_dir, "regolith", tillotson(1800.0), table=tbl)
+        write_eos(eos_dir, "basalt", tillotson(2860.0), table=tbl)
+        mats = [("nm", 3), ("name", "[vacuum_, regolith, basalt]"), ("postfix", "[null, aneos.sd, aneos.sd]"),
+                ("yshear", "[, RockStrengthT, RockStrengthT]"), ("damage", "[, Ivanov, Ivanov]"),
+                ("ytens", "[, SimpleTensile, SimpleTensile]"), ("Porosity", "[, None, None]"),
+                ("yint0", "[, 1.0e7, 1.0e7]"), ("yintfri", "[, 1.2, 1.2]"), ("yintlim", "[, 2.5e9, 2.5e9]"),
+                ("ydam0", "[, 1.0e4, 1.0e4]"), ("ydamfri", "[, 0.6, 0.6]"), ("ydamlim", "[, 2.5e9, 2.5e9]"),
+                ("OhnakaXi", "[, 1.0e6, 1.0e6]"), ("SimonT0", "[, 1373.0, 1373.0]"),
+                ("Toff", "[, -1.0, -1.0]")]
+        r = rc * dx
+        proj = [("material", "basalt"), ("radiu", r), ("center", "[0.0, %g]" % r), ("velocity", "[0.0, %g]" % -vel)]
+        target = [("type", "plane"), ("number", 2), ("material", "[regolith, basalt]"),
+                  ("depth", "[%g, %g]" % (reg * dx, ny * dx * 2)),
+                  ("max_depth", ny * dx * 1.5), ("temperature", "[const, 293.0]")]
+        strip = 1 if tracer_strip is None else tracer_strip
+        cycle = [("dt0", 1.0e-5), ("maxdt", 1.0e-2), ("time0", 0.0), ("time1", 1000.0), ("maxstep", 100000)]
+    else:
+        raise ValueError(which)
+    dc = damp_cells if damp_cells is not None else (30 if nx >= 200 else max(4, nx // 6))
+    numerical = [("damping", "pml2"), ("DampX", dc), ("DampY", dc), ("LambdaX", 10.0), ("LambdaY", 10.0),
+                 ("tarcer_strip", strip)]
+    for k, v in over.items():
+        numerical.append((k, v))
+    sections = [
+        ("processor", [("npgx", 1), ("npgy", 1)]),
+        ("mesh", [("dx", dx), ("dy", dx), ("npx", nx), ("npy", ny), ("Ox", 0.0), ("Oy", 0.7), ("ext", "off")]),
+        ("cycle", cycle),
+        ("projectile", proj),
+        ("target", target),
+        ("condition", [("gravity", "[0.0, -1.62]")]),
+        ("boundary", [("left", "freeslip"), ("right", "freeslip"), ("bottom", "fixed"), ("top", "outflow")]),
+        ("material", mats),
+        ("numerical", numerical),
+    ]
+    inp = os.path.join(path_dir, which + ".inp")
+    with open(inp, "w") as f:
+        f.write(_inp_lines(sections))
+    return inp, eos_dir
