@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the per-timestep solver path on one MI355X.
+
+Workload: the simple-crater-with-strength configuration (1000 x 1500 cells of
+10 m, vacuum + porous-regolith layer (60 cells, rho0 1800) over basalt
+(rho0 2860), both as 256 x 256 log-spaced (rho, e) tables filled from
+Tillotson, ROCK strength with Ivanov damage, 30-layer pml2, projectile of
+radius 20 cells at 12 km/s, ~1M tracers), stepped from the moment of contact.
+
+Prints one JSON line: cell-steps/s (physical cells x steps / time of the timed
+steps, inputs resident in HBM), the roofline of the dominant kernel measured
+with HIP events on the solver's stream, and the CPU baseline (the single-thread
+C oracle on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "salec-2d-public_amd"))
+
+import numpy as np  # noqa: E402
+
+import pysalec as ps  # noqa: E402
+
+METRIC = "cell-steps/s FP64, 1 GPU: plain vs batched ensemble; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def phase_bytes(n):
+    """Algorithmic (compulsory, unique) bytes per unit for each phase: FP64
+    values read + written once per unit; vertex data shared by four cells is
+    counted once per cell.  Face packets are data-dependent intermediate
+    storage and are not counted, so flux/advect figures are lower bounds."""
+    d = 8
+    return {
+        "trace_profile": ("cell", 3 * d),
+        "vertex_vof_vel": ("vertex", (2 * n + 19) * d),
+        "update_e_vel": ("cell", 11 * d),
+        "check_v_vel": ("vertex", 7 * d),
+        "gradvel_dt": ("cell", 32 * d),
+        "vibration_stress": ("cell", (n + 29) * d),
+        "calculate_acc": ("vertex", 46 * d),
+        "calculate_v_dis": ("vertex", 8 * d),
+        "update_energy": ("cell", (4 * n + 15) * d),
+        "tracers": ("tracer", 104),
+        "flux": ("cell", (4 * n + 33) * d),
+        "advect": ("cell", (6 * n + 25) * d),
+        "damp_on_cell": ("padded_cell", 3 * d),
+        "update_state": ("cell", (4 * n + 6) * d),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--nx", type=int, default=1000)
+    ap.add_argument("--ny", type=int, default=1500)
+    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--workdir", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the solver keeps the whole mesh on one device "
+              "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    wd = args.workdir or tempfile.mkdtemp(prefix="salec_bench_")
+    t0 = time.time()
+    inp, eos = ps.write_config(os.path.join(wd, "c1"), "c1", nx=args.nx, ny=args.ny)
+    log("inputs written in %.1fs" % (time.time() - t0))
+    t0 = time.time()
+    h = ps.HostRun(inp, eos)
+    log("host setup %dx%d in %.1fs" % (h.nx, h.ny, time.time() - t0))
+    nmat = h.nmat
+    state0 = h.state()
+    ntr = ps.lib().salec_host_count_tracers(h.h)
+    dev = ps.Device(h.cfg, max_tracers=ntr)
+    t0 = time.time()
+    dev.upload_state(state0)
+    damp = dev.init(h.damp_t_in)
+    h.cfg.damp_time = damp
+    tr = h.tracers(dev.download("V_VEL"))
+    dev.upload_tracers(tr)
+    dev.set_time(h.time0, h.dt0)
+    log("device init + %d tracers in %.1fs" % (tr.n, time.time() - t0))
+    init_state = dev.state() if args.cpu_steps > 0 else None
+
+    # warmup with every phase timed: find the dominant kernel
+    dev.set_profiling(True)
+    for _ in range(args.warmup):
+        dev.step(sync=False)
+    dev.sync()
+    phases = dev.phase_times()
+    tot = sum(p[1] for p in phases)
+    for p in sorted(phases, key=lambda q: -q[1]):
+        if p[2]:
+            log("  phase %-20s %8.3f ms/step  %5.1f%%" % (p[0], p[1] / p[2], 100.0 * p[1] / max(tot, 1e-30)))
+    dom = max((p for p in phases if p[0] in phase_bytes(nmat)), key=lambda p: p[1])
+    dom_id = [p[0] for p in phases].index(dom[0])
+    log("warmup %d steps; dominant phase %s (%.3f ms/call)" % (args.warmup, dom[0], dom[1] / max(dom[2], 1)))
+
+    # timed region: only the dominant phase carries events (two records/step)
+    dev.set_profiling(1 << dom_id)
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.step(sync=False)
+    dev.sync()
+    el = time.perf_counter() - t0
+    dom_t = [p for p in dev.phase_times() if p[0] == dom[0]][0]
+    dev.set_profiling(False)
+    t_now, dt_now, nstep = dev.get_time()
+    rc, badf, badi = dev.check_finite()
+    if rc != 0:
+        log("non-finite state after timing: field %s index %d" % (badf, badi))
+    cells = h.nx * h.ny
+    ms_step = el / args.steps * 1e3
+    value = cells * args.steps / el
+    log("timed %d steps: %.3f ms/step, %.3e cell-steps/s (t=%.4e s)" % (args.steps, ms_step, value, t_now))
+
+    unit, bpu = phase_bytes(nmat)[dom[0]]
+    units = {"cell": cells, "vertex": (h.nx + 3) * (h.ny + 3), "tracer": tr.n,
+             "padded_cell": (h.nx + 4) * (h.ny + 4)}[unit]
+    avg_ms = dom_t[1] / max(dom_t[2], 1)
+    achieved = bpu * units / (avg_ms * 1e-3) / 1e9
+    b_cell = 8 * (6 * nmat + 111)
+    step_gbs = (b_cell * cells + 104 * tr.n) / (ms_step * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom[0], "bytes_per_unit": bpu, "units_per_launch": units, "unit_kind": unit,
+                "avg_launch_ms": round(avg_ms, 5),
+                "step_algorithmic_GBs": round(step_gbs, 2), "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                "step_bytes_per_cell": b_cell}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        o = po.Oracle(h.cfg, init_state, tracers=tr, t=h.time0, dt=h.dt0)
+        t0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            o.step()
+        cel = time.perf_counter() - t0
+        cpu = {"value": round(cells * args.cpu_steps / cel, 1), "unit": "cell-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": "first %d steps of the same %dx%d workload from contact (with %d tracers), "
+                         "single-thread C oracle, %.1f s" % (args.cpu_steps, h.nx, h.ny, tr.n, cel)}
+        log("cpu oracle: %d steps in %.1fs" % (args.cpu_steps, cel))
+
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "simple crater with strength: %dx%d cells, vacuum+regolith+basalt "
+                                  "(256x256 rho-e tables), ROCK+Ivanov, pml2 30 layers, %d tracers, "
+                                  "r=20 cells at 12 km/s, from contact" % (h.nx, h.ny, tr.n),
+                      "cells": cells, "tracers": int(tr.n), "nmat": nmat,
+                      "sim_time_after": t_now},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -241,9 +241,13 @@ int  salec_sync(salec_ctx *ctx);
  * SALEC_ENONFINITE with the first offending (field, flat index). */
 int  salec_check(salec_ctx *ctx, int *field, long *index);
 
-/* Per-phase timing of the last salec_step when profiling is enabled. */
-int  salec_set_profiling(salec_ctx *ctx, int on);
-int  salec_phase_times(salec_ctx *ctx, int max_phases, const char **names, float *ms);
+/* Phase timing with HIP events on the context stream.  mask: bit k times
+ * phase k (~0u = all, 0 = off); resets the accumulated totals.  Timing adds
+ * two event records per timed phase and no host synchronisation. */
+int  salec_set_profiling(salec_ctx *ctx, unsigned mask);
+/* Accumulated milliseconds and call counts per phase since the last
+ * salec_set_profiling; returns the number of phases. */
+int  salec_phase_times(salec_ctx *ctx, int max_phases, const char **names, double *total_ms, long *calls);
 
 const char *salec_strerror(int code);
 int  salec_abi_version(void);

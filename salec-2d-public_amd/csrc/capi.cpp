@@ -70,9 +70,12 @@ struct salec_ctx {
     long tr_cap = 0;
     double *d_vx = nullptr, *d_vy = nullptr;
     int initialized = 0;
-    int profiling = 0;
-    hipEvent_t ev[PH_COUNT + 1]{};
-    float phase_ms[PH_COUNT]{};
+    unsigned prof_mask = 0;              /* phases timed with HIP events   */
+    std::vector<hipEvent_t> ev_pool;     /* created on demand, reused      */
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_pending;  /* (phase, start event) */
+    double phase_ms[PH_COUNT]{};
+    long phase_calls[PH_COUNT]{};
     unsigned long long *d_nf = nullptr;
 };
 
@@ -485,7 +488,6 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     }
     TRY(dalloc(c, &c->d_vx, g.VW));
     TRY(dalloc(c, &c->d_vy, g.VH));
-    for (int k = 0; k <= PH_COUNT; ++k) TRY(hip_ok(hipEventCreate(&c->ev[k])));
 #undef TRY
     *out = c;
     return SALEC_OK;
@@ -496,7 +498,7 @@ void salec_destroy(salec_ctx *c)
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : c->owned) hipFree(p);
-    for (int k = 0; k <= PH_COUNT; ++k) if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -634,6 +636,16 @@ int salec_set_damp_time(salec_ctx *c, double damp_time)
     return SALEC_OK;
 }
 
+static hipEvent_t next_event(salec_ctx *c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
 int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
 {
     if (!c) return SALEC_EINVAL;
@@ -641,41 +653,48 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     const Grid &g = c->g;
     const Fields &F = c->F;
     const Params &P = c->P;
-    bool prof = c->profiling != 0;
-#define MARK(ph) do { if (prof) hipEventRecord(c->ev[ph], s); } while (0)
-    MARK(PH_PROFILE);
-    launch_trace_profile(s, g, F, P, c->d_prof, c->prof_chunks);
-    launch_profile_finish(s, g, F, P, c->d_prof, c->prof_chunks);
-    MARK(PH_GF_A);      run_program(c, c->gfA);
-    MARK(PH_VERTEX);    launch_vertex_vof_vel(s, g, F, P);
-    MARK(PH_RST_VEL);   run_program(c, c->rstV);
-    MARK(PH_E_VEL);     launch_update_e_vel(s, g, F);
-    MARK(PH_GF_EVEL);   run_program(c, c->gfE);
-    MARK(PH_CHECK_V);   launch_check_v_vel(s, g, F);
-    MARK(PH_GRADVEL_DT); launch_gradvel_dt(s, g, F, P, c->d_time, c->d_part, c->d_counter, c->gv_blocks);
-    MARK(PH_STRESS);    launch_stress(s, g, F, P, c->d_mats, c->d_time);
-    MARK(PH_GF_B);      run_program(c, c->gfB);
-    MARK(PH_ACC);       launch_acc(s, g, F, P, c->d_time);
-    MARK(PH_RST_ACC);   run_program(c, c->rstA);
-    MARK(PH_VDIS);      launch_v_dis(s, g, F, c->d_time);
-    MARK(PH_RST_DIS);   run_program(c, c->rstD);
-    MARK(PH_ENERGY);    launch_energy(s, g, F, c->d_mats, c->d_time);
-    MARK(PH_TRACERS);   launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time);
-    MARK(PH_FLUX);      launch_flux(s, g, F, c->bx, c->by, 0);
-    MARK(PH_GF_C);      run_program(c, c->gfC);
-    MARK(PH_FLUX_RING); launch_flux(s, g, F, c->bx, c->by, 1);
-    MARK(PH_ADVECT);    launch_advect(s, g, F, P, c->bx, c->by);
-    MARK(PH_DAMP);      launch_damp(s, g, F, P, c->d_time);
-    MARK(PH_EJECTA);    launch_ejecta_flow(s, g, F, P, c->d_time);
-    MARK(PH_STATE);     launch_update_state(s, g, F, P, c->d_mats);
-    MARK(PH_TIME);      launch_advance_time(s, c->d_time);
-    if (prof) hipEventRecord(c->ev[PH_COUNT], s);
-#undef MARK
+    const unsigned mask = c->prof_mask;
+    /* phase timing: a start event before and an end event after the phase's
+     * launches, both on the context stream; read back by salec_phase_times */
+    auto T0 = [&](int ph) {
+        if (!(mask & (1u << ph))) return;
+        size_t i = c->ev_used;
+        hipEvent_t e = next_event(c);
+        if (e) { hipEventRecord(e, s); c->ev_pending.push_back({ph, i}); }
+    };
+    auto T1 = [&](int ph) {
+        if (!(mask & (1u << ph))) return;
+        hipEvent_t e = next_event(c);
+        if (e) hipEventRecord(e, s);
+    };
+#define PHASE(ph, ...) do { T0(ph); __VA_ARGS__; T1(ph); } while (0)
+    PHASE(PH_PROFILE, launch_trace_profile(s, g, F, P, c->d_prof, c->prof_chunks);
+                      launch_profile_finish(s, g, F, P, c->d_prof, c->prof_chunks));
+    PHASE(PH_GF_A, run_program(c, c->gfA));
+    PHASE(PH_VERTEX, launch_vertex_vof_vel(s, g, F, P));
+    PHASE(PH_RST_VEL, run_program(c, c->rstV));
+    PHASE(PH_E_VEL, launch_update_e_vel(s, g, F));
+    PHASE(PH_GF_EVEL, run_program(c, c->gfE));
+    PHASE(PH_CHECK_V, launch_check_v_vel(s, g, F));
+    PHASE(PH_GRADVEL_DT, launch_gradvel_dt(s, g, F, P, c->d_time, c->d_part, c->d_counter, c->gv_blocks));
+    PHASE(PH_STRESS, launch_stress(s, g, F, P, c->d_mats, c->d_time));
+    PHASE(PH_GF_B, run_program(c, c->gfB));
+    PHASE(PH_ACC, launch_acc(s, g, F, P, c->d_time));
+    PHASE(PH_RST_ACC, run_program(c, c->rstA));
+    PHASE(PH_VDIS, launch_v_dis(s, g, F, c->d_time));
+    PHASE(PH_RST_DIS, run_program(c, c->rstD));
+    PHASE(PH_ENERGY, launch_energy(s, g, F, c->d_mats, c->d_time));
+    PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
+    PHASE(PH_FLUX, launch_flux(s, g, F, c->bx, c->by, 0));
+    PHASE(PH_GF_C, run_program(c, c->gfC));
+    PHASE(PH_FLUX_RING, launch_flux(s, g, F, c->bx, c->by, 1));
+    PHASE(PH_ADVECT, launch_advect(s, g, F, P, c->bx, c->by));
+    PHASE(PH_DAMP, launch_damp(s, g, F, P, c->d_time));
+    PHASE(PH_EJECTA, launch_ejecta_flow(s, g, F, P, c->d_time));
+    PHASE(PH_STATE, launch_update_state(s, g, F, P, c->d_mats));
+    PHASE(PH_TIME, launch_advance_time(s, c->d_time));
+#undef PHASE
     HIPCHK(hipGetLastError());
-    if (prof) {
-        HIPCHK(hipEventSynchronize(c->ev[PH_COUNT]));
-        for (int k = 0; k < PH_COUNT; ++k) hipEventElapsedTime(&c->phase_ms[k], c->ev[k], c->ev[k + 1]);
-    }
     if (!(flags & SALEC_STEP_NOSYNC) || st) {
         DevTime h{};
         HIPCHK(hipMemcpyAsync(&h, c->d_time, sizeof h, hipMemcpyDeviceToHost, s));
@@ -715,20 +734,35 @@ int salec_check(salec_ctx *c, int *field, long *index)
     return SALEC_OK;
 }
 
-int salec_set_profiling(salec_ctx *c, int on)
+int salec_set_profiling(salec_ctx *c, unsigned mask)
 {
     if (!c) return SALEC_EINVAL;
-    c->profiling = on;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->prof_mask = mask;
+    c->ev_pending.clear();
+    c->ev_used = 0;
+    for (int k = 0; k < PH_COUNT; ++k) { c->phase_ms[k] = 0.; c->phase_calls[k] = 0; }
     return SALEC_OK;
 }
 
-int salec_phase_times(salec_ctx *c, int max_phases, const char **names, float *ms)
+int salec_phase_times(salec_ctx *c, int max_phases, const char **names, double *total_ms, long *calls)
 {
     if (!c) return SALEC_EINVAL;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto &pe : c->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[pe.second], c->ev_pool[pe.second + 1]) == hipSuccess) {
+            c->phase_ms[pe.first] += ms;
+            c->phase_calls[pe.first] += 1;
+        }
+    }
+    c->ev_pending.clear();
+    c->ev_used = 0;
     int n = std::min(max_phases, (int)PH_COUNT);
     for (int k = 0; k < n; ++k) {
         if (names) names[k] = kPhaseNames[k];
-        if (ms) ms[k] = c->phase_ms[k];
+        if (total_ms) total_ms[k] = c->phase_ms[k];
+        if (calls) calls[k] = c->phase_calls[k];
     }
     return PH_COUNT;
 }

@@ -736,38 +736,36 @@ static void jac9(const double *Xi, const double *xl, double J[4])
     for (int k = 0; k < 9; ++k) J[2] += ip9(k, 1, xl) * Xi[2 * k + 1];
     for (int k = 0; k < 9; ++k) J[3] += ip9(k, 2, xl) * Xi[2 * k + 1];
 }
-static double fd2x9(const double *Xi, const double *Yi, const double *xl, int dim)
-{
-    double J[4];
-    jac9(Xi, xl, J);
-    double det = J[0] * J[3] - J[1] * J[2];
-    double iv[4];
-    iv[0] = J[3] / det; iv[3] = J[0] / det; iv[1] = -J[2] / det; iv[2] = -J[1] / det;
-    double a = 0, b = 0, c = 0;
-    for (int k = 0; k < 9; ++k) a += ip9(k, 3, xl) * Yi[k];
-    for (int k = 0; k < 9; ++k) b += ip9(k, 4, xl) * Yi[k];
-    for (int k = 0; k < 9; ++k) c += ip9(k, 5, xl) * Yi[k];
-    if (dim == 0) return a * iv[0] * iv[0] + 2.0 * b * iv[0] * iv[2] + c * iv[2] * iv[2];
-    return a * iv[1] * iv[1] + 2.0 * b * iv[1] * iv[3] + c * iv[3] * iv[3];
-}
-static double laplace2d9(const double *Xi, const double *Yi, double gamma)
+/* Laplace2d9 weights of all 9 nodes at once (src/geometry/interpolate_2d9.c:309-424).
+ * With a one-hot nodal vector the reference's nodal sums reduce exactly to a
+ * single basis term, so evaluating each quadrature point once gives the same
+ * numbers as calling Laplace2d9 nine times. */
+static void laplace2d9_all(const double *Xi, double gamma, double *wout)
 {
 #define S35 0.77459666924148337703585307995648
     static const double gp[9][2] = {{-S35, -S35}, {-S35, 0.}, {-S35, S35}, {0., -S35}, {0., 0.}, {0., S35}, {S35, -S35}, {S35, 0.}, {S35, S35}};
     static const double gw[9] = {(5.0 / 9.0) * (5.0 / 9.0), (5.0 / 9.0) * (8.0 / 9.0), (5.0 / 9.0) * (5.0 / 9.0),
                                  (8.0 / 9.0) * (5.0 / 9.0), (8.0 / 9.0) * (8.0 / 9.0), (8.0 / 9.0) * (5.0 / 9.0),
                                  (5.0 / 9.0) * (5.0 / 9.0), (5.0 / 9.0) * (8.0 / 9.0), (5.0 / 9.0) * (5.0 / 9.0)};
-    double rv = 0., ra = 0.;
+    double rv[9], ra = 0.;
+    for (int p = 0; p < 9; ++p) rv[p] = 0.;
     for (int k = 0; k < 9; ++k) {
         double xl[2] = {gp[k][0] * gamma, gp[k][1] * gamma};
-        double dxx = fd2x9(Xi, Yi, xl, 0), dyy = fd2x9(Xi, Yi, xl, 2);
         double J[4];
         jac9(Xi, xl, J);
+        double det = J[0] * J[3] - J[1] * J[2];
+        double iv[4];
+        iv[0] = J[3] / det; iv[3] = J[0] / det; iv[1] = -J[2] / det; iv[2] = -J[1] / det;
         double jd = J[0] * J[3] - J[1] * J[2];
-        rv += (gamma * gamma) * gw[k] * (dxx + dyy) * jd;
+        for (int p = 0; p < 9; ++p) {
+            double a = ip9(p, 3, xl), b = ip9(p, 4, xl), c = ip9(p, 5, xl);
+            double dxx = a * iv[0] * iv[0] + 2.0 * b * iv[0] * iv[2] + c * iv[2] * iv[2];
+            double dyy = a * iv[1] * iv[1] + 2.0 * b * iv[1] * iv[3] + c * iv[3] * iv[3];
+            rv[p] += (gamma * gamma) * gw[k] * (dxx + dyy) * jd;
+        }
         ra += (gamma * gamma) * gw[k] * 1.0 * jd;
     }
-    return rv / ra;
+    for (int p = 0; p < 9; ++p) wout[p] = rv[p] / ra;
 }
 
 static double hanning(double x) { return 0.5 * (1.0 - cos(2.0 * M_PI * x)); }
@@ -816,6 +814,9 @@ static void init_mesh(salec_host *h)
     double dly = Spacing2(h->damp_y, Yext, YeL, YeR, YdL, YdR, Ny) * h->dy;
     double dlx = Spacing2(Nx, Xext, XeL, XeR, XdL, XdR, Nx) * h->dx - Spacing2(Nx - h->damp_x, Xext, XeL, XeR, XdL, XdR, Nx) * h->dx;
     double csd = 5.0e3, pml_b = 2.3;
+    double adx = 0., ady = 0.;
+    /* cells are independent: each iteration writes only its own cell */
+#pragma omp parallel for schedule(static) reduction(max:adx, ady)
     for (int j = 0; j < CH; ++j)
         for (int k = 0; k < CW; ++k) {
             long c = k + (long)CW * j;
@@ -838,8 +839,8 @@ static void init_mesh(salec_host *h)
             case SALEC_DAMP_PML: case SALEC_DAMP_PML2: case SALEC_DAMP_PML3:
                 if (ddy <= dly) es[NC + c] = pow(1.0 - ddy / dly, 2.0) / (2.0 * dly) * (3.0 * csd) * h->lambda_y;
                 if (ddx <= dlx) es[c] = pow(1.0 - ddx / dlx, 2.0) / (2.0 * dlx) * (3.0 * csd) * h->lambda_x;
-                h->cfg.alpha_damp_x = (csd * 2.0) / dlx * pml_b;
-                h->cfg.alpha_damp_y = (csd * 2.0) / dly * pml_b;
+                adx = (csd * 2.0) / dlx * pml_b;
+                ady = (csd * 2.0) / dly * pml_b;
                 break;
             case SALEC_DAMP_SBC:
                 es[c] = es[NC + c] = 1.0;
@@ -853,7 +854,12 @@ static void init_mesh(salec_host *h)
                 break;
             }
         }
+    if (h->cfg.damp_type == SALEC_DAMP_PML || h->cfg.damp_type == SALEC_DAMP_PML2 || h->cfg.damp_type == SALEC_DAMP_PML3) {
+        h->cfg.alpha_damp_x = adx;
+        h->cfg.alpha_damp_y = ady;
+    }
     static const int voff[4] = {2, 3, 0, 1};
+#pragma omp parallel for schedule(static)
     for (int j = 0; j < VH; ++j)
         for (int k = 0; k < VW; ++k) {
             int xd = k < VW - 1 - k ? k : VW - 1 - k, yd = j < VH - 1 - j ? j : VH - 1 - j;
@@ -869,18 +875,15 @@ static void init_mesh(salec_host *h)
                 h->f[SALEC_V_SUF][(2 * i) * NV + v] = esuf[(voff[i] * 2) * NC + e[i]];
                 h->f[SALEC_V_SUF][(2 * i + 1) * NV + v] = esuf[(voff[i] * 2 + 1) * NC + e[i]];
             }
-            double xi[18], yi[9];
+            double xi[18], wl[9];
             int q = 0;
             for (int dj = -1; dj <= 1; ++dj)
                 for (int dk = -1; dk <= 1; ++dk, ++q) {
                     long w = (k + dk) + (long)VW * (j + dj);
                     xi[2 * q] = vpos[w]; xi[2 * q + 1] = vpos[NV + w];
                 }
-            for (int p = 0; p < 9; ++p) {
-                for (int r = 0; r < 9; ++r) yi[r] = 0.;
-                yi[p] = 1.;
-                h->f[SALEC_V_LAPLACE][p * NV + v] = laplace2d9(xi, yi, 0.5);
-            }
+            laplace2d9_all(xi, 0.5, wl);
+            for (int p = 0; p < 9; ++p) h->f[SALEC_V_LAPLACE][p * NV + v] = wl[p];
         }
     free(esuf); free(esub);
     /* face normals, src/construct.c:311-354 */

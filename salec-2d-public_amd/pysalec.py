@@ -135,8 +135,8 @@ def lib():
     L.salec_step.argtypes = [vp, C.c_uint, C.POINTER(Stats)]
     L.salec_sync.argtypes = [vp]
     L.salec_check.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)]
-    L.salec_set_profiling.argtypes = [vp, C.c_int]
-    L.salec_phase_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_float)]
+    L.salec_set_profiling.argtypes = [vp, C.c_uint]
+    L.salec_phase_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.POINTER(C.c_long)]
     L.salec_strerror.argtypes = [C.c_int]
     L.salec_strerror.restype = C.c_char_p
     L.salec_field_name.argtypes = [C.c_int]
@@ -290,14 +290,18 @@ class Device:
         tr.n = s.n
         return tr
 
-    def set_profiling(self, on):
-        check(lib().salec_set_profiling(self.ctx, 1 if on else 0), "salec_set_profiling")
+    def set_profiling(self, mask):
+        """mask: True/all phases, False/off, or an int bit mask of phases."""
+        m = 0xFFFFFFFF if mask is True else (0 if mask is False else int(mask))
+        check(lib().salec_set_profiling(self.ctx, m), "salec_set_profiling")
 
     def phase_times(self):
+        """[(phase name, total ms, calls)] accumulated since set_profiling."""
         names = (C.c_char_p * 64)()
-        ms = (C.c_float * 64)()
-        n = lib().salec_phase_times(self.ctx, 64, names, ms)
-        return [(names[k].decode(), ms[k]) for k in range(n)]
+        ms = (C.c_double * 64)()
+        calls = (C.c_long * 64)()
+        n = lib().salec_phase_times(self.ctx, 64, names, ms, calls)
+        return [(names[k].decode(), ms[k], calls[k]) for k in range(n)]
 
     def check_finite(self):
         f, i = C.c_int(), C.c_long()
