@@ -11,6 +11,22 @@
  * the reference's padded grids) so that states can be compared element by
  * element.  Loops run in the reference's order and ghost tables are applied
  * entry by entry in the reference's index order.
+ *
+ * Pinned (tests/test_oracle_eos.py, tests/test_oracle_remap_strength.py):
+ * the EOS closure (calculate_state[_correct], Tillotson pressure, sound
+ * speed and cold table, table lookup, bisearch) against vectors written by
+ * the reference's own EOS sources and against thermodynamics; the remap
+ * geometry (swept volume, cut length, edge ratio, MSTACS) and the strength
+ * models against closed forms and limits.
+ *
+ * Parity unpinned: update_v_vof, update_v_vel_inc, update_e_vel,
+ * check_v_vel, update_e_gradvel + calculate_dt, block_vibration, the stress
+ * update around the strength functions, calculate_acc, calculate_v_dis,
+ * update_energy, the composition of calculate_flux_mixed and
+ * advect_flux_default, damp_on_cell, ejecta_flow, the mixture rules of
+ * update_state, ghost fills/restrictions, fl_update_node and trace_profile
+ * have no reference vector or closed form; they are checked only as a whole
+ * (rest state, lithostatic equilibrium, mass conservation, a recorded run).
  */
 #ifndef SALEC_ORACLE_H
 #define SALEC_ORACLE_H
@@ -45,6 +61,8 @@ void oracle_step(oracle_state *s, const salec_config *cfg);
 void oracle_update_state(oracle_state *s, const salec_config *cfg);
 double oracle_calculate_state(const salec_material *m, double den, double eng,
                               double *tem, double *pre, double *csd);
+double oracle_calculate_state_correct(const salec_material *m, double den, double eng, double plim,
+                                      double *tem, double *pre, double *csd, double pty);
 double oracle_strength_fn(int fn, const double *paras, const salec_material *m);
 double oracle_get_flux_volcyl(const double *a, const double *b, const double *da, const double *db);
 double oracle_cut_length(double vf, double a, double b);

@@ -49,6 +49,8 @@ def olib():
     d, dp = C.c_double, C.POINTER(C.c_double)
     L.oracle_calculate_state.argtypes = [MP, d, d, dp, dp, dp]
     L.oracle_calculate_state.restype = d
+    L.oracle_calculate_state_correct.argtypes = [MP, d, d, d, dp, dp, dp, d]
+    L.oracle_calculate_state_correct.restype = d
     L.oracle_strength_fn.argtypes = [C.c_int, dp, MP]
     L.oracle_strength_fn.restype = d
     L.oracle_get_flux_volcyl.argtypes = [dp, dp, dp, dp]

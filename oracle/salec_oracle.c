@@ -314,6 +314,11 @@ static double calculate_state_correct(const salec_material *m, double den, doubl
     *tem = tt; *pre = tp; *csd = tc;
     return cf;
 }
+double oracle_calculate_state_correct(const salec_material *m, double den, double eng, double plim,
+                                      double *tem, double *pre, double *csd, double pty)
+{
+    return calculate_state_correct(m, den, eng, plim, tem, pre, csd, &pty);
+}
 
 /* ------------------------------------------------------------ strength -- */
 /* src/strength.c:505-575 */

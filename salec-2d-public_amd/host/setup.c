@@ -323,7 +323,8 @@ static int boundary_str2opt(const char *s, int is_lr)
     return SALEC_GHT_ZERO;
 }
 
-#define ERR(...) do { if (err) snprintf(err, errlen, __VA_ARGS__); salec_host_free(h); return rc; } while (0)
+#define ERR(...) do { if (err) snprintf(err, errlen, __VA_ARGS__); salec_host_free(h); inp_close(f); \
+                          return rc == SALEC_OK ? SALEC_EINVAL : rc; } while (0)
 
 int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out, char *err, int errlen)
 {
@@ -375,6 +376,7 @@ int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out,
     else if (!strcasecmp(buf, "sbc")) { c->damp_type = SALEC_DAMP_SBC; h->lambda_x = Wind(h->lambda_x, 0.0, 0.5); h->lambda_y = Wind(h->lambda_y, 0.0, 0.5); }
     else if (!strcasecmp(buf, "ctlg") || !strcasecmp(buf, "rock")) c->damp_type = SALEC_DAMP_CTLG;
     else { c->damp_type = SALEC_DAMP_NONE; h->damp_x = h->damp_y = 0; h->lambda_x = h->lambda_y = 0.; h->damp_t_in = -1.0; h->damp_x_left = h->damp_y_top = 0; }
+    if (c->damp_type == SALEC_DAMP_PML3) { rc = SALEC_EINVAL; ERR("damping 'pml3' is not supported"); }
     /* cycle, src/sale2d.c:6-13 */
     h->time0 = inp_d(f, "cycle.time0", 0.0); h->time1 = inp_d(f, "cycle.time1", 100.0);
     h->maxdt = inp_d(f, "cycle.maxdt", 1.0); h->dt0 = inp_d(f, "cycle.dt0", 0.0);
@@ -1115,4 +1117,26 @@ int salec_host_init_tracers(salec_host *h, salec_tracers *tr)
     }
     tr->n = n;
     return SALEC_OK;
+}
+
+void salec_crater_metrics(const double *yprof, const double *col_r, int nx, double *depth, double *diam)
+{
+    double zmin = yprof[2];
+    for (int i = 3; i < nx + 2; ++i) zmin = yprof[i] < zmin ? yprof[i] : zmin;
+    *depth = -zmin;
+    *diam = 0.0;
+    if (!(yprof[2] < 0.0)) return;
+    for (int i = 3; i < nx + 2; ++i) {
+        if (yprof[i] >= 0.0) {
+            double y0 = yprof[i - 1], y1 = yprof[i];
+            *diam = 2.0 * (col_r[i - 1] + (0.0 - y0) * (col_r[i] - col_r[i - 1]) / (y1 - y0));
+            return;
+        }
+    }
+    *diam = 2.0 * col_r[nx + 1];
+}
+
+void salec_host_laplace9(const double *xi, double gamma, double *w)
+{
+    laplace2d9_all(xi, gamma, w);
 }

@@ -52,6 +52,21 @@ int  salec_write_sd_table(const char *path, const double p[12], int rhostep,
                           int nden, int neng, double rho_min, double rho_max,
                           double e_min, double e_max);
 
+/* The nine 9-point Laplacian weights of one vertex stencil: xi holds the
+ * (r, z) of the 3 x 3 vertices, row by row from the bottom-left; the weight
+ * of node k is Laplace2d9(xi, one-hot k, gamma) of
+ * src/geometry/interpolate_2d9.c:398-424 (generate2d_laplace,
+ * src/construct.c:585-594, uses gamma = 0.5). */
+void salec_host_laplace9(const double *xi, double gamma, double *w);
+
+/* Transient crater metrics from the crater profile y_profile (trace_profile,
+ * src/ejecta.c:8-63) of one step: depth = -min over the physical columns of
+ * the profile; diameter = twice the radius at which the profile, walking
+ * out from the axis, first rises back to z = 0 (linear between the two
+ * column centres that bracket the crossing; 0 if it never dips below 0).
+ * yprof and col_r are indexed by padded column (nx + 4 entries). */
+void salec_crater_metrics(const double *yprof, const double *col_r, int nx, double *depth, double *diam);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,10 @@ TRANSIENT = ["E_VEL", "E_GRAD_VEL", "E_DIV_VEL", "E_Q", "E_STA2S", "E_STRENGTH",
              "V_DIS", "Y_PROFILE"]
 
 SALEC_OK = 0
+# strength/damage/tensile/porosity model registry (salec.h SALEC_FN_*)
+(SALEC_FN_SIMPLEROCK, SALEC_FN_SIMPLESHEAR, SALEC_FN_JOHNSONCOOK1, SALEC_FN_JOHNSONCOOK2, SALEC_FN_ROCKSTRENGTHT,
+ SALEC_FN_ICESTRENGTH, SALEC_FN_COLLINS, SALEC_FN_IVANOV, SALEC_FN_SIMPLETENSILE, SALEC_FN_NONE, SALEC_FN_WUNNEMANN,
+ SALEC_FN_VONMISES1, SALEC_FN_VONMISES2) = range(13)
 DAMP = {"pml": 0, "pml2": 1, "pml3": 2, "sbc": 3, "ctlg": 4, "none": 5}
 
 
@@ -119,6 +123,9 @@ def lib():
     L.salec_host_count_tracers.restype = C.c_long
     L.salec_host_init_tracers.argtypes = [vp, C.POINTER(Tracers)]
     L.salec_host_set_damp_time.argtypes = [vp, C.c_double]
+    L.salec_host_laplace9.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
+    L.salec_crater_metrics.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.salec_write_tillotson.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int]
     L.salec_write_sd_table.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int,
                                        C.c_double, C.c_double, C.c_double, C.c_double]
@@ -320,6 +327,23 @@ class Device:
             pass
 
 
+def laplace9(xi, gamma=0.5):
+    """9-point Laplacian weights of one 3 x 3 vertex stencil (salec_host_laplace9)."""
+    x = np.ascontiguousarray(xi, dtype=np.float64).reshape(18)
+    w = np.zeros(9)
+    lib().salec_host_laplace9(_dp(x), gamma, _dp(w))
+    return w
+
+
+def crater_metrics(y_profile, col_r, nx):
+    """(depth, diameter) of the transient crater from a y_profile download."""
+    y = np.ascontiguousarray(y_profile, dtype=np.float64).reshape(-1)
+    r = np.ascontiguousarray(col_r, dtype=np.float64).reshape(-1)
+    d, D = C.c_double(), C.c_double()
+    lib().salec_crater_metrics(_dp(y), _dp(r), nx, C.byref(d), C.byref(D))
+    return d.value, D.value
+
+
 # --------------------------------------------------------------- inputs ---
 # Tillotson constants: Rho0, Cv, A, B, E0, a, b, alpha, beta, Eiv, Ecv, Tref
 GRANITE = [2680.0, 8.96e2, 1.8e10, 1.8e10, 1.6e7, 0.5, 1.3, 5.0, 5.0, 3.5e6, 1.8e7, 293.0]
@@ -356,7 +380,7 @@ def _inp_lines(sections):
 
 
 def write_config(path_dir, which, nx=None, ny=None, table_n=(256, 256), tracer_strip=None, proj_cells=None,
-                 reg_cells=None, velocity=None, damp_cells=None, **over):
+                 reg_cells=None, velocity=None, damp_cells=None, sections_over=None, **over):
     """Write the .inp and EOS files of a benchmark configuration.
 
     which = "c0": plain hydro, one granite Tillotson material, no strength,
@@ -366,7 +390,10 @@ def write_config(path_dir, which, nx=None, ny=None, table_n=(256, 256), tracer_s
                   filled from Tillotson, ROCK + Ivanov damage, 1000 x 1500
                   cells, projectile radius 20 cells at 12 km/s, ~1M tracers.
     Cells are 10 m.  nx, ny shrink the mesh for tests; the projectile radius
-    and regolith thickness (in cells) shrink with nx unless given.
+    and regolith thickness (in cells) shrink with nx unless given.  Keyword
+    arguments add or replace [numerical] keys; sections_over maps
+    "section.key" to a value and adds or replaces any key (material models,
+    boundaries, gravity ...).
     """
     os.makedirs(path_dir, exist_ok=True)
     eos_dir = os.path.join(path_dir, "eos")
@@ -415,8 +442,6 @@ def write_config(path_dir, which, nx=None, ny=None, table_n=(256, 256), tracer_s
     dc = damp_cells if damp_cells is not None else (30 if nx >= 200 else max(4, nx // 6))
     numerical = [("damping", "pml2"), ("DampX", dc), ("DampY", dc), ("LambdaX", 10.0), ("LambdaY", 10.0),
                  ("tarcer_strip", strip)]
-    for k, v in over.items():
-        numerical.append((k, v))
     sections = [
         ("processor", [("npgx", 1), ("npgy", 1)]),
         ("mesh", [("dx", dx), ("dy", dx), ("npx", nx), ("npy", ny), ("Ox", 0.0), ("Oy", 0.7), ("ext", "off")]),
@@ -428,6 +453,21 @@ def write_config(path_dir, which, nx=None, ny=None, table_n=(256, 256), tracer_s
         ("material", mats),
         ("numerical", numerical),
     ]
+    merged = {"numerical." + k: v for k, v in over.items()}
+    merged.update(sections_over or {})
+    for sk, v in merged.items():
+        sec, key = sk.split(".", 1)
+        for name, kv in sections:
+            if name == sec:
+                for i, (k, _) in enumerate(kv):
+                    if k.lower() == key.lower():
+                        kv[i] = (k, v)
+                        break
+                else:
+                    kv.append((key, v))
+                break
+        else:
+            sections.append((sec, [(key, v)]))
     inp = os.path.join(path_dir, which + ".inp")
     with open(inp, "w") as f:
         f.write(_inp_lines(sections))

@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Writes the golden vectors under tests/golden/.
+
+The reference holds no golden vectors of its own.  These come from:
+  * oracle/_ref/libref_eos.so -- the reference's own EOS and geometry sources
+    (src/eos/eos_state.c, src/eos/aneos_sieden.c, src/geometry/*.c,
+    src/io/InputParser.c) compiled unchanged by `make -C oracle ref`:
+      eos_ref.npz      calculate_state / calculate_state_correct on seeded
+                       (rho, rho*e) samples of the three test EOS files,
+                       state_reference values and the Tillotson cold table;
+      laplace_ref.npz  Laplace2d9 weights (gamma 0.5) of seeded stencils;
+      bisearch_ref.npz bisearch on seeded sorted lists;
+  * the oracle itself, as a regression record of the composed step:
+      c1_small_oracle.npz  the 24 x 36 crater-with-strength case after
+                           initialisation and after 12 steps, with the dt
+                           sequence.
+Run from the repository root after `make -C oracle all ref` and building
+the product library (the host setup writes the EOS files and inputs).
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "salec-2d-public_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import pysalec as ps  # noqa: E402
+import refeos  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+SMALL = dict(nx=24, ny=36)
+
+EOS_CASES = [("c0", "granite.tillotson", refeos.EOS_TILLOTSON, 1),
+             ("c1", "regolith.aneos.sd", refeos.EOS_SIEDEN, 1),
+             ("c1", "basalt.aneos.sd", refeos.EOS_SIEDEN, 2)]
+
+
+def eos_samples(seed, n=4000):
+    rng = np.random.default_rng(seed)
+    den = 10.0 ** rng.uniform(-1.5, 4.3, n)
+    e_sp = 10.0 ** rng.uniform(-2.0, 9.2, n)
+    e_sp[rng.random(n) < 0.05] = 0.0
+    e_sp[rng.random(n) < 0.03] *= -1e-3
+    return den, den * e_sp
+
+
+def write_eos_ref(wd):
+    out = {}
+    for ci, (which, fn, ty, m) in enumerate(EOS_CASES):
+        inp, eos = ps.write_config(os.path.join(wd, which), which, **SMALL)
+        E = refeos.RefEOS(os.path.join(eos, fn), ty)
+        den, eng = eos_samples(100 + ci)
+        res = np.array([E.state(a, b) for a, b in zip(den, eng)])
+        plim = np.where(np.arange(den.size) % 2 == 0, -1.0e7, 0.0)
+        resc = np.array([E.state_correct(a, b, p) for a, b, p in zip(den, eng, plim)])
+        k = "%s_%s" % (which, fn.split(".")[0])
+        out[k + "_den"], out[k + "_eng"], out[k + "_plim"] = den, eng, plim
+        out[k + "_state"], out[k + "_state_correct"] = res, resc
+        sr = E.state_reference()
+        out[k + "_stateref"] = np.array([sr[n] for n in sorted(sr)])
+        out[k + "_stateref_names"] = np.array(sorted(sr))
+        if ty == refeos.EOS_TILLOTSON:
+            xd, ye, sd = E.cold_table()
+            out[k + "_cold_xden"], out[k + "_cold_yeng"] = np.array(xd), np.array(ye)
+    np.savez_compressed(os.path.join(OUT, "eos_ref.npz"), **out)
+
+
+def write_laplace_ref():
+    R = refeos.rlib()
+    import ctypes as C
+    rng = np.random.default_rng(7)
+    n = 64
+    pos = np.zeros((n, 18))
+    w = np.zeros((n, 9))
+    for s in range(n):
+        dx, dy = 10.0 ** rng.uniform(-1, 2), 10.0 ** rng.uniform(-1, 2)
+        q = 0
+        for j in (-1, 0, 1):
+            for k in (-1, 0, 1):
+                jit = 0.0 if s < 8 else 0.15
+                pos[s, 2 * q] = 50.0 + k * dx * (1 + jit * rng.uniform(-1, 1))
+                pos[s, 2 * q + 1] = -20.0 + j * dy * (1 + jit * rng.uniform(-1, 1))
+                q += 1
+        xi = (C.c_double * 18)(*pos[s])
+        for t in range(9):
+            yi = (C.c_double * 9)(*[1.0 if u == t else 0.0 for u in range(9)])
+            w[s, t] = R.ref_laplace2d9(xi, yi, 0.5)
+    np.savez_compressed(os.path.join(OUT, "laplace_ref.npz"), pos=pos, w=w)
+
+
+def write_bisearch_ref():
+    R = refeos.rlib()
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    lists, xs, idx = [], [], []
+    for s in range(40):
+        n = int(rng.integers(2, 40))
+        xl = np.sort(rng.uniform(-5, 5, n))
+        arr = (C.c_double * n)(*xl)
+        x = np.concatenate([rng.uniform(-6, 6, 20), xl, [xl[0] - 1, xl[-1] + 1]])
+        lists.append(np.pad(xl, (0, 40 - n), constant_values=np.nan))
+        xs.append(np.pad(x, (0, 62 - x.size), constant_values=np.nan))
+        idx.append(np.pad([R.ref_bisearch(v, arr, n) for v in x], (0, 62 - x.size), constant_values=-9))
+    np.savez_compressed(os.path.join(OUT, "bisearch_ref.npz"), lists=np.array(lists), xs=np.array(xs),
+                        idx=np.array(idx))
+
+
+def write_oracle_small(wd, steps=12):
+    import parity as P
+    h, o = P.make_run("c1", workdir=os.path.join(wd, "c1run"), **SMALL)
+    init = {k: v.copy() for k, v in o.state.items()}
+    dts = []
+    for _ in range(steps):
+        o.step()
+        dts.append(o.dt)
+    out = {"damp_time": h.cfg.damp_time, "dts": np.array(dts), "t": o.t}
+    for k in P.COMPARED:
+        out["init_" + k] = init[k]
+        out["step_" + k] = o.state[k]
+    np.savez_compressed(os.path.join(OUT, "c1_small_oracle.npz"), **out)
+
+
+if __name__ == "__main__":
+    if not refeos.available():
+        sys.exit("oracle/_ref/libref_eos.so missing: run make -C oracle ref")
+    wd = tempfile.mkdtemp(prefix="salec_golden_")
+    write_eos_ref(wd)
+    write_laplace_ref()
+    write_bisearch_ref()
+    write_oracle_small(wd)
+    for f in sorted(os.listdir(OUT)):
+        print(f, os.path.getsize(os.path.join(OUT, f)))

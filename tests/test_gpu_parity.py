@@ -1,0 +1,252 @@
+"""GPU parity of the HIP step path against the oracle, through the C ABI.
+
+Bar (per step, from identical state): relative L-infinity <= 1e-11 on every
+compared FP64 field, exact equality of the integer-valued state (material
+masks, vacuum flags, void-corner counts, damage and vibration flags) and of
+tracer cells/liveness.  The kernels are compiled without FMA contraction and
+follow the oracle's operand order, so in practice the states are equal bit
+for bit; the tests assert the bar, and report the worst difference seen.
+
+Cases: the two benchmark configurations on small meshes (c0: one Tillotson
+material, plain hydro; c1: two tabulated materials with ROCK strength,
+Ivanov damage, pml2 and tracers), the committed golden run, and variants
+that switch on the other models and boundary/damping options.  At the full
+c1 size the tests check size-independent properties (determinism, finite
+state, volume fractions summing to one).
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import parity as P
+import pyoracle as po
+import pysalec as ps
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg):
+    print("[gpu-parity] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def setup_pair(which, nx, ny, tracers=False, **kw):
+    """Host setup, oracle init and device init from the same uploaded state."""
+    h = ps.HostRun(*ps.write_config(os.path.join(tempfile.mkdtemp(prefix="gp_"), which), which, nx=nx, ny=ny, **kw))
+    pre = h.state()
+    o = po.Oracle(h.cfg, pre, t=h.time0, dt=h.dt0)
+    damp = o.init()
+    ntr = ps.lib().salec_host_count_tracers(h.h) if tracers else 0
+    dev = ps.Device(h.cfg, max_tracers=max(ntr, 1))
+    dev.upload_state(pre)
+    dd = dev.init(h.damp_t_in)
+    assert dd == damp
+    h.cfg.damp_time = damp
+    tr = None
+    if tracers:
+        tr = h.tracers(o.state["V_VEL"])
+        o = po.Oracle(h.cfg, o.state, tracers=tr, t=h.time0, dt=h.dt0)
+    return h, o, dev, tr
+
+
+def reseeded(h, o, dev, steps, tracers=False):
+    """Each step starts both sides from the oracle's state; returns the worst
+    relative difference seen."""
+    worst = 0.0
+    for k in range(steps):
+        dev.upload_state(o.state)
+        dev.set_time(o.t, o.dt)
+        if tracers:
+            dev.upload_tracers(o.tr)
+        o.step()
+        s = dev.step()
+        assert s.dt == o.dt, "step %d: dt %r vs %r" % (k, s.dt, o.dt)
+        g = dev.state()
+        wf, w, mism = P.compare(g, o.state, h.nmat)
+        bad = {a: b for a, b in mism.items() if b}
+        assert w <= P.FIELD_RTOL and not bad, "step %d: worst %s %.3e, integer-state mismatches %s" % (k, wf, w, bad)
+        if tracers:
+            tc = P.tracer_compare(dev.download_tracers(o.tr.n), o.tr)
+            assert all(tc[q] == 0 for q in ("idx", "idy", "alive", "matid")), "step %d tracers %s" % (k, tc)
+            assert tc["pos"] <= P.FIELD_RTOL and tc["vel"] <= P.FIELD_RTOL, "step %d tracers %s" % (k, tc)
+        worst = max(worst, w)
+    return worst
+
+
+@pytest.mark.parametrize("which,nx,ny", [("c0", 48, 72), ("c1", 48, 72)])
+def test_init_parity(which, nx, ny):
+    say("init %s %dx%d" % (which, nx, ny))
+    h, o, dev, _ = setup_pair(which, nx, ny)
+    g = dev.state()
+    wf, w, mism = P.compare(g, o.state, h.nmat, fields=[f[0] for f in ps.FIELDS])
+    assert w == 0.0 and not any(mism.values()), (wf, w, mism)
+    dev.close()
+
+
+@pytest.mark.parametrize("which,nx,ny,steps", [("c0", 48, 72, 40), ("c1", 48, 72, 40)])
+def test_step_parity_reseeded(which, nx, ny, steps):
+    say("reseeded %s %dx%d, %d steps" % (which, nx, ny, steps))
+    h, o, dev, tr = setup_pair(which, nx, ny, tracers=(which == "c1"))
+    if tr is not None:
+        dev.upload_tracers(tr)
+    w = reseeded(h, o, dev, steps, tracers=(tr is not None))
+    say("  worst relative difference %.3e" % w)
+    dev.close()
+
+
+@pytest.mark.parametrize("which", ["c0", "c1"])
+def test_lockstep_parity(which):
+    """Both sides run 60 steps on their own; any rounding difference would
+    grow, so this checks that none appears."""
+    say("lockstep %s 40x60, 60 steps" % which)
+    h, o, dev, tr = setup_pair(which, 40, 60, tracers=(which == "c1"))
+    dev.upload_state(o.state)
+    dev.set_time(o.t, o.dt)
+    if tr is not None:
+        dev.upload_tracers(o.tr)
+    for _ in range(60):
+        o.step()
+        dev.step(sync=False)
+    dev.sync()
+    t, dt, st = dev.get_time()
+    assert (t, dt) == (o.t, o.dt)
+    wf, w, mism = P.compare(dev.state(), o.state, h.nmat)
+    assert w <= P.FIELD_RTOL and not any(mism.values()), (wf, w, mism)
+    if tr is not None:
+        tc = P.tracer_compare(dev.download_tracers(o.tr.n), o.tr)
+        assert all(tc[k] == 0 for k in ("idx", "idy", "alive", "matid")), tc
+        assert tc["pos"] <= P.FIELD_RTOL, tc
+    dev.close()
+
+
+def test_golden_run():
+    """The device reproduces the committed oracle record of the 24 x 36
+    crater-with-strength run (12 steps) without running the oracle."""
+    say("golden c1 24x36")
+    g = np.load(os.path.join(GOLD, "c1_small_oracle.npz"))
+    h = ps.HostRun(*ps.write_config(os.path.join(tempfile.mkdtemp(prefix="gg_"), "c1"), "c1", nx=24, ny=36))
+    dev = ps.Device(h.cfg, max_tracers=1)
+    dev.upload_state(h.state())
+    assert dev.init(h.damp_t_in) == g["damp_time"]
+    s = dev.state()
+    for k in P.COMPARED:
+        assert P.rel_linf(s[k], g["init_" + k]) == 0.0, k
+    dev.set_time(h.time0, h.dt0)
+    dts = [dev.step().dt for _ in range(len(g["dts"]))]
+    assert np.array_equal(dts, g["dts"])
+    s = dev.state()
+    for k in P.COMPARED:
+        assert P.rel_linf(s[k], g["step_" + k]) <= P.FIELD_RTOL, k
+    dev.close()
+
+
+VARIANTS = {
+    "damp_pml": ("c1", dict(damping="pml")),
+    # the reference's sponge layer adds the reference state on top of the
+    # damped one in y (src/velocity.c:746-747), so densities in the bottom
+    # layer grow geometrically and dt collapses within ~20 steps: compare the
+    # first steps only
+    "damp_sbc": ("c1", dict(damping="sbc", LambdaX=0.3, LambdaY=0.3), 0, 10),
+    "damp_off": ("c0", dict(damping="off")),
+    "outflow_right_freeslip_bottom": ("c0", dict(sections_over={"boundary.right": "outflow",
+                                                               "boundary.bottom": "freeslip"})),
+    # SimpleRock is registry entry 0, which str2func refuses (src/strength.c:590)
+    "simpleshear_damage": ("c1", dict(sections_over={"material.damage": "[, SimpleShear, Ivanov]"})),
+    "johnsoncook2_collins": ("c1", dict(sections_over={"material.yshear": "[, JohnsonCook2, JohnsonCook1]",
+                                                      "material.damage": "[, Collins, Collins]"})),
+    "vonmises": ("c1", dict(sections_over={"material.yshear": "[, VonMises1, VonMises2]",
+                                          "material.ytens": "[, None, SimpleTensile]"})),
+    "ice_strength": ("c1", dict(sections_over={"material.yshear": "[, IceStrength, RockStrengthT]"})),
+    "acoustic_fluidisation": ("c1", dict(sections_over={"material.Toff": "[, 16.0, 16.0]",
+                                                       "material.Cvib": "[, 0.1, 0.1]",
+                                                       "material.BlockSize": "[, 50.0, 50.0]"})),
+    "tensile_failure_partpres": ("c1", dict(tensile_failure=1, partpres=1)),
+    "ejecta_resistance": ("c1", dict(sections_over={"ejecta.resistance": "0.5", "ejecta.friction": "0.2",
+                                                   "ejecta.threshold": "1.0"})),
+    "anc_uniform_stretched_mesh": ("c0", dict(ANCPAT="uniform", ANC=0.3,
+                                              sections_over={"mesh.ext": "on", "mesh.ex": "[1.04, 6, 6]",
+                                                             "mesh.ey": "[1.04, 6, 6]"})),
+}
+
+
+@pytest.mark.parametrize("name", sorted(VARIANTS))
+def test_variant_parity(name):
+    which, kw = VARIANTS[name][:2]
+    ahead, steps = VARIANTS[name][2:] if len(VARIANTS[name]) > 2 else (15, 25)
+    say("variant %s (%s 32x48, %d steps after %d)" % (name, which, steps, ahead))
+    h, o, dev, tr = setup_pair(which, 32, 48, tracers=(which == "c1"), **kw)
+    if tr is not None:
+        dev.upload_tracers(tr)
+    # run the oracle ahead so that the compared steps have developed
+    # damage, yielding, mixed cells and ejecta
+    for _ in range(ahead):
+        o.step()
+    w = reseeded(h, o, dev, steps, tracers=(tr is not None))
+    say("  worst relative difference %.3e" % w)
+    dev.close()
+
+
+def test_device_determinism_and_reinit():
+    """Two contexts stepping the same state give identical results, and a
+    context reused after re-upload repeats itself."""
+    say("determinism c1 40x60")
+    h, o, dev, tr = setup_pair("c1", 40, 60, tracers=True)
+    s0 = dev.state()
+    dev2 = ps.Device(h.cfg, max_tracers=tr.n)
+    runs = []
+    for d in (dev, dev2, dev):
+        d.upload_state(s0)
+        d.set_damp_time(h.cfg.damp_time)
+        d.set_time(h.time0, h.dt0)
+        d.upload_tracers(tr)
+        for _ in range(30):
+            d.step(sync=False)
+        d.sync()
+        runs.append((d.state(), d.download_tracers(tr.n)))
+    for s, t in runs[1:]:
+        for k in P.COMPARED:
+            assert np.array_equal(s[k], runs[0][0][k], equal_nan=True), k
+        assert np.array_equal(t.pos, runs[0][1].pos)
+    dev.close()
+    dev2.close()
+
+
+def test_full_size_c1_properties():
+    """The 1000 x 1500 benchmark workload: 60 steps from contact stay finite,
+    volume fractions sum to one in every physical cell, the material masses
+    stay positive, and a second context reproduces the state bit for bit."""
+    say("full size c1 1000x1500: setup")
+    h = ps.HostRun(*ps.write_config(os.path.join(tempfile.mkdtemp(prefix="gf_"), "c1"), "c1"))
+    st0 = h.state()
+    ntr = ps.lib().salec_host_count_tracers(h.h)
+    out = []
+    for rep in range(2):
+        dev = ps.Device(h.cfg, max_tracers=ntr)
+        dev.upload_state(st0)
+        damp = dev.init(h.damp_t_in)
+        h.cfg.damp_time = damp
+        tr = h.tracers(dev.download("V_VEL"))
+        dev.upload_tracers(tr)
+        dev.set_time(h.time0, h.dt0)
+        say("  context %d: %d tracers, stepping" % (rep, tr.n))
+        for _ in range(60):
+            dev.step(sync=False)
+        dev.sync()
+        rc, f, i = dev.check_finite()
+        assert rc == 0, (f, i)
+        out.append(dev.state(["M_VOF", "M_DEN", "M_ENG", "V_VEL", "E_STE", "E_DAM", "Y_PROFILE"]))
+        trd = dev.download_tracers(tr.n)
+        out[-1]["tr_pos"] = trd.pos
+        dev.close()
+    a, b = out
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    vof = a["M_VOF"][:, 2:-2, 2:-2]
+    assert np.abs(vof.sum(axis=0) - 1.0).max() <= 1e-12
+    assert (a["M_DEN"][:, 2:-2, 2:-2] >= 0).all() and (a["M_ENG"][:, 2:-2, 2:-2] >= 0).all()
+    assert np.abs(a["V_VEL"]).max() > 1000.0      # the impact is under way
+    say("  max |v| %.1f m/s, max damage %.3f" % (np.abs(a["V_VEL"]).max(), a["E_DAM"].max()))

@@ -1,0 +1,212 @@
+"""CPU tests of the library boundary and the host side:
+
+* libsalec.so loads here (no GPU) and exports every function that
+  include/salec.h and the host setup header declare;
+* the host's 9-point Laplacian weights match the reference's own
+  Laplace2d9 (golden vectors from oracle/_ref) and reproduce the Laplacian
+  of quadratics on rectangular stencils; the V_LAPLACE field of a setup is
+  those weights at every internal vertex;
+* the .inp loader rejects what the build does not support with an error
+  code and a message instead of exiting;
+* crater metrics of a synthetic profile;
+* the composed oracle step: a regression record (golden), rest stays at
+  rest, the lithostatic setup is an equilibrium, and mass is conserved in a
+  closed box without damping.
+"""
+import ctypes as C
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+import parity as P
+import pyoracle as po
+import pysalec as ps
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+HEADERS = [os.path.join(ROOT, "include", "salec.h"),
+           os.path.join(ROOT, "salec-2d-public_amd", "host", "setup.h")]
+
+
+def declared_functions(path):
+    txt = open(path).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\**\s*(salec_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    L = ps.lib()
+    names = []
+    for h in HEADERS:
+        names += declared_functions(h)
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.salec_abi_version() >= 1
+    for code in (0, -1, -2, -3, -4, -5, -6, -7):
+        assert L.salec_strerror(code)
+
+
+def test_field_table_matches_header():
+    L = ps.lib()
+    for i, (name, grid, nc) in enumerate(ps.FIELDS):
+        assert L.salec_field_name(i).decode() == name
+        g, n = C.c_int(), C.c_int()
+        assert L.salec_field_info(i, 3, C.byref(g), C.byref(n)) == 0
+        assert g.value == grid
+        assert n.value == (3 if nc < 0 else nc)
+
+
+def test_laplace_weights_golden():
+    g = np.load(os.path.join(GOLD, "laplace_ref.npz"))
+    for pos, w in zip(g["pos"], g["w"]):
+        got = ps.laplace9(pos)
+        assert np.allclose(got, w, rtol=1e-13, atol=1e-13 * np.abs(w).max()), (got, w)
+
+
+def test_laplace_weights_reproduce_quadratics():
+    g = np.load(os.path.join(GOLD, "laplace_ref.npz"))
+    for pos in g["pos"][:8]:                       # undistorted rectangular stencils
+        w = ps.laplace9(pos)
+        x, y = pos[0::2] - pos[8], pos[1::2] - pos[9]
+        s = np.abs(w).max()
+        for f, lap in ((np.ones(9), 0.0), (x, 0.0), (y, 0.0), (x * y, 0.0), (x * x, 2.0), (y * y, 2.0),
+                       (x * x - y * y, 0.0)):
+            assert abs(w @ f - lap) <= 1e-10 * (s * np.abs(f).max() + 1.0)
+
+
+def test_v_laplace_field_on_stretched_mesh(tmp_path):
+    inp, eos = ps.write_config(str(tmp_path / "c0"), "c0", nx=20, ny=24,
+                               sections_over={"mesh.ext": "on", "mesh.ex": "[1.05, 4, 4]", "mesh.ey": "[1.05, 4, 4]"},
+                               ANCPAT="uniform")
+    h = ps.HostRun(inp, eos)
+    vp = h.field("V_POS")
+    vl = h.field("V_LAPLACE")
+    VH, VW = vp.shape[1:]
+    # internal vertices (away from the two-deep pad) carry the weights of their 3 x 3 stencil
+    for j in range(4, VH - 4, 3):
+        for k in range(4, VW - 4, 3):
+            xi = np.array([[vp[0, j + dj, k + dk], vp[1, j + dj, k + dk]] for dj in (-1, 0, 1) for dk in (-1, 0, 1)])
+            assert np.array_equal(vl[:, j, k], ps.laplace9(xi.reshape(-1))), (j, k)
+    assert np.ptp(np.diff(vp[0, 5, :])) > 0.01  # the mesh really is stretched
+
+
+@pytest.mark.parametrize("over,what", [
+    ({"numerical.damping": "pml3"}, "pml3"),
+    ({"material.Porosity": "[, Wunnemann]"}, "orosity"),
+    ({"target.type": "sphere"}, "sphere"),
+])
+def test_loader_rejects_unsupported(tmp_path, over, what):
+    inp, eos = ps.write_config(str(tmp_path / "c0"), "c0", nx=16, ny=16, sections_over=over)
+    with pytest.raises(RuntimeError) as e:
+        ps.HostRun(inp, eos)
+    assert what.lower() in str(e.value).lower()
+
+
+def test_crater_metrics():
+    nx = 10
+    r = np.arange(nx + 4) * 10.0 - 15.0
+    y = np.full(nx + 4, 5.0)
+    y[2:7] = [-40.0, -35.0, -20.0, -10.0, 10.0]   # rises through 0 between columns 5 and 6
+    d, D = ps.crater_metrics(y, r, nx)
+    assert d == 40.0
+    assert D == pytest.approx(2 * (r[5] + 10.0 * (10.0 / 20.0)))
+    d, D = ps.crater_metrics(np.full(nx + 4, 5.0), r, nx)
+    assert d == -5.0 and D == 0.0
+
+
+def test_oracle_regression_golden():
+    """The composed oracle step reproduces its recorded 24 x 36 run exactly."""
+    g = np.load(os.path.join(GOLD, "c1_small_oracle.npz"))
+    h, o = P.make_run("c1", nx=24, ny=36)
+    assert h.cfg.damp_time == g["damp_time"]
+    for k in P.COMPARED:
+        assert np.array_equal(o.state[k], g["init_" + k], equal_nan=True), k
+    dts = []
+    for _ in range(len(g["dts"])):
+        o.step()
+        dts.append(o.dt)
+    assert np.array_equal(dts, g["dts"])
+    for k in P.COMPARED:
+        assert np.array_equal(o.state[k], g["step_" + k], equal_nan=True), k
+
+
+def test_oracle_rest_state_stays_at_rest():
+    """No projectile motion, no gravity, target at the EOS reference state
+    (pressure below 2 Pa after setup): nothing drives motion, so velocities
+    stay at rounding level and the deviatoric stress at zero."""
+    h, o = P.make_run("c0", nx=16, ny=24, velocity=0.0,
+                      sections_over={"condition.gravity": "[0.0, 0.0]"})
+    assert np.abs(o.state["E_PRE"]).max() < 2.0
+    for _ in range(5):
+        o.step()
+    assert np.abs(o.state["V_VEL"]).max() < 1e-6
+    assert np.abs(o.state["E_STE"]).max() == 0.0
+
+
+def test_oracle_hydrostatic_equilibrium():
+    """Under gravity the lithostatic setup is an equilibrium of the discrete
+    step: after 300 steps (0.2 s) the largest speed stays below 1 % of the
+    free-fall speed g t and does not grow."""
+    h, o = P.make_run("c0", nx=16, ny=24, velocity=0.0, damping="off", sections_over={"boundary.top": "fixed"})
+    vmax = []
+    for k in range(300):
+        o.step()
+        if k % 100 == 99:
+            vmax.append(np.abs(o.state["V_VEL"]).max())
+    assert vmax[-1] < 0.01 * 1.62 * o.t
+    assert vmax[-1] < 1.2 * vmax[0]
+
+
+def test_oracle_mass_conserved_in_closed_box():
+    """Closed box (all boundaries fixed), no damping: the remap moves mass
+    between cells and the total changes only through the step's
+    non-conservative cleanups (volume-fraction renormalisation, the density
+    cutoff), which stay near 1e-8 relative over these steps.  A flux booked
+    to one side of a face only would change it by percents."""
+    h, o = P.make_run("c0", nx=20, ny=30, damping="off",
+                      sections_over={"boundary.top": "fixed", "boundary.right": "fixed"})
+    vol = o.state["E_VOL"][0][2:-2, 2:-2]
+
+    def mass():
+        S = o.state
+        return float(np.sum(S["M_VOF"][:, 2:-2, 2:-2] * S["M_DEN"][:, 2:-2, 2:-2] * vol))
+
+    m0 = mass()
+    for _ in range(25):
+        o.step()
+    assert abs(mass() - m0) <= 1e-6 * m0
+    assert np.abs(o.state["V_VEL"]).max() > 100.0  # the impact is under way
+
+
+def test_create_validates_before_touching_a_device(tmp_path):
+    """Invalid configurations are refused with EINVAL/ENOSYM before any
+    device call; here (no GPU) a valid one then reports EDEVICE."""
+    inp, eos = ps.write_config(str(tmp_path / "c1"), "c1", nx=16, ny=24)
+    h = ps.HostRun(inp, eos)
+    L = ps.lib()
+    ctx = C.c_void_p()
+    cfg = ps.Config.from_buffer_copy(h.cfg)
+    cfg.nx = 2
+    assert L.salec_create(C.byref(cfg), C.byref(ctx)) == -1
+    cfg = ps.Config.from_buffer_copy(h.cfg)
+    cfg.damp_type = ps.DAMP["pml3"]
+    assert L.salec_create(C.byref(cfg), C.byref(ctx)) == -1
+    mats = (ps.Material * h.nmat)(*[ps.Material.from_buffer_copy(h.cfg.mat[m]) for m in range(h.nmat)])
+    mats[1].yfunc = 99
+    cfg = ps.Config.from_buffer_copy(h.cfg)
+    cfg.mat = C.cast(mats, C.POINTER(ps.Material))
+    assert L.salec_create(C.byref(cfg), C.byref(ctx)) == -7
+    mats[1].yfunc = h.cfg.mat[1].yfunc
+    mats[2].tab_data = None
+    assert L.salec_create(C.byref(cfg), C.byref(ctx)) == -1
+    try:
+        import torch
+        have_gpu = torch.cuda.is_available()
+    except Exception:
+        have_gpu = False
+    if not have_gpu:
+        assert L.salec_create(C.byref(h.cfg), C.byref(ctx)) == -3

@@ -1,0 +1,221 @@
+"""Pins of the oracle's EOS closure (src/eos/eos_state.c, src/eos/aneos_sieden.c).
+
+* Golden vectors written by the reference's own EOS code (oracle/_ref, see
+  tests/golden/make_golden.py): bit-exact.
+* Thermodynamics the closure must satisfy whatever its code: the Tillotson
+  pressure in each regime against the published form (Tillotson 1962;
+  Melosh 1989, Impact Cratering, sec. 3.x), continuity at Eiv and Ecv, the
+  sound speed against c^2 = dP/drho|_E + P/rho^2 dP/dE|_rho by finite
+  differences, the cold-energy curve against an independent ODE solve of
+  dE/drho = P/rho^2, and the table lookup against its nodes and bilinear
+  centre values.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import pysalec as ps
+import refeos
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SMALL = dict(nx=24, ny=36)
+CASES = [("c0", "granite.tillotson", refeos.EOS_TILLOTSON, 1),
+         ("c1", "regolith.aneos.sd", refeos.EOS_SIEDEN, 1),
+         ("c1", "basalt.aneos.sd", refeos.EOS_SIEDEN, 2)]
+
+
+@pytest.fixture(scope="module")
+def runs(tmp_path_factory):
+    d = tmp_path_factory.mktemp("eos")
+    out = {}
+    for which in ("c0", "c1"):
+        inp, eos = ps.write_config(str(d / which), which, **SMALL)
+        out[which] = (ps.HostRun(inp, eos), eos)
+    return out
+
+
+def ostate(m, den, eng):
+    L = po.olib()
+    a = [C.c_double() for _ in range(3)]
+    L.oracle_calculate_state(C.byref(m), den, eng, *[C.byref(x) for x in a])
+    return a[0].value, a[1].value, a[2].value
+
+
+def ostate_correct(m, den, eng, plim, pty=1.0):
+    L = po.olib()
+    a = [C.c_double() for _ in range(3)]
+    L.oracle_calculate_state_correct(C.byref(m), den, eng, plim, *[C.byref(x) for x in a], pty)
+    return a[0].value, a[1].value, a[2].value
+
+
+def till_pres(m, e, rho):
+    p, c = C.c_double(), C.c_double()
+    po.olib().oracle_till_pres(C.byref(m), e, rho, C.byref(p), C.byref(c))
+    return p.value, c.value
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[1] for c in CASES])
+def test_calculate_state_golden(runs, case):
+    which, fn, ty, m = case
+    g = np.load(os.path.join(GOLD, "eos_ref.npz"))
+    k = "%s_%s" % (which, fn.split(".")[0])
+    mat = runs[which][0].cfg.mat[m]
+    den, eng, plim = g[k + "_den"], g[k + "_eng"], g[k + "_plim"]
+    got = np.array([ostate(mat, a, b) for a, b in zip(den, eng)])
+    assert np.array_equal(got, g[k + "_state"], equal_nan=True), \
+        "calculate_state differs from the reference at %d of %d samples" % (
+            np.count_nonzero(~((got == g[k + "_state"]) | (np.isnan(got) & np.isnan(g[k + "_state"])))), den.size)
+    gotc = np.array([ostate_correct(mat, a, b, p) for a, b, p in zip(den, eng, plim)])
+    assert np.array_equal(gotc, g[k + "_state_correct"], equal_nan=True)
+
+
+@pytest.mark.skipif(not refeos.available(), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("case", CASES, ids=[c[1] for c in CASES])
+def test_calculate_state_live_reference(runs, case):
+    """Fresh samples (other seed, porous distension) against the reference build."""
+    which, fn, ty, m = case
+    h, eos = runs[which]
+    E = refeos.RefEOS(os.path.join(eos, fn), ty)
+    mat = h.cfg.mat[m]
+    rng = np.random.default_rng(2024)
+    for _ in range(3000):
+        den = 10 ** rng.uniform(-2, 4.4)
+        eng = den * 10 ** rng.uniform(-3, 9.5)
+        assert ostate(mat, den, eng) == E.state(den, eng)
+        plim = -10 ** rng.uniform(5, 9)
+        assert ostate_correct(mat, den, eng, plim) == E.state_correct(den, eng, plim)
+
+
+def test_tillotson_cold_table_golden(runs):
+    g = np.load(os.path.join(GOLD, "eos_ref.npz"))
+    mat = runs["c0"][0].cfg.mat[1]
+    n = mat.till_nden
+    xd, ye = np.zeros(n), np.zeros(n)
+    po.olib().oracle_till_cold_energy(C.byref(mat), n, xd.ctypes.data_as(C.POINTER(C.c_double)),
+                                      ye.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.array_equal(xd, g["c0_granite_cold_xden"])
+    assert np.array_equal(ye, g["c0_granite_cold_yeng"])
+    # the host setup's own table (used for temperatures) is the same
+    host = np.ctypeslib.as_array(mat.till_yeng, shape=(n,))
+    assert np.array_equal(host, ye)
+
+
+def test_tillotson_cold_curve_ode(runs):
+    """E_c(rho) solves dE/drho = P(E, rho)/rho^2 from E(rho0) = 0.  The table
+    is a fixed-step RK4 solution (step rho0/150), whose truncation error
+    stays below 1e-5 relative; a wrong stage weight or node gives errors of
+    order 1e-3 and more."""
+    from scipy.integrate import solve_ivp
+    mat = runs["c0"][0].cfg.mat[1]
+    n = mat.till_nden
+    xd, ye = np.zeros(n), np.zeros(n)
+    po.olib().oracle_till_cold_energy(C.byref(mat), n, xd.ctypes.data_as(C.POINTER(C.c_double)),
+                                      ye.ctypes.data_as(C.POINTER(C.c_double)))
+    rho0 = mat.TLRho0
+    f = lambda r, e: [till_pres(mat, e[0], r)[0] / (r * r)]  # noqa: E731
+    for tgt in (xd[300], xd[500], xd[1000], xd[1400], xd[20], xd[100]):
+        sol = solve_ivp(f, (rho0, tgt), [0.0], rtol=1e-12, atol=1e-6, method="DOP853")
+        k = int(np.argmin(np.abs(xd - tgt)))
+        assert abs(sol.y[0, -1] - ye[k]) <= 1e-5 * max(1.0, abs(ye[k])), (tgt, sol.y[0, -1], ye[k])
+
+
+def _textbook_tillotson(p, e, rho):
+    """Tillotson (1962) as written in Melosh (1989): compressed/cold branch,
+    expanded hot branch, and the linear blend in E between Eiv and Ecv.  The
+    reference uses A in place of B for the quadratic term when rho < rho0
+    (src/eos/eos_state.c:330); the form below follows it there."""
+    rho0, A, B, E0, a, b, al, be, Eiv, Ecv = (p.TLRho0, p.TLA, p.TLB, p.TLE0, p.TLa, p.TLb, p.TLAlpha,
+                                             p.TLBeta, p.TLEiv, p.TLEcv)
+    eta = rho / rho0
+    mu = eta - 1.0
+    w = e / (E0 * eta ** 2) + 1.0
+    Bq = A if eta < 1.0 else B
+    pc = (a + b / w) * rho * e + A * mu + Bq * mu * mu
+    z = rho0 / rho - 1.0
+    ph = a * rho * e + (b * rho * e / w + A * mu * np.exp(-be * z)) * np.exp(-al * z * z)
+    if rho >= rho0 or e < Eiv:
+        return pc
+    if e > Ecv:
+        return ph
+    return ((e - Eiv) * ph + (Ecv - e) * pc) / (Ecv - Eiv)
+
+
+def test_tillotson_regimes_closed_form(runs):
+    mat = runs["c0"][0].cfg.mat[1]
+    rng = np.random.default_rng(5)
+    rho0 = mat.TLRho0
+    for _ in range(2000):
+        rho = rho0 * 10 ** rng.uniform(-1.5, 0.6)
+        e = 10 ** rng.uniform(3, 8)
+        P, _ = till_pres(mat, e, rho)
+        ref = _textbook_tillotson(mat, e, rho)
+        assert abs(P - ref) <= 1e-10 * max(abs(ref), 1e3), (rho, e, P, ref)
+
+
+def test_tillotson_continuity_at_eiv_ecv(runs):
+    mat = runs["c0"][0].cfg.mat[1]
+    for rho in np.linspace(0.3, 0.99, 12) * mat.TLRho0:
+        for E in (mat.TLEiv, mat.TLEcv):
+            lo, _ = till_pres(mat, E * (1 - 1e-12), rho)
+            hi, _ = till_pres(mat, E * (1 + 1e-12), rho)
+            assert abs(lo - hi) <= 1e-6 * max(abs(lo), 1e3), (rho, E, lo, hi)
+
+
+def test_tillotson_sound_speed_thermodynamic(runs):
+    """c^2 = dP/drho|_E + (P/rho^2) dP/dE|_rho wherever c is not reset
+    (P > 0.01) or truncated."""
+    mat = runs["c0"][0].cfg.mat[1]
+    rng = np.random.default_rng(9)
+    rho0 = mat.TLRho0
+    checked = 0
+    for _ in range(600):
+        rho = rho0 * 10 ** rng.uniform(-0.4, 0.5)
+        e = 10 ** rng.uniform(4, 8)
+        if min(abs(e - mat.TLEiv), abs(e - mat.TLEcv)) < 1e-3 * e or abs(rho - rho0) < 1e-3 * rho0:
+            continue
+        P, c = till_pres(mat, e, rho)
+        if P <= 1.0 or c * c < 1.0:
+            continue
+        hr, he = 1e-6 * rho, 1e-6 * e
+        dPdr = (till_pres(mat, e, rho + hr)[0] - till_pres(mat, e, rho - hr)[0]) / (2 * hr)
+        dPde = (till_pres(mat, e + he, rho)[0] - till_pres(mat, e - he, rho)[0]) / (2 * he)
+        c2 = dPdr + P / rho ** 2 * dPde
+        assert abs(c * c - c2) <= 2e-5 * c2, (rho, e, c * c, c2)
+        checked += 1
+    assert checked > 300
+
+
+def test_sd_table_nodes_and_centres(runs):
+    mat = runs["c1"][0].cfg.mat[2]
+    nd, ne = mat.tab_nden, mat.tab_ntem
+    yd = np.ctypeslib.as_array(mat.tab_yden, shape=(nd,))
+    xe = np.ctypeslib.as_array(mat.tab_xtem, shape=(ne,))
+    data = np.ctypeslib.as_array(mat.tab_data, shape=(ne, nd, 3))
+    rng = np.random.default_rng(3)
+    for _ in range(400):
+        i, j = int(rng.integers(1, ne - 2)), int(rng.integers(1, nd - 2))
+        T, P, c = ostate(mat, yd[j], yd[j] * xe[i])
+        exp = data[i, j]
+        assert np.allclose([T, P, c], exp, rtol=1e-14, atol=0), (i, j)
+        r, e = 0.5 * (yd[j] + yd[j + 1]), 0.5 * (xe[i] + xe[i + 1])
+        T, P, c = ostate(mat, r, r * e)
+        cen = 0.25 * (data[i, j] + data[i, j + 1] + data[i + 1, j] + data[i + 1, j + 1])
+        assert np.allclose([T, P, c], cen, rtol=1e-12, atol=1e-9 * np.abs(cen).max()), (i, j)
+
+
+def test_bisearch_golden_and_bruteforce():
+    g = np.load(os.path.join(GOLD, "bisearch_ref.npz"))
+    L = po.olib()
+    for xl, xs, idx in zip(g["lists"], g["xs"], g["idx"]):
+        xl = np.ascontiguousarray(xl[np.isfinite(xl)])
+        n = xl.size
+        for x, i in zip(xs, idx):
+            if not np.isfinite(x):
+                continue
+            got = L.oracle_bisearch(float(x), xl.ctypes.data_as(C.POINTER(C.c_double)), n)
+            assert got == i
+            if xl[0] <= x < xl[-1]:
+                assert xl[got] <= x < xl[got + 1]
