@@ -38,7 +38,8 @@ def phase_bytes(n):
     """Algorithmic (compulsory, unique) bytes per unit for each phase: FP64
     values read + written once per unit; vertex data shared by four cells is
     counted once per cell.  Face packets are data-dependent intermediate
-    storage and are not counted, so flux/advect figures are lower bounds."""
+    storage and are not counted, so the flux and remap figures are lower
+    bounds.  Fused phases count a value shared by their parts once."""
     d = 8
     return {
         "trace_profile": ("cell", 3 * d),
@@ -49,12 +50,9 @@ def phase_bytes(n):
         "vibration_stress": ("cell", (n + 29) * d),
         "calculate_acc": ("vertex", 46 * d),
         "calculate_v_dis": ("vertex", 8 * d),
-        "update_energy": ("cell", (4 * n + 15) * d),
         "tracers": ("tracer", 104),
-        "flux": ("cell", (4 * n + 33) * d),
-        "advect": ("cell", (6 * n + 25) * d),
-        "damp_on_cell": ("padded_cell", 3 * d),
-        "update_state": ("cell", (4 * n + 6) * d),
+        "energy_flux": ("cell", (5 * n + 44) * d),           # update_energy (4n+15) + flux (4n+33) - shared (3n+4)
+        "advect_damp_state": ("cell", (7 * n + 31) * d),     # advect (6n+25) + damping sigma (2) + EOS (n+4)
     }
 
 

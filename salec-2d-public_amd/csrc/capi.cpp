@@ -36,14 +36,14 @@ struct Program {
 
 enum {
     PH_PROFILE = 0, PH_GF_A, PH_VERTEX, PH_RST_VEL, PH_E_VEL, PH_GF_EVEL, PH_CHECK_V, PH_GRADVEL_DT,
-    PH_STRESS, PH_GF_B, PH_ACC, PH_RST_ACC, PH_VDIS, PH_RST_DIS, PH_ENERGY, PH_TRACERS, PH_FLUX,
-    PH_GF_C, PH_FLUX_RING, PH_ADVECT, PH_DAMP, PH_EJECTA, PH_STATE, PH_TIME, PH_COUNT
+    PH_STRESS, PH_GF_B, PH_ACC, PH_RST_ACC, PH_VDIS, PH_RST_DIS, PH_TRACERS, PH_ENERGY_FLUX,
+    PH_GF_C, PH_FLUX_RING, PH_REMAP_CLOSE, PH_TIME, PH_COUNT
 };
 const char *kPhaseNames[PH_COUNT] = {
     "trace_profile", "ghost_vof_dvel_den", "vertex_vof_vel", "restrict_v_vel", "update_e_vel", "ghost_e_vel",
     "check_v_vel", "gradvel_dt", "vibration_stress", "ghost_pre_ste_q", "calculate_acc", "restrict_v_acc",
-    "calculate_v_dis", "restrict_v_dis", "update_energy", "tracers", "flux", "ghost_den_vib_eng",
-    "flux_ghost_ring", "advect", "damp_on_cell", "ejecta_flow", "update_state", "advance_time"};
+    "calculate_v_dis", "restrict_v_dis", "tracers", "energy_flux", "ghost_den_vib_eng", "flux_ghost_ring",
+    "advect_damp_state", "advance_time"};
 
 } // namespace
 
@@ -59,7 +59,6 @@ struct salec_ctx {
     double *bx = nullptr, *by = nullptr;
     DevTime *d_time = nullptr;
     double *d_part = nullptr;            /* dt / dampref partial minima   */
-    unsigned *d_counter = nullptr;
     int gv_blocks = 0;
     double *d_prof = nullptr;            /* trace_profile chunk minima    */
     int prof_chunks = 0;
@@ -436,8 +435,7 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     c->gv_blocks = gradvel_dt_blocks(g);
     int nb = std::max(c->gv_blocks, dampref_blocks(g));
     TRY(dalloc(c, &c->d_part, (size_t)nb));
-    TRY(dalloc(c, &c->d_counter, 1));
-    c->prof_chunks = (g.CH + 63) / 64;
+    c->prof_chunks = (g.CH + PROF_CHUNK - 1) / PROF_CHUNK;
     TRY(dalloc(c, &c->d_prof, (size_t)c->prof_chunks * g.CW));
     TRY(dalloc(c, &c->d_nf, 1));
     /* materials and their tables */
@@ -676,22 +674,18 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     PHASE(PH_E_VEL, launch_update_e_vel(s, g, F));
     PHASE(PH_GF_EVEL, run_program(c, c->gfE));
     PHASE(PH_CHECK_V, launch_check_v_vel(s, g, F));
-    PHASE(PH_GRADVEL_DT, launch_gradvel_dt(s, g, F, P, c->d_time, c->d_part, c->d_counter, c->gv_blocks));
+    PHASE(PH_GRADVEL_DT, launch_gradvel_dt(s, g, F, P, c->d_time, c->d_part, c->gv_blocks));
     PHASE(PH_STRESS, launch_stress(s, g, F, P, c->d_mats, c->d_time));
     PHASE(PH_GF_B, run_program(c, c->gfB));
     PHASE(PH_ACC, launch_acc(s, g, F, P, c->d_time));
     PHASE(PH_RST_ACC, run_program(c, c->rstA));
     PHASE(PH_VDIS, launch_v_dis(s, g, F, c->d_time));
     PHASE(PH_RST_DIS, run_program(c, c->rstD));
-    PHASE(PH_ENERGY, launch_energy(s, g, F, c->d_mats, c->d_time));
     PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
-    PHASE(PH_FLUX, launch_flux(s, g, F, c->bx, c->by, 0));
+    PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, c->d_mats, c->d_time, c->bx, c->by));
     PHASE(PH_GF_C, run_program(c, c->gfC));
     PHASE(PH_FLUX_RING, launch_flux(s, g, F, c->bx, c->by, 1));
-    PHASE(PH_ADVECT, launch_advect(s, g, F, P, c->bx, c->by));
-    PHASE(PH_DAMP, launch_damp(s, g, F, P, c->d_time));
-    PHASE(PH_EJECTA, launch_ejecta_flow(s, g, F, P, c->d_time));
-    PHASE(PH_STATE, launch_update_state(s, g, F, P, c->d_mats));
+    PHASE(PH_REMAP_CLOSE, launch_remap_close(s, g, F, P, c->d_mats, c->bx, c->by, c->d_time));
     PHASE(PH_TIME, launch_advance_time(s, c->d_time));
 #undef PHASE
     HIPCHK(hipGetLastError());

@@ -10,6 +10,7 @@
  * (src/advect.c:1347-1354).
  */
 #include "device_math.h"
+#include <type_traits>
 
 namespace salec {
 
@@ -18,10 +19,30 @@ namespace salec {
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+/* Material count as a template parameter: NM > 0 makes it a compile-time
+ * constant, so per-material arrays are fully unrolled into registers
+ * instead of living in scratch; NM = 0 is the generic form for any nmat. */
+template <int NM> struct Mats {
+    static constexpr int cap = NM;
+    __device__ static constexpr int n(const Grid &) { return NM; }
+};
+template <> struct Mats<0> {
+    static constexpr int cap = SALEC_MAXMAT;
+    __device__ static int n(const Grid &g) { return g.nmat; }
+};
+template <class L> static void nm_dispatch(int nmat, L launch)
+{
+    switch (nmat) {
+    case 2: launch(std::integral_constant<int, 2>{}); break;
+    case 3: launch(std::integral_constant<int, 3>{}); break;
+    case 4: launch(std::integral_constant<int, 4>{}); break;
+    default: launch(std::integral_constant<int, 0>{}); break;
+    }
+}
+
 /* ------------------------------------------------------ trace_profile -- */
 /* src/ejecta.c:8-63: per-column minimum of vacuum-bearing cell heights and
  * the ejecta flag.  Pass 1: thread per (column, row chunk). */
-#define PROF_CHUNK 64
 __global__ void k_trace_profile(Grid g, Fields F, Params P, double *part, int nchunk)
 {
     int col = blockIdx.x * blockDim.x + threadIdx.x;
@@ -37,16 +58,21 @@ __global__ void k_trace_profile(Grid g, Fields F, Params P, double *part, int nc
     }
     part[(long)ch * g.CW + col] = yp;
 }
-__global__ void k_profile_finish(Grid g, Fields F, Params P, const double *part, int nchunk)
+/* pass 2: one wavefront per column reduces the chunk minima */
+__global__ void __launch_bounds__(256) k_profile_finish(Grid g, Fields F, Params P, const double *part, int nchunk)
 {
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (col >= g.CW) return;
     double yp = P.threshold_z * 2.0;
-    for (int ch = 0; ch < nchunk; ++ch) {
+    for (int ch = lane; ch < nchunk; ch += 64) {
         double v = part[(long)ch * g.CW + col];
         yp = v < yp ? v : yp;
     }
-    F.f[SALEC_Y_PROFILE][col] = yp;
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(yp, off, 64);
+        yp = o < yp ? o : yp;
+    }
+    if (lane == 0) F.f[SALEC_Y_PROFILE][col] = yp;
 }
 void launch_trace_profile(hipStream_t s, const Grid &g, const Fields &F, const Params &P, double *part, int nchunk)
 {
@@ -55,7 +81,7 @@ void launch_trace_profile(hipStream_t s, const Grid &g, const Fields &F, const P
 }
 void launch_profile_finish(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *part, int nchunk)
 {
-    k_profile_finish<<<nblk(g.CW, 256), 256, 0, s>>>(g, F, P, part, nchunk);
+    k_profile_finish<<<nblk(g.CW, 4), 256, 0, s>>>(g, F, P, part, nchunk);
 }
 
 /* ------------------------------------------------------------- ghosts -- */
@@ -283,10 +309,9 @@ __device__ __forceinline__ double wave_min(double v)
     }
     return v;
 }
-__global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, Params P, DevTime *tm, double *part, unsigned *counter)
+__global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, Params P, double *part)
 {
     __shared__ double red[GV_BS / 64];
-    __shared__ bool last;
     int k, j;
     double cand = HUGE_VAL;  /* identity of MIN over finite values */
     bool has = false;
@@ -335,49 +360,49 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, Params P
         double b = red[0];
         for (int w = 1; w < GV_BS / 64; ++w) b = red[w] < b ? red[w] : b;
         part[blockIdx.x] = b;
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last) return;
-    /* last block: reduce the per-block minima */
-    __threadfence();
+}
+/* dt = Min(max_dt, MIN(1.1*dt_prev, min over the block partials)) */
+#define DT_BS 1024
+__global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, const double *part, int nparts)
+{
+    __shared__ double red[DT_BS / 64];
     double r = HUGE_VAL;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-        double v = __builtin_nontemporal_load(part + b);
+    for (int b = threadIdx.x; b < nparts; b += DT_BS) {
+        double v = part[b];
         r = v < r ? v : r;
     }
     r = wave_min(r);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = r;
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = red[0];
-        for (int w = 1; w < GV_BS / 64; ++w) b = red[w] < b ? red[w] : b;
+        for (int w = 1; w < DT_BS / 64; ++w) b = red[w] < b ? red[w] : b;
         double rtn = 1.1 * tm->dt;
         rtn = b < rtn ? b : rtn;       /* MIN(rtn_dt, dt_rz) over all cells */
-        double ldt = dMin(P.max_dt, rtn);
-        tm->dt = ldt;
-        *counter = 0u;
+        tm->dt = dMin(P.max_dt, rtn);
     }
 }
 int gradvel_dt_blocks(const Grid &g) { return nblk((long)g.nx * g.ny, GV_BS); }
 void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const Params &P, DevTime *tm,
-                       double *part, unsigned *counter, int nblocks)
+                       double *part, int nblocks)
 {
-    k_gradvel_dt<<<nblocks, GV_BS, 0, s>>>(g, F, P, tm, part, counter);
+    k_gradvel_dt<<<nblocks, GV_BS, 0, s>>>(g, F, P, part);
+    k_dt_finish<<<1, DT_BS, 0, s>>>(P, tm, part, nblocks);
 }
 
 /* ------------------------------------ block_vibration + update_e_stress -- */
 /* block_vibration (src/strength.c:625-695) then update_e_stress
  * (src/strength.c:7-349, avg_strength mixing) for one physical cell. */
-__global__ void k_stress(Grid g, Fields F, Params P, const DevMat *mats, const DevTime *tm)
+template <int NM>
+__global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, const DevMat *mats, const DevTime *tm)
 {
     int k, j;
     if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
     const double dt = tm->dt, time = tm->t;
     long c = k + (long)g.CW * j;
-    int n = g.nmat;
+    const int n = Mats<NM>::n(g);
     double v0 = CPL(M_VOF, 0, c);
     double dam = CPL(E_DAM, 0, c);
     double vib = CPL(E_VIB, 0, c);
@@ -517,7 +542,9 @@ __global__ void k_stress(Grid g, Fields F, Params P, const DevMat *mats, const D
 }
 void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm)
 {
-    k_stress<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats, tm);
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_stress<decltype(I)::value><<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats, tm);
+    });
 }
 
 /* -------------------------------------------------------- calculate_acc -- */
@@ -648,13 +675,11 @@ void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const DevTime *
 
 /* -------------------------------------------------------- update_energy -- */
 /* src/sale2d.c:564-712 (porosity model off) */
-__global__ void k_energy(Grid g, Fields F, const DevMat *mats, const DevTime *tm)
+template <int NM>
+__device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, const DevMat *mats, double dt, int k, int j)
 {
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
-    const double dt = tm->dt;
     long c = k + (long)g.CW * j;
-    int n = g.nmat;
+    const int n = Mats<NM>::n(g);
     if (CPL(M_VOF, 0, c) > TOLVOF || CPL(E_DEN, 0, c) < TOLRHO) return;
     double lp = CPL(E_PRE, 0, c) + CPL(E_Q, 0, c);
     double div = CPL(E_DIV_VEL, 0, c);
@@ -678,9 +703,18 @@ __global__ void k_energy(Grid g, Fields F, const DevMat *mats, const DevTime *tm
     }
     CPL(E_CVS, 0, c) = dWind(CPL(E_CVS, 0, c) + dcvs * dt, -10.0, 10.0);
 }
+template <int NM>
+__global__ void k_energy(Grid g, Fields F, const DevMat *mats, const DevTime *tm)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    energy_cell<NM>(g, F, mats, tm->dt, k, j);
+}
 void launch_energy(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm)
 {
-    k_energy<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, mats, tm);
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_energy<decltype(I)::value><<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, mats, tm);
+    });
 }
 
 /* ------------------------------------------------------------- tracers -- */
@@ -739,10 +773,11 @@ void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params 
 /* ------------------------------------------------ calculate_flux_mixed -- */
 /* src/advect.c:861-1136 with _calculate_flux_set_bphi (:1138-1224).  The
  * donor writes the packet of each of its outflow faces (swept volume < 0). */
-__device__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by, int k, int j)
+template <int NM>
+__device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by, int k, int j)
 {
     long c = k + (long)g.CW * j;
-    int n = g.nmat;
+    const int n = Mats<NM>::n(g);
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
     double px[4], py[4], dx[4], dy[4];
     for (int i = 0; i < 4; ++i) {
@@ -754,15 +789,16 @@ __device__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by
     double bv = flux_volcyl(px[1], py[1], px[0], py[0], dx[1], dy[1], dx[0], dy[0]);
     double uv = flux_volcyl(px[3], py[3], px[2], py[2], dx[3], dy[3], dx[2], dy[2]);
     double lbru[4] = {lv, bv, rv, uv};
-    double ev[SALEC_MAXMAT];
+    double ev[Mats<NM>::cap];
     for (int m = 0; m < n; ++m) ev[m] = CPL(M_VOF, m, c);
     double evol = CPL(E_VOL, 0, c);
-    double fv[SALEC_MAXMAT][4];
+    double fv[Mats<NM>::cap][4];
     int pm = 0;
+    double evpm = ev[0];
     if (ev[0] < TOLVOF)
-        for (int q = 1; q < n; ++q) if (ev[q] > ev[pm]) pm = q;
+        for (int q = 1; q < n; ++q) if (ev[q] > evpm) { pm = q; evpm = ev[q]; }
     double mlv = lv < 0 ? lv : 0, mrv = rv < 0 ? rv : 0, mbv = bv < 0 ? bv : 0, muv = uv < 0 ? uv : 0;
-    if (ev[pm] > 1.0 - TOLVOF) {
+    if (evpm > 1.0 - TOLVOF) {
         for (int q = 0; q < n; ++q) {
             if (q == pm) { fv[q][0] = mlv; fv[q][2] = -1.0 * mrv; fv[q][1] = mbv; fv[q][3] = -1.0 * muv; }
             else { fv[q][0] = fv[q][1] = fv[q][2] = fv[q][3] = 0.; }
@@ -778,7 +814,7 @@ __device__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by
         vnorm2(g0, g1);
         double ca = CPL(E_POS, 2, c) * fabs(g0), cb = CPL(E_POS, 3, c) * fabs(g1);
         if (ca < cb) { double t = ca; ca = cb; cb = t; }
-        double dstar = cut_length(ev[pm], ca, cb);
+        double dstar = cut_length(evpm, ca, cb);
         double w[4];
         for (int i = 0; i < 4; ++i) { double z = 0; z += g0 * px[i]; z += g1 * py[i]; w[i] = z; }
         double mw = w[0] > w[1] ? w[0] : w[1];
@@ -800,7 +836,7 @@ __device__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by
         bn[2][0] = F.f[SALEC_BX_NORMAL][k + 1 + (long)g.BXW * j]; bn[2][1] = F.f[SALEC_BX_NORMAL][g.NBX + k + 1 + (long)g.BXW * j];
         bn[1][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * j]; bn[1][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * j];
         bn[3][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * (j + 1)]; bn[3][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * (j + 1)];
-        double fnp[SALEC_MAXMAT][4];
+        double fnp[Mats<NM>::cap][4];
         double snp[4] = {0., 0., 0., 0.};
         for (int m = 0; m < n; ++m) fnp[m][0] = fnp[m][1] = fnp[m][2] = fnp[m][3] = 0.;
         for (int m = 1; m < n; ++m) {
@@ -820,7 +856,7 @@ __device__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by
         }
         for (int i = 0; i < 4; ++i) {
             if (snp[i] > TOLVOF) { for (int m = 1; m < n; ++m) fnp[m][i] = fnp[m][i] / snp[i]; }
-            else { for (int m = 1; m < n; ++m) fnp[m][i] = ev[m] / (1.0 - ev[pm]); }
+            else { for (int m = 1; m < n; ++m) fnp[m][i] = ev[m] / (1.0 - evpm); }
         }
         for (int q = 0; q < n; ++q) {
             if (q == pm) {
@@ -864,13 +900,15 @@ __device__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by
         ph[(long)(3 * n + 9) * st + o] = fm * CPL(E_EJT, 0, c);
     }
 }
+template <int NM>
 __global__ void __launch_bounds__(128) k_flux_physical(Grid g, Fields F, double *bx, double *by)
 {
     int k, j;
     if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
-    flux_cell(g, F, bx, by, k, j);
+    flux_cell<NM>(g, F, bx, by, k, j);
 }
 /* ghost ring (distance < 2): rows 0,1,CH-2,CH-1 and columns 0,1,CW-2,CW-1 */
+template <int NM>
 __global__ void __launch_bounds__(128) k_flux_ring(Grid g, Fields F, double *bx, double *by)
 {
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -888,12 +926,15 @@ __global__ void __launch_bounds__(128) k_flux_ring(Grid g, Fields F, double *bx,
     } else {
         return;
     }
-    flux_cell(g, F, bx, by, k, j);
+    flux_cell<NM>(g, F, bx, by, k, j);
 }
 void launch_flux(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by, int ring)
 {
-    if (!ring) k_flux_physical<<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
-    else k_flux_ring<<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
+    nm_dispatch(g.nmat, [&](auto I) {
+        constexpr int NM = decltype(I)::value;
+        if (!ring) k_flux_physical<NM><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
+        else k_flux_ring<NM><<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
+    });
 }
 
 /* --------------------------------------------------------- advect_flux -- */
@@ -901,12 +942,12 @@ void launch_flux(hipStream_t s, const Grid &g, const Fields &F, double *bx, doub
  * holds a packet from this step iff its swept volume is non-zero: the cell
  * on the outflow side wrote it; faces with zero volume were zeroed by
  * clean_flux in the reference and contribute exactly nothing. */
-__global__ void __launch_bounds__(128) k_advect(Grid g, Fields F, Params P, const double *bx, const double *by)
+template <int NM>
+__device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const Params &P, const double *bx,
+                                            const double *by, int k, int j)
 {
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
     long c = k + (long)g.CW * j;
-    int n = g.nmat, PL = g.PL;
+    const int n = Mats<NM>::n(g), PL = 3 * n + 10;
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
     double px[4], py[4], dx[4], dy[4];
     for (int i = 0; i < 4; ++i) {
@@ -920,7 +961,7 @@ __global__ void __launch_bounds__(128) k_advect(Grid g, Fields F, Params P, cons
     sv[3] = flux_volcyl(px[3], py[3], px[2], py[2], dx[3], dy[3], dx[2], dy[2]);
     long fi[4] = {k + (long)g.BXW * j, k + (long)g.BYW * j, k + 1 + (long)g.BXW * j, k + (long)g.BYW * (j + 1)};
     const double dir[4] = {1.0, 1.0, -1.0, -1.0};
-    double dphi[3 * SALEC_MAXMAT + 10];
+    double dphi[3 * Mats<NM>::cap + 10];
     for (int q = 0; q < PL; ++q) dphi[q] = 0.0;
     for (int i = 0; i < 4; ++i) {
         if (!(sv[i] < 0. || sv[i] > 0.)) continue;
@@ -941,8 +982,8 @@ __global__ void __launch_bounds__(128) k_advect(Grid g, Fields F, Params P, cons
     }
     double evol = CPL(E_VOL, 0, c);
     double emass = evol * CPL(E_DEN, 0, c);
-    double sphi[3 * SALEC_MAXMAT + 10];
-    double ev[SALEC_MAXMAT], md[SALEC_MAXMAT], me[SALEC_MAXMAT];
+    double sphi[3 * Mats<NM>::cap + 10];
+    double ev[Mats<NM>::cap], md[Mats<NM>::cap], me[Mats<NM>::cap];
     for (int m = 0; m < n; ++m) {
         ev[m] = CPL(M_VOF, m, c); md[m] = CPL(M_DEN, m, c); me[m] = CPL(M_ENG, m, c);
         sphi[m] = 0.0 + ev[m] * evol;
@@ -1014,21 +1055,27 @@ __global__ void __launch_bounds__(128) k_advect(Grid g, Fields F, Params P, cons
     CPL(E_DAM, 0, c) = dam; CPL(E_TPS, 0, c) = tps; CPL(E_VIB, 0, c) = vib; CPL(E_EJT, 0, c) = ejt;
     CPL(E_DEN, 0, c) = eden;
 }
+template <int NM>
+__global__ void __launch_bounds__(128) k_advect(Grid g, Fields F, Params P, const double *bx, const double *by)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    advect_cell<NM>(g, F, P, bx, by, k, j);
+}
 void launch_advect(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *bx, const double *by)
 {
-    k_advect<<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, P, bx, by);
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_advect<decltype(I)::value><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, P, bx, by);
+    });
 }
 
 /* ---------------------------------------------------------- damp_on_cell -- */
 /* damp_on_cell_pml2 (src/velocity.c:487-588), _pml (:378-483), _sbc
  * (:688-767); all cells including the ghost ring; inactive once t >= damp_time */
-__global__ void k_damp(Grid g, Fields F, Params P, const DevTime *tm)
+template <int NM>
+__device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const Params &P, double dt, long c)
 {
-    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (c >= g.NC) return;
-    const double dt = tm->dt, t = tm->t;
-    if (P.damp_time <= t) return;
-    int n = g.nmat;
+    const int n = Mats<NM>::n(g);
     if (CPL(M_VOF, 0, c) > TOLVOF) return;
     double sx = CPL(E_SIGMA, 0, c), sy = CPL(E_SIGMA, 1, c);
     double adx = P.alpha_damp_x, ady = P.alpha_damp_y;
@@ -1106,19 +1153,26 @@ __global__ void k_damp(Grid g, Fields F, Params P, const DevTime *tm)
             }
     }
 }
+template <int NM>
+__global__ void k_damp(Grid g, Fields F, Params P, const DevTime *tm)
+{
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= g.NC) return;
+    if (P.damp_time <= tm->t) return;
+    damp_cell<NM>(g, F, P, tm->dt, c);
+}
 void launch_damp(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
 {
     if (P.damp_type != SALEC_DAMP_PML && P.damp_type != SALEC_DAMP_PML2 && P.damp_type != SALEC_DAMP_SBC) return;
-    k_damp<<<nblk(g.NC, 256), 256, 0, s>>>(g, F, P, tm);
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_damp<decltype(I)::value><<<nblk(g.NC, 256), 256, 0, s>>>(g, F, P, tm);
+    });
 }
 
 /* ---------------------------------------------------------- ejecta_flow -- */
 /* src/ejecta.c:65-134 */
-__global__ void k_ejecta_flow(Grid g, Fields F, Params P, const DevTime *tm)
+__device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, const Params &P, double dt, int k, int j)
 {
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
-    const double dt = tm->dt;
     long c = k + (long)g.CW * j;
     long v0 = k - 1 + (long)g.VW * (j - 1), v1 = k + (long)g.VW * (j - 1);
     double gy = 0.;
@@ -1137,6 +1191,12 @@ __global__ void k_ejecta_flow(Grid g, Fields F, Params P, const DevTime *tm)
         }
     }
 }
+__global__ void k_ejecta_flow(Grid g, Fields F, Params P, const DevTime *tm)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    ejecta_cell(g, F, P, tm->dt, k, j);
+}
 void launch_ejecta_flow(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
 {
     if (P.flow_resistance <= 0. && P.flow_friction <= 0.) return;
@@ -1145,12 +1205,11 @@ void launch_ejecta_flow(hipStream_t s, const Grid &g, const Fields &F, const Par
 
 /* --------------------------------------------------------- update_state -- */
 /* src/sale2d.c:1635-1742 */
-__global__ void k_update_state(Grid g, Fields F, Params P, const DevMat *mats)
+template <int NM>
+__device__ __forceinline__ void state_cell(const Grid &g, const Fields &F, const Params &P, const DevMat *mats, int k, int j)
 {
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
     long c = k + (long)g.CW * j;
-    int n = g.nmat;
+    const int n = Mats<NM>::n(g);
     double dam = CPL(E_DAM, 0, c);
     double pre = 0.0, csd = 0.0, tem = 0.0, den = 0.0, mu = 0.0;
     for (int m = 1; m < n; ++m) {
@@ -1177,9 +1236,18 @@ __global__ void k_update_state(Grid g, Fields F, Params P, const DevMat *mats)
     }
     CPL(E_PRE, 0, c) = pre; CPL(E_CSD, 0, c) = csd; CPL(E_TEM, 0, c) = tem; CPL(E_DEN, 0, c) = den; CPL(E_MU, 0, c) = mu;
 }
+template <int NM>
+__global__ void k_update_state(Grid g, Fields F, Params P, const DevMat *mats)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    state_cell<NM>(g, F, P, mats, k, j);
+}
 void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats)
 {
-    k_update_state<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats);
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_update_state<decltype(I)::value><<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats);
+    });
 }
 
 /* update_cycle_control, src/sale2d.c:71-80 */
@@ -1188,6 +1256,59 @@ __global__ void k_advance_time(DevTime *tm)
     if (threadIdx.x == 0 && blockIdx.x == 0) { tm->t += tm->dt; tm->step += 1; }
 }
 void launch_advance_time(hipStream_t s, DevTime *tm) { k_advance_time<<<1, 64, 0, s>>>(tm); }
+
+/* ------------------------------------------------------- fused phases -- */
+/* update_energy (src/sale2d.c:564-712) and calculate_flux_mixed
+ * (src/advect.c:861-1224) on the same physical cell.  The tracer update
+ * that the reference runs between them reads neither m_eng nor e_cvs, and
+ * a cell's flux reads only its own energy, so the per-cell chain gives the
+ * reference's values. */
+template <int NM>
+__global__ void __launch_bounds__(128) k_energy_flux(Grid g, Fields F, const DevMat *mats, const DevTime *tm,
+                                                     double *bx, double *by)
+{
+    int k, j;
+    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    energy_cell<NM>(g, F, mats, tm->dt, k, j);
+    flux_cell<NM>(g, F, bx, by, k, j);
+}
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm,
+                        double *bx, double *by)
+{
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_energy_flux<decltype(I)::value><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, mats, tm, bx, by);
+    });
+}
+
+/* advect_flux, damp_on_cell, ejecta_flow and update_state
+ * (src/salec2d.c:248-257) as one per-cell chain over the padded grid:
+ * physical cells run all four, ghost-ring cells only the damping, which is
+ * all the reference does to them at this point.  Each phase reads and
+ * writes only its own cell (advect reads its neighbours through the face
+ * packets written before this kernel), so no cell sees another's update. */
+template <int NM>
+__global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, Params P, const DevMat *mats,
+                                                     const double *bx, const double *by, const DevTime *tm)
+{
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= g.NC) return;
+    int k = (int)(c % g.CW), j = (int)(c / g.CW);
+    bool phys = k >= 2 && k < g.CW - 2 && j >= 2 && j < g.CH - 2;
+    const double dt = tm->dt, t = tm->t;
+    if (phys) advect_cell<NM>(g, F, P, bx, by, k, j);
+    if (!(P.damp_time <= t)) damp_cell<NM>(g, F, P, dt, c);
+    if (phys) {
+        if (P.flow_resistance > 0. || P.flow_friction > 0.) ejecta_cell(g, F, P, dt, k, j);
+        state_cell<NM>(g, F, P, mats, k, j);
+    }
+}
+void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats,
+                        const double *bx, const double *by, const DevTime *tm)
+{
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_remap_close<decltype(I)::value><<<nblk(g.NC, 128), 128, 0, s>>>(g, F, P, mats, bx, by, tm);
+    });
+}
 
 /* ------------------------------------------------------- setup tail -- */
 /* record_avg_value, src/sale2d.c:1562-1627: dampref for every cell and the

@@ -7,6 +7,9 @@
 #include <stdint.h>
 #include "../../include/salec.h"
 
+/* rows per partial of the crater-profile column minimum */
+#define PROF_CHUNK 8
+
 namespace salec {
 
 /* material parameters as the kernels read them (state_reference + tables) */
@@ -63,7 +66,7 @@ void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const 
 void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F);
 void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F);
 void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const Params &P, DevTime *tm,
-                       double *part, unsigned *counter, int nblocks);
+                       double *part, int nblocks);
 int  gradvel_dt_blocks(const Grid &g);
 void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm);
 void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
@@ -78,6 +81,10 @@ void launch_ejecta_flow(hipStream_t s, const Grid &g, const Fields &F, const Par
 void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats);
 void launch_advance_time(hipStream_t s, DevTime *tm);
 /* setup tail (src/salec2d.c:76-94) */
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm,
+                        double *bx, double *by);
+void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats,
+                        const double *bx, const double *by, const DevTime *tm);
 void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part, int nblocks);
 int  dampref_blocks(const Grid &g);
 void launch_init_v_field(hipStream_t s, const Grid &g, const Fields &F);
