@@ -27,6 +27,8 @@ enum { VT_ALWAYS_COPY = 0, VT_NORMAL, VT_VELOCITY_INC, VT_VELOCITY, VT_STRESS, V
 
 struct Program {
     GhostSlots slots{};
+    int field[4] = {0, 0, 0, 0};          /* slot -> field id; bases are taken at launch */
+    std::vector<int> phys_dst;            /* physical cells the program writes (cell grids) */
     int nslots = 0;
     GhostOp *d_ops = nullptr;
     int nops = 0;
@@ -56,7 +58,9 @@ struct salec_ctx {
     Fields F{};
     DevMat *d_mats = nullptr;
     std::vector<void *> owned;           /* device allocations to free */
-    double *bx = nullptr, *by = nullptr;
+    double *bx = nullptr, *by = nullptr;    /* per-material face volumes (x, y faces) */
+    double *d_top = nullptr;                 /* top-row m_den/m_eng snapshot          */
+    Fields Falt{};                           /* second buffers of the remapped fields */
     DevTime *d_time = nullptr;
     double *d_part = nullptr;            /* dt / dampref partial minima   */
     int gv_blocks = 0;
@@ -318,6 +322,7 @@ static int build_program(salec_ctx *c, Program &P, const std::vector<SlotSpec> &
         salec_field_info(f, c->g.nmat, &grid, &nc);
         int W = grid == SALEC_GRID_CELL ? c->g.CW : c->g.VW;
         int H = grid == SALEC_GRID_CELL ? c->g.CH : c->g.VH;
+        P.field[s] = f;
         P.slots.base[s] = c->F.f[f];
         P.slots.plane[s] = grid == SALEC_GRID_CELL ? c->g.NC : c->g.NV;
         P.slots.ncomp[s] = nc;
@@ -335,6 +340,10 @@ static int build_program(salec_ctx *c, Program &P, const std::vector<SlotSpec> &
             o.slot = s; o.dst = L; o.src = e.src; o.m0 = e.m[0]; o.m1 = e.m[1]; o.mr = e.m[2]; o.pad = 0;
             ops.push_back(o);
             dsts[L] = 1;
+            if (grid == SALEC_GRID_CELL) {
+                int x = L % W, y = L / W;
+                if (x >= 2 && x < W - 2 && y >= 2 && y < H - 2) P.phys_dst.push_back(L);
+            }
         }
         for (size_t i = first; i < ops.size(); ++i) {
             const GhostOp &o = ops[i];
@@ -355,8 +364,13 @@ static int build_program(salec_ctx *c, Program &P, const std::vector<SlotSpec> &
 
 static void run_program(salec_ctx *c, const Program &P)
 {
-    launch_ghost(c->stream, P.slots, P.d_ops, P.nops, c->d_stage, P.two_phase, P.maxc);
+    GhostSlots sl = P.slots;   /* field buffers swap between steps */
+    for (int s = 0; s < P.nslots; ++s) sl.base[s] = c->F.f[P.field[s]];
+    launch_ghost(c->stream, sl, P.d_ops, P.nops, c->d_stage, P.two_phase, P.maxc);
 }
+
+/* Fields the remap writes into the second buffer set (k_remap_close). */
+const int kSwapped[] = {SALEC_M_DEN, SALEC_M_ENG, SALEC_E_STE, SALEC_E_DAM, SALEC_E_TPS, SALEC_E_VIB, SALEC_E_EJT};
 
 static DevMat to_dev(const salec_material &m)
 {
@@ -429,8 +443,14 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
         salec_field_info(f, g.nmat, &grid, &nc);
         TRY(dalloc(c, &c->F.f[f], (size_t)salec_grid_points(grid, g.nx, g.ny) * nc));
     }
-    TRY(dalloc(c, &c->bx, (size_t)g.PL * g.NBX));
-    TRY(dalloc(c, &c->by, (size_t)g.PL * g.NBY));
+    for (int f : kSwapped) {
+        int grid, nc;
+        salec_field_info(f, g.nmat, &grid, &nc);
+        TRY(dalloc(c, &c->Falt.f[f], (size_t)salec_grid_points(grid, g.nx, g.ny) * nc));
+    }
+    TRY(dalloc(c, &c->bx, (size_t)g.nmat * g.NBX));
+    TRY(dalloc(c, &c->by, (size_t)g.nmat * g.NBY));
+    TRY(dalloc(c, &c->d_top, (size_t)2 * g.nmat * g.CW));
     TRY(dalloc(c, &c->d_time, 1));
     c->gv_blocks = gradvel_dt_blocks(g);
     int nb = std::max(c->gv_blocks, dampref_blocks(g));
@@ -466,6 +486,11 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     TRY(build_program(c, c->gfE, {{SALEC_E_VEL, VT_VELOCITY}}));
     TRY(build_program(c, c->gfB, {{SALEC_E_PRE, VT_ALWAYS_COPY}, {SALEC_E_STE, VT_STRESS}, {SALEC_E_Q, VT_ALWAYS_COPY}}));
     TRY(build_program(c, c->gfC, {{SALEC_M_DEN, VT_DENSITY}, {SALEC_E_VIB, VT_ALWAYS_COPY}, {SALEC_M_ENG, VT_DENSITY}}));
+    /* the remap reads donor m_den/m_eng after this fill and relies on it
+     * changing no physical cell outside the top row (ght_clean), which the
+     * flux kernel snapshots */
+    for (int L : c->gfC.phys_dst)
+        if (L / g.CW != g.CH - 3) { salec_destroy(c); return SALEC_EINVAL; }
     TRY(build_program(c, c->rstV, {{SALEC_V_VEL, VT_RESTRICT}}));
     TRY(build_program(c, c->rstA, {{SALEC_V_ACC, VT_RESTRICT}}));
     TRY(build_program(c, c->rstD, {{SALEC_V_DIS, VT_RESTRICT}}));
@@ -682,10 +707,15 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     PHASE(PH_VDIS, launch_v_dis(s, g, F, c->d_time));
     PHASE(PH_RST_DIS, run_program(c, c->rstD));
     PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
-    PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, c->d_mats, c->d_time, c->bx, c->by));
+    PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, c->d_mats, c->d_time, c->bx, c->by, c->d_top));
     PHASE(PH_GF_C, run_program(c, c->gfC));
-    PHASE(PH_FLUX_RING, launch_flux(s, g, F, c->bx, c->by, 1));
-    PHASE(PH_REMAP_CLOSE, launch_remap_close(s, g, F, P, c->d_mats, c->bx, c->by, c->d_time));
+    PHASE(PH_FLUX_RING, launch_flux_ring(s, g, F, c->bx, c->by));
+    {
+        Fields Fn = c->F;
+        for (int f : kSwapped) Fn.f[f] = c->Falt.f[f];
+        PHASE(PH_REMAP_CLOSE, launch_remap_close(s, g, F, Fn, P, c->d_mats, c->bx, c->by, c->d_top, c->d_time));
+        for (int f : kSwapped) std::swap(c->F.f[f], c->Falt.f[f]);
+    }
     PHASE(PH_TIME, launch_advance_time(s, c->d_time));
 #undef PHASE
     HIPCHK(hipGetLastError());

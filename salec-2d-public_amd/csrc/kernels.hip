@@ -703,20 +703,6 @@ __device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, cons
     }
     CPL(E_CVS, 0, c) = dWind(CPL(E_CVS, 0, c) + dcvs * dt, -10.0, 10.0);
 }
-template <int NM>
-__global__ void k_energy(Grid g, Fields F, const DevMat *mats, const DevTime *tm)
-{
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
-    energy_cell<NM>(g, F, mats, tm->dt, k, j);
-}
-void launch_energy(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm)
-{
-    nm_dispatch(g.nmat, [&](auto I) {
-        k_energy<decltype(I)::value><<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, mats, tm);
-    });
-}
-
 /* ------------------------------------------------------------- tracers -- */
 /* fl_update_node, src/tracer.c:52-175; a tracer that leaves the physical
  * cells is deleted (its neighbour rank is absent on a single domain). */
@@ -879,33 +865,15 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, double
             for (int i = 0; i < 4; ++i) fv[q][i] *= sc;
         }
     }
+    /* the donor side of each outflow face records the face's per-material
+     * volumes; the receiver reads everything else from the donor cell */
     long fi[4] = {k + (long)g.BXW * j, k + (long)g.BYW * j, k + 1 + (long)g.BXW * j, k + (long)g.BYW * (j + 1)};
     for (int i = 0; i < 4; ++i) {
         if (lbru[i] >= 0.) continue;
         double *ph = (i == 0 || i == 2) ? bx : by;
         long st = (i == 0 || i == 2) ? g.NBX : g.NBY;
-        long o = fi[i];
-        for (int q = 0; q < n; ++q) {
-            ph[(long)(q) * st + o] = fv[q][i];
-            ph[(long)(q + n) * st + o] = CPL(M_DEN, q, c);
-            ph[(long)(q + 2 * n) * st + o] = CPL(M_ENG, q, c);
-        }
-        double fm = 1.0;
-        ph[(long)(3 * n) * st + o] = CPL(E_VEL, 0, c);
-        ph[(long)(3 * n + 1) * st + o] = CPL(E_VEL, 1, c);
-        for (int q = 0; q < 4; ++q) ph[(long)(3 * n + 2 + q) * st + o] = fm * CPL(E_STE, q, c);
-        ph[(long)(3 * n + 6) * st + o] = fm * CPL(E_DAM, 0, c);
-        ph[(long)(3 * n + 7) * st + o] = fm * CPL(E_TPS, 0, c);
-        ph[(long)(3 * n + 8) * st + o] = fm * CPL(E_VIB, 0, c);
-        ph[(long)(3 * n + 9) * st + o] = fm * CPL(E_EJT, 0, c);
+        for (int q = 0; q < n; ++q) ph[(long)q * st + fi[i]] = fv[q][i];
     }
-}
-template <int NM>
-__global__ void __launch_bounds__(128) k_flux_physical(Grid g, Fields F, double *bx, double *by)
-{
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
-    flux_cell<NM>(g, F, bx, by, k, j);
 }
 /* ghost ring (distance < 2): rows 0,1,CH-2,CH-1 and columns 0,1,CW-2,CW-1 */
 template <int NM>
@@ -928,23 +896,35 @@ __global__ void __launch_bounds__(128) k_flux_ring(Grid g, Fields F, double *bx,
     }
     flux_cell<NM>(g, F, bx, by, k, j);
 }
-void launch_flux(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by, int ring)
+void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by)
 {
     nm_dispatch(g.nmat, [&](auto I) {
         constexpr int NM = decltype(I)::value;
-        if (!ring) k_flux_physical<NM><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
-        else k_flux_ring<NM><<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
+        k_flux_ring<NM><<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
     });
 }
 
 /* --------------------------------------------------------- advect_flux -- */
-/* advect_flux_default, src/advect.c:26-302 (porosity model off).  A face
- * holds a packet from this step iff its swept volume is non-zero: the cell
- * on the outflow side wrote it; faces with zero volume were zeroed by
- * clean_flux in the reference and contribute exactly nothing. */
+/* advect_flux_default, src/advect.c:26-302 (porosity model off).
+ *
+ * The reference's donor packs a face packet (_calculate_flux_set_bphi,
+ * src/advect.c:1138-1224): its per-material volumes, densities, energies,
+ * velocity, stress, damage, tps, vibration and ejecta flag.  Here the donor
+ * stores only the volumes (fx/fy); the receiver reads the rest from the
+ * donor cell itself.  That is the same data provided the donor's values are
+ * the ones it had when the packet would have been made:
+ *  - advect's own outputs go to a second buffer set (Fn), so no cell reads
+ *    a neighbour's updated state;
+ *  - the only in-between write to donor data is the density ghost fill,
+ *    which zeroes m_den/m_eng of the top physical row (ght_clean); the flux
+ *    kernel snapshots that row into `top` first;
+ *  - ghost-ring donors pack after that fill in the reference too.
+ * A face carries flux iff its swept volume is non-zero; the donor is the
+ * side on which it is negative (exactly antisymmetric, src/advect.c:1347). */
+#define CPN(fld, c, i) (Fn.f[SALEC_##fld][(long)(c) * g.NC + (i)])
 template <int NM>
-__device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const Params &P, const double *bx,
-                                            const double *by, int k, int j)
+__device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
+                                            const double *bx, const double *by, const double *top, int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g), PL = 3 * n + 10;
@@ -963,19 +943,29 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     const double dir[4] = {1.0, 1.0, -1.0, -1.0};
     double dphi[3 * Mats<NM>::cap + 10];
     for (int q = 0; q < PL; ++q) dphi[q] = 0.0;
+    const long nb[4] = {c - 1, c - g.CW, c + 1, c + g.CW};
+    const int nk[4] = {k - 1, k, k + 1, k}, nj[4] = {j, j - 1, j, j + 1};
     for (int i = 0; i < 4; ++i) {
         if (!(sv[i] < 0. || sv[i] > 0.)) continue;
         const double *ph = (i == 0 || i == 2) ? bx : by;
         long st = (i == 0 || i == 2) ? g.NBX : g.NBY;
         long o = fi[i];
+        const bool self = sv[i] < 0.;
+        const long d = self ? c : nb[i];
+        const int kd = self ? k : nk[i], jd = self ? j : nj[i];
+        const bool snap = jd == g.CH - 3 && kd >= 2 && kd < g.CW - 2;
         double tail[10];
-        for (int q = 0; q < 10; ++q) tail[q] = ph[(long)(3 * n + q) * st + o];
+        tail[0] = CPL(E_VEL, 0, d); tail[1] = CPL(E_VEL, 1, d);
+        for (int q = 0; q < 4; ++q) tail[2 + q] = CPL(E_STE, q, d);
+        tail[6] = CPL(E_DAM, 0, d); tail[7] = CPL(E_TPS, 0, d);
+        tail[8] = CPL(E_VIB, 0, d); tail[9] = CPL(E_EJT, 0, d);
         for (int m = 0; m < n; ++m) {
             double mv = dir[i] * ph[(long)m * st + o];
-            double dm = ph[(long)(m + n) * st + o];
+            double dm = snap ? top[(long)m * g.CW + kd] : CPL(M_DEN, m, d);
+            double de = snap ? top[(long)(n + m) * g.CW + kd] : CPL(M_ENG, m, d);
             dphi[m] += mv;
             dphi[m + n] += mv * dm;
-            dphi[m + 2 * n] += mv * ph[(long)(m + 2 * n) * st + o];
+            dphi[m + 2 * n] += mv * de;
             double pa = mv * dm;
             for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * tail[q];
         }
@@ -1049,26 +1039,12 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
         dv0 = dv1 = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
         dam = tps = vib = ejt = 0.;
     }
-    for (int m = 0; m < n; ++m) { CPL(M_VOF, m, c) = ev[m]; CPL(M_DEN, m, c) = md[m]; CPL(M_ENG, m, c) = me[m]; }
+    for (int m = 0; m < n; ++m) { CPL(M_VOF, m, c) = ev[m]; CPN(M_DEN, m, c) = md[m]; CPN(M_ENG, m, c) = me[m]; }
     CPL(E_DVEL, 0, c) = dv0; CPL(E_DVEL, 1, c) = dv1;
-    for (int q = 0; q < 4; ++q) CPL(E_STE, q, c) = ste[q];
-    CPL(E_DAM, 0, c) = dam; CPL(E_TPS, 0, c) = tps; CPL(E_VIB, 0, c) = vib; CPL(E_EJT, 0, c) = ejt;
+    for (int q = 0; q < 4; ++q) CPN(E_STE, q, c) = ste[q];
+    CPN(E_DAM, 0, c) = dam; CPN(E_TPS, 0, c) = tps; CPN(E_VIB, 0, c) = vib; CPN(E_EJT, 0, c) = ejt;
     CPL(E_DEN, 0, c) = eden;
 }
-template <int NM>
-__global__ void __launch_bounds__(128) k_advect(Grid g, Fields F, Params P, const double *bx, const double *by)
-{
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
-    advect_cell<NM>(g, F, P, bx, by, k, j);
-}
-void launch_advect(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *bx, const double *by)
-{
-    nm_dispatch(g.nmat, [&](auto I) {
-        k_advect<decltype(I)::value><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, P, bx, by);
-    });
-}
-
 /* ---------------------------------------------------------- damp_on_cell -- */
 /* damp_on_cell_pml2 (src/velocity.c:487-588), _pml (:378-483), _sbc
  * (:688-767); all cells including the ghost ring; inactive once t >= damp_time */
@@ -1153,22 +1129,6 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
             }
     }
 }
-template <int NM>
-__global__ void k_damp(Grid g, Fields F, Params P, const DevTime *tm)
-{
-    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (c >= g.NC) return;
-    if (P.damp_time <= tm->t) return;
-    damp_cell<NM>(g, F, P, tm->dt, c);
-}
-void launch_damp(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
-{
-    if (P.damp_type != SALEC_DAMP_PML && P.damp_type != SALEC_DAMP_PML2 && P.damp_type != SALEC_DAMP_SBC) return;
-    nm_dispatch(g.nmat, [&](auto I) {
-        k_damp<decltype(I)::value><<<nblk(g.NC, 256), 256, 0, s>>>(g, F, P, tm);
-    });
-}
-
 /* ---------------------------------------------------------- ejecta_flow -- */
 /* src/ejecta.c:65-134 */
 __device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, const Params &P, double dt, int k, int j)
@@ -1191,18 +1151,6 @@ __device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, cons
         }
     }
 }
-__global__ void k_ejecta_flow(Grid g, Fields F, Params P, const DevTime *tm)
-{
-    int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
-    ejecta_cell(g, F, P, tm->dt, k, j);
-}
-void launch_ejecta_flow(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
-{
-    if (P.flow_resistance <= 0. && P.flow_friction <= 0.) return;
-    k_ejecta_flow<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, tm);
-}
-
 /* --------------------------------------------------------- update_state -- */
 /* src/sale2d.c:1635-1742 */
 template <int NM>
@@ -1262,51 +1210,71 @@ void launch_advance_time(hipStream_t s, DevTime *tm) { k_advance_time<<<1, 64, 0
  * (src/advect.c:861-1224) on the same physical cell.  The tracer update
  * that the reference runs between them reads neither m_eng nor e_cvs, and
  * a cell's flux reads only its own energy, so the per-cell chain gives the
- * reference's values. */
+ * reference's values.  Cells of the top physical row also snapshot their
+ * m_den and m_eng (see advect_cell). */
 template <int NM>
 __global__ void __launch_bounds__(128) k_energy_flux(Grid g, Fields F, const DevMat *mats, const DevTime *tm,
-                                                     double *bx, double *by)
+                                                     double *bx, double *by, double *top)
 {
     int k, j;
     if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
     energy_cell<NM>(g, F, mats, tm->dt, k, j);
     flux_cell<NM>(g, F, bx, by, k, j);
+    if (j == g.CH - 3) {
+        const int n = Mats<NM>::n(g);
+        long c = k + (long)g.CW * j;
+        for (int m = 0; m < n; ++m) {
+            top[(long)m * g.CW + k] = CPL(M_DEN, m, c);
+            top[(long)(n + m) * g.CW + k] = CPL(M_ENG, m, c);
+        }
+    }
 }
 void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm,
-                        double *bx, double *by)
+                        double *bx, double *by, double *top)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_energy_flux<decltype(I)::value><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, mats, tm, bx, by);
+        k_energy_flux<decltype(I)::value><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, mats, tm, bx, by, top);
     });
 }
 
 /* advect_flux, damp_on_cell, ejecta_flow and update_state
  * (src/salec2d.c:248-257) as one per-cell chain over the padded grid:
  * physical cells run all four, ghost-ring cells only the damping, which is
- * all the reference does to them at this point.  Each phase reads and
- * writes only its own cell (advect reads its neighbours through the face
- * packets written before this kernel), so no cell sees another's update. */
+ * all the reference does to them at this point.  Reads come from F; the
+ * fields a neighbour's remap reads (m_den, m_eng, e_ste, e_dam, e_tps,
+ * e_vib, e_ejt) are written to the second buffer set Fn, ring cells copying
+ * theirs across, and the caller swaps the two sets afterwards.  Each phase
+ * otherwise touches only its own cell. */
 template <int NM>
-__global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, Params P, const DevMat *mats,
-                                                     const double *bx, const double *by, const DevTime *tm)
+__global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, Fields Fn, Params P, const DevMat *mats,
+                                                     const double *bx, const double *by, const double *top,
+                                                     const DevTime *tm)
 {
     long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (c >= g.NC) return;
     int k = (int)(c % g.CW), j = (int)(c / g.CW);
     bool phys = k >= 2 && k < g.CW - 2 && j >= 2 && j < g.CH - 2;
+    const int n = Mats<NM>::n(g);
     const double dt = tm->dt, t = tm->t;
-    if (phys) advect_cell<NM>(g, F, P, bx, by, k, j);
-    if (!(P.damp_time <= t)) damp_cell<NM>(g, F, P, dt, c);
     if (phys) {
-        if (P.flow_resistance > 0. || P.flow_friction > 0.) ejecta_cell(g, F, P, dt, k, j);
-        state_cell<NM>(g, F, P, mats, k, j);
+        advect_cell<NM>(g, F, Fn, P, bx, by, top, k, j);
+    } else {
+        for (int m = 0; m < n; ++m) { CPN(M_DEN, m, c) = CPL(M_DEN, m, c); CPN(M_ENG, m, c) = CPL(M_ENG, m, c); }
+        for (int q = 0; q < 4; ++q) CPN(E_STE, q, c) = CPL(E_STE, q, c);
+        CPN(E_DAM, 0, c) = CPL(E_DAM, 0, c); CPN(E_TPS, 0, c) = CPL(E_TPS, 0, c);
+        CPN(E_VIB, 0, c) = CPL(E_VIB, 0, c); CPN(E_EJT, 0, c) = CPL(E_EJT, 0, c);
+    }
+    if (!(P.damp_time <= t)) damp_cell<NM>(g, Fn, P, dt, c);
+    if (phys) {
+        if (P.flow_resistance > 0. || P.flow_friction > 0.) ejecta_cell(g, Fn, P, dt, k, j);
+        state_cell<NM>(g, Fn, P, mats, k, j);
     }
 }
-void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats,
-                        const double *bx, const double *by, const DevTime *tm)
+void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
+                        const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_remap_close<decltype(I)::value><<<nblk(g.NC, 128), 128, 0, s>>>(g, F, P, mats, bx, by, tm);
+        k_remap_close<decltype(I)::value><<<nblk(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
     });
 }
 

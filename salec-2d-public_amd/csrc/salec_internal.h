@@ -71,20 +71,16 @@ int  gradvel_dt_blocks(const Grid &g);
 void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm);
 void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
 void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const DevTime *tm);
-void launch_energy(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm);
 void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const Tracers &T,
                     const double *vx, const double *vy, const DevTime *tm);
-void launch_flux(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by, int ring);
-void launch_advect(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *bx, const double *by);
-void launch_damp(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
-void launch_ejecta_flow(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
 void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats);
 void launch_advance_time(hipStream_t s, DevTime *tm);
 /* setup tail (src/salec2d.c:76-94) */
+void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by);
 void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm,
-                        double *bx, double *by);
-void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats,
-                        const double *bx, const double *by, const DevTime *tm);
+                        double *bx, double *by, double *top);
+void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
+                        const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm);
 void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part, int nblocks);
 int  dampref_blocks(const Grid &g);
 void launch_init_v_field(hipStream_t s, const Grid &g, const Fields &F);
