@@ -60,6 +60,38 @@ def phase_bytes(n):
     }
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (GB/s, read + write) of a plain 1 GiB
+    buffer copy through the HIP runtime the solver library uses: the
+    practical HBM ceiling beside the 8 TB/s peak.  Under a `rocprofv3 --pmc`
+    run its copy kernel also calibrates FETCH_SIZE and WRITE_SIZE against a
+    known byte count."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    x, y = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(x), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(y), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemset(x, 0, C.c_size_t(nbytes))
+    kD2D = 3  # hipMemcpyDeviceToDevice
+    hip.hipMemcpy(y, x, C.c_size_t(nbytes), kD2D)
+    hip.hipDeviceSynchronize()
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(y, x, C.c_size_t(nbytes), kD2D, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip.hipFree(x)
+    hip.hipFree(y)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    return 2.0 * nbytes / (ms.value / reps * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,11 +180,14 @@ def main():
     achieved = bpu * units / (avg_ms * 1e-3) / 1e9
     b_cell = 8 * (6 * nmat + 111)
     step_gbs = (b_cell * cells + 104 * tr.n) / (ms_step * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth()
+    log("device copy: %.1f GB/s" % copy_gbs)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": dom[0], "bytes_per_unit": bpu, "units_per_launch": units, "unit_kind": unit,
                 "avg_launch_ms": round(avg_ms, 5),
                 "step_algorithmic_GBs": round(step_gbs, 2), "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                "copy_GBs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
                 "step_bytes_per_cell": b_cell}
 
     cpu = None

@@ -372,6 +372,40 @@ static void run_program(salec_ctx *c, const Program &P)
 /* Fields the remap writes into the second buffer set (k_remap_close). */
 const int kSwapped[] = {SALEC_M_DEN, SALEC_M_ENG, SALEC_E_STE, SALEC_E_DAM, SALEC_E_TPS, SALEC_E_VIB, SALEC_E_EJT};
 
+/* host bisearch (same rule as the device's) for the hint table */
+static int host_bisearch(double x, const double *xl, int n)
+{
+    int left = 0, right = n - 1;
+    if (x < xl[left]) return left;
+    if (x >= xl[right]) return right - 1;
+    while (left + 1 < right) {
+        int mid = (left + right) / 2;
+        if (xl[mid] > x) right = mid; else left = mid;
+    }
+    return left;
+}
+
+/* AxisHint for one table axis (see salec_internal.h); nb = 0 unless the
+ * axis is strictly increasing with a positive second node. */
+static int build_hint(salec_ctx *c, const double *xl, int n, AxisHint *h)
+{
+    h->nb = 0; h->start = nullptr;
+    if (n < 3 || !(xl[1] > 0.0)) return SALEC_OK;
+    for (int i = 1; i < n; ++i) if (!(xl[i] > xl[i - 1])) return SALEC_OK;
+    int nb = 2 * n;
+    double l1 = std::log(xl[1]), ln = std::log(xl[n - 1]);
+    if (!(ln > l1)) return SALEC_OK;
+    double inv = nb / (ln - l1);
+    std::vector<int> st(nb);
+    for (int b = 0; b < nb; ++b) st[b] = host_bisearch(std::exp(l1 + b / inv), xl, n);
+    int *d;
+    int rc = dalloc(c, &d, (size_t)nb);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(d, st.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    h->start = d; h->l1 = l1; h->inv = inv; h->nb = nb;
+    return SALEC_OK;
+}
+
 static DevMat to_dev(const salec_material &m)
 {
     DevMat d;
@@ -477,6 +511,10 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
             TRY(hip_ok(hipMemcpy(xt, mm.tab_xtem, sizeof(double) * mm.tab_ntem, hipMemcpyHostToDevice)));
             TRY(hip_ok(hipMemcpy(dd, mm.tab_data, sizeof(double) * mm.tab_nden * mm.tab_ntem * 3, hipMemcpyHostToDevice)));
             dm[m].tab_yden = yd; dm[m].tab_xtem = xt; dm[m].tab_data = dd;
+            if (mm.eos_type == SALEC_EOS_ANEOS_SD) {
+                TRY(build_hint(c, mm.tab_yden, mm.tab_nden, &dm[m].hden));
+                TRY(build_hint(c, mm.tab_xtem, mm.tab_ntem, &dm[m].heng));
+            }
         }
     }
     TRY(dalloc(c, &c->d_mats, SALEC_MAXMAT));

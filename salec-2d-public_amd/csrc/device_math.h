@@ -42,6 +42,24 @@ __device__ __forceinline__ int bisearch(double x, const double *xl, int n)
     return left;
 }
 
+/* bisearch with a hint (AxisHint): the same interval index, found from a
+ * log-spaced start instead of ~log2(n) dependent loads.  For x in
+ * [x[1], x[n-1]) the walk below ends at the unique i with x[i] <= x < x[i+1],
+ * which is what bisearch converges to on a strictly increasing axis; the
+ * edges and NaN take bisearch's own branches. */
+__device__ __forceinline__ int axis_index(double x, const double *xl, int n, const AxisHint &h)
+{
+    if (h.nb == 0 || x != x) return bisearch(x, xl, n);
+    if (x < xl[1]) return 0;
+    if (x >= xl[n - 1]) return n - 2;
+    double u = (log(x) - h.l1) * h.inv;
+    int b = u < 0. ? 0 : (u >= (double)(h.nb - 1) ? h.nb - 1 : (int)u);
+    int i = h.start[b];
+    while (i > 0 && xl[i] > x) --i;
+    while (i < n - 2 && xl[i + 1] <= x) ++i;
+    return i;
+}
+
 /* TillPres with TillCold/TillHot/TillTran, src/eos/eos_state.c:322-438 */
 __device__ inline void till_pres(const DevMat &t, double E, double R, double *P, double *Cs)
 {
@@ -96,8 +114,8 @@ __device__ inline double till_temp(const DevMat &t, double E, double R)
 __device__ inline void sd_state(const DevMat &t, double den, double eng, double *tem, double *pre, double *csd)
 {
     double tDen = den, tEng = eng / den;
-    int DI = bisearch(tDen, t.tab_yden, t.tab_nden), DJ = DI + 1;
-    int TI = bisearch(tEng, t.tab_xtem, t.tab_ntem), TJ = TI + 1;
+    int DI = axis_index(tDen, t.tab_yden, t.tab_nden, t.hden), DJ = DI + 1;
+    int TI = axis_index(tEng, t.tab_xtem, t.tab_ntem, t.heng), TJ = TI + 1;
     double Dxr = (tDen - t.tab_yden[DI]) / (t.tab_yden[DJ] - t.tab_yden[DI]);
     double Dxl = dWind(1.0 - Dxr, 0.0, 1.0);
     double Exr = (tEng - t.tab_xtem[TI]) / (t.tab_xtem[TJ] - t.tab_xtem[TI]);

@@ -12,6 +12,17 @@
 
 namespace salec {
 
+/* Search hint of a strictly increasing table axis whose second node is
+ * positive: the range [x[1], x[n-1]) is cut into nb equal steps of log(x),
+ * and start[b] is the interval index of the lower edge of step b.  A lookup
+ * starts there and walks to the interval holding x, so it returns what the
+ * binary search returns.  nb = 0 means: use the binary search. */
+struct AxisHint {
+    const int *start;
+    double l1, inv;
+    int nb, pad_;
+};
+
 /* material parameters as the kernels read them (state_reference + tables) */
 struct DevMat {
     int eos_type, yfunc, dfunc, tfunc;
@@ -27,6 +38,7 @@ struct DevMat {
     double Tlidc1, Tlidc2, Tlidc3, Tsolc1, Tsolc2, Tsolc3;
     double Toff, Cvib, VibMax, Tdamp, Pvlim, Acvis;
     double alpha0, chi, gamfrac, melt_viscosity;
+    AxisHint hden, heng;            /* search hints of the table axes       */
 };
 
 struct Grid {
