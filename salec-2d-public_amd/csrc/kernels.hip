@@ -19,6 +19,34 @@ namespace salec {
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+/* XCD-aware mapping of the grid sweeps.  Workgroups are dispatched to the
+ * eight XCDs round-robin, so consecutive blocks (adjacent rows of the mesh)
+ * land on different XCDs, and each XCD's L2 fetches the rows its stencils
+ * share with other XCDs again from HBM.  Here consecutive logical blocks are
+ * grouped in chunks of XCD_CHUNK threads (a few mesh rows) and whole chunks
+ * are dealt to the XCDs in turn: rows inside a chunk share one L2, and the
+ * interleave of chunks keeps the work of every XCD spread over the mesh
+ * (vacuum and target rows cost very different amounts).  Launch sizes are
+ * a multiple of NXCD * blocks-per-chunk so the remap is a bijection of
+ * block indices: every element is still handled exactly once. */
+#define NXCD 8
+#ifndef XCD_CHUNK
+#define XCD_CHUNK 4096
+#endif
+static inline int nblk8(long n, int bs)
+{
+    if (XCD_CHUNK == 0) return nblk(n, bs);
+    int grp = NXCD * (XCD_CHUNK / bs);
+    return (nblk(n, bs) + grp - 1) / grp * grp;
+}
+__device__ __forceinline__ long xcd_thread()
+{
+    const long G = XCD_CHUNK / blockDim.x;
+    if (G == 0 || gridDim.x % (NXCD * G)) return blockIdx.x * (long)blockDim.x + threadIdx.x;  /* not from nblk8 */
+    const long b = blockIdx.x, x = b % NXCD, q = b / NXCD;
+    return (((q / G) * NXCD + x) * G + q % G) * (long)blockDim.x + threadIdx.x;
+}
+
 /* Material count as a template parameter: NM > 0 makes it a compile-time
  * constant, so per-material arrays are fully unrolled into registers
  * instead of living in scratch; NM = 0 is the generic form for any nmat. */
@@ -175,7 +203,7 @@ __device__ __forceinline__ bool physical_cell(const Grid &g, long t, int &k, int
 __global__ void k_vertex_vof_vel(Grid g, Fields F, Params P)
 {
     int k, j;
-    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!active_vertex(g, xcd_thread(), k, j)) return;
     long v = k + (long)g.VW * j;
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
     for (int m = 0; m < g.nmat; ++m) {
@@ -219,7 +247,7 @@ __global__ void k_vertex_vof_vel(Grid g, Fields F, Params P)
 void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const Params &P)
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
-    k_vertex_vof_vel<<<nblk(n, 256), 256, 0, s>>>(g, F, P);
+    k_vertex_vof_vel<<<nblk8(n, 256), 256, 0, s>>>(g, F, P);
 }
 
 /* ------------------------------------------------------- update_e_vel -- */
@@ -227,7 +255,7 @@ void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const 
 __global__ void k_update_e_vel(Grid g, Fields F)
 {
     int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!physical_cell(g, xcd_thread(), k, j)) return;
     long c = k + (long)g.CW * j;
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
     double den = CPL(E_DEN, 0, c);
@@ -257,7 +285,7 @@ __global__ void k_update_e_vel(Grid g, Fields F)
 }
 void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F)
 {
-    k_update_e_vel<<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F);
+    k_update_e_vel<<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F);
 }
 
 /* -------------------------------------------------------- check_v_vel -- */
@@ -265,7 +293,7 @@ void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F)
 __global__ void k_check_v_vel(Grid g, Fields F)
 {
     int k, j;
-    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!active_vertex(g, xcd_thread(), k, j)) return;
     long v = k + (long)g.VW * j;
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
     const int voff[4] = {2, 3, 0, 1};
@@ -291,7 +319,7 @@ __global__ void k_check_v_vel(Grid g, Fields F)
 void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F)
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
-    k_check_v_vel<<<nblk(n, 256), 256, 0, s>>>(g, F);
+    k_check_v_vel<<<nblk8(n, 256), 256, 0, s>>>(g, F);
 }
 
 /* ------------------------------------------- update_e_gradvel + dt min -- */
@@ -315,7 +343,7 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, Params P
     int k, j;
     double cand = HUGE_VAL;  /* identity of MIN over finite values */
     bool has = false;
-    if (physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) {
+    if (physical_cell(g, xcd_thread(), k, j)) {
         long c = k + (long)g.CW * j;
         long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
         double vx[4], vy[4];
@@ -384,7 +412,7 @@ __global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, cons
         tm->dt = dMin(P.max_dt, rtn);
     }
 }
-int gradvel_dt_blocks(const Grid &g) { return nblk((long)g.nx * g.ny, GV_BS); }
+int gradvel_dt_blocks(const Grid &g) { return nblk8((long)g.nx * g.ny, GV_BS); }
 void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const Params &P, DevTime *tm,
                        double *part, int nblocks)
 {
@@ -399,7 +427,7 @@ template <int NM>
 __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, const DevMat *mats, const DevTime *tm)
 {
     int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!physical_cell(g, xcd_thread(), k, j)) return;
     const double dt = tm->dt, time = tm->t;
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g);
@@ -543,7 +571,7 @@ __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, cons
 void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_stress<decltype(I)::value><<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats, tm);
+        k_stress<decltype(I)::value><<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats, tm);
     });
 }
 
@@ -556,7 +584,7 @@ template <bool PML>
 __global__ void k_acc(Grid g, Fields F, Params P, const DevTime *tm)
 {
     int k, j;
-    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!active_vertex(g, xcd_thread(), k, j)) return;
     const double dt = tm->dt;
     long v = k + (long)g.VW * j;
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
@@ -647,9 +675,9 @@ void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, 
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
     if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3)
-        k_acc<true><<<nblk(n, 256), 256, 0, s>>>(g, F, P, tm);
+        k_acc<true><<<nblk8(n, 256), 256, 0, s>>>(g, F, P, tm);
     else
-        k_acc<false><<<nblk(n, 256), 256, 0, s>>>(g, F, P, tm);
+        k_acc<false><<<nblk8(n, 256), 256, 0, s>>>(g, F, P, tm);
 }
 
 /* ------------------------------------------------------ calculate_v_dis -- */
@@ -657,7 +685,7 @@ void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, 
 __global__ void k_v_dis(Grid g, Fields F, const DevTime *tm)
 {
     int k, j;
-    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!active_vertex(g, xcd_thread(), k, j)) return;
     const double dt = tm->dt;
     long v = k + (long)g.VW * j;
     double pb = 0.5 * dt * dt;
@@ -670,7 +698,7 @@ __global__ void k_v_dis(Grid g, Fields F, const DevTime *tm)
 void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const DevTime *tm)
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
-    k_v_dis<<<nblk(n, 256), 256, 0, s>>>(g, F, tm);
+    k_v_dis<<<nblk8(n, 256), 256, 0, s>>>(g, F, tm);
 }
 
 /* -------------------------------------------------------- update_energy -- */
@@ -1188,13 +1216,13 @@ template <int NM>
 __global__ void k_update_state(Grid g, Fields F, Params P, const DevMat *mats)
 {
     int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!physical_cell(g, xcd_thread(), k, j)) return;
     state_cell<NM>(g, F, P, mats, k, j);
 }
 void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_update_state<decltype(I)::value><<<nblk((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats);
+        k_update_state<decltype(I)::value><<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats);
     });
 }
 
@@ -1217,7 +1245,7 @@ __global__ void __launch_bounds__(128) k_energy_flux(Grid g, Fields F, const Dev
                                                      double *bx, double *by, double *top)
 {
     int k, j;
-    if (!physical_cell(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!physical_cell(g, xcd_thread(), k, j)) return;
     energy_cell<NM>(g, F, mats, tm->dt, k, j);
     flux_cell<NM>(g, F, bx, by, k, j);
     if (j == g.CH - 3) {
@@ -1233,7 +1261,7 @@ void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const Dev
                         double *bx, double *by, double *top)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_energy_flux<decltype(I)::value><<<nblk((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, mats, tm, bx, by, top);
+        k_energy_flux<decltype(I)::value><<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, mats, tm, bx, by, top);
     });
 }
 
@@ -1250,7 +1278,7 @@ __global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, Fields Fn
                                                      const double *bx, const double *by, const double *top,
                                                      const DevTime *tm)
 {
-    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long c = xcd_thread();
     if (c >= g.NC) return;
     int k = (int)(c % g.CW), j = (int)(c / g.CW);
     bool phys = k >= 2 && k < g.CW - 2 && j >= 2 && j < g.CH - 2;
@@ -1274,7 +1302,7 @@ void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fie
                         const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_remap_close<decltype(I)::value><<<nblk(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
+        k_remap_close<decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
     });
 }
 
@@ -1321,7 +1349,7 @@ void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part,
 __global__ void k_init_v_field(Grid g, Fields F)
 {
     int k, j;
-    if (!active_vertex(g, blockIdx.x * (long)blockDim.x + threadIdx.x, k, j)) return;
+    if (!active_vertex(g, xcd_thread(), k, j)) return;
     long v = k + (long)g.VW * j;
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
     double u0 = 0.0, u1 = 0.0, em = 0., evv = 0.;
@@ -1346,7 +1374,7 @@ __global__ void k_init_v_field(Grid g, Fields F)
 void launch_init_v_field(hipStream_t s, const Grid &g, const Fields &F)
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
-    k_init_v_field<<<nblk(n, 256), 256, 0, s>>>(g, F);
+    k_init_v_field<<<nblk8(n, 256), 256, 0, s>>>(g, F);
 }
 
 /* first non-finite element (flat index + 1, 0 if none) */

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsalec.so")
+# SALEC_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("SALEC_LIBRARY") or os.path.join(HERE, "libsalec.so")
 
 FIELDS = [
     ("M_VOF", 0, -1), ("M_DEN", 0, -1), ("M_ENG", 0, -1), ("M_PTY", 0, -1), ("E_STE", 0, 4),
