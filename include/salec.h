@@ -195,6 +195,13 @@ typedef struct salec_ctx salec_ctx;
 
 /* step flags */
 #define SALEC_STEP_NOSYNC   0x1u  /* do not wait for / return dt and t      */
+/* Split a step at the reference's output point (src/salec2d.c:241-245,
+ * after the ghost-ring flux, before advect_flux) so that the host can
+ * download and write the fields the reference writes there: a call with
+ * SALEC_STEP_TO_OUTPUT runs the first part, the next call must carry
+ * SALEC_STEP_FROM_OUTPUT and runs the rest (SALEC_ESTATE otherwise). */
+#define SALEC_STEP_TO_OUTPUT   0x2u
+#define SALEC_STEP_FROM_OUTPUT 0x4u
 
 /* Field metadata: grid kind and component count (nmat expanded). */
 int  salec_field_info(int field, int nmat, int *grid, int *ncomp);
@@ -233,6 +240,14 @@ int  salec_get_time(salec_ctx *ctx, double *t, double *dt, long *step);
 
 /* One iteration of the loop body src/salec2d.c:115-272 (output excluded). */
 int  salec_step(salec_ctx *ctx, unsigned flags, salec_stats *stats);
+
+/* Checkpoint and restore: every field, the tracers, the time state and the
+ * damp time, written to / read from one file.  Restore requires a context
+ * created with the same nx, ny, nmat and a tracer capacity at least the
+ * checkpoint's count.  The reference has no checkpointing (state is written
+ * only as VTK output, src/io/output.c); this replaces re-running from t0. */
+int  salec_checkpoint(salec_ctx *ctx, const char *path);
+int  salec_restore(salec_ctx *ctx, const char *path);
 
 /* Block until all work queued on the context has finished. */
 int  salec_sync(salec_ctx *ctx);

@@ -54,8 +54,11 @@ void oracle_detach(oracle_state *s);
  * record_avg_value, damp time, init_v_field, cleaning).  Returns damp time. */
 double oracle_init(oracle_state *s, const salec_config *cfg, double *damp_time_out);
 
-/* One loop iteration, src/salec2d.c:115-272. */
+/* One loop iteration, src/salec2d.c:115-272, and its two halves split at
+ * the reference's output point (src/salec2d.c:241-245). */
 void oracle_step(oracle_state *s, const salec_config *cfg);
+void oracle_step_to_output(oracle_state *s, const salec_config *cfg);
+void oracle_step_from_output(oracle_state *s, const salec_config *cfg);
 
 /* Individual phases, exported for unit tests. */
 void oracle_update_state(oracle_state *s, const salec_config *cfg);

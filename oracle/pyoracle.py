@@ -42,6 +42,8 @@ def olib():
     L.oracle_init.argtypes = [P, CP, C.POINTER(C.c_double)]
     L.oracle_init.restype = C.c_double
     L.oracle_step.argtypes = [P, CP]
+    L.oracle_step_to_output.argtypes = [P, CP]
+    L.oracle_step_from_output.argtypes = [P, CP]
     L.oracle_update_state.argtypes = [P, CP]
     L.oracle_ghost_fill.argtypes = [P, CP, C.c_int, C.c_int]
     L.oracle_restrict.argtypes = [P, CP, C.c_int]
@@ -101,6 +103,12 @@ class Oracle:
 
     def step(self):
         self.L.oracle_step(C.byref(self.s), C.byref(self.cfg))
+
+    def step_to_output(self):
+        self.L.oracle_step_to_output(C.byref(self.s), C.byref(self.cfg))
+
+    def step_from_output(self):
+        self.L.oracle_step_from_output(C.byref(self.s), C.byref(self.cfg))
 
     @property
     def t(self):

@@ -1708,8 +1708,9 @@ double oracle_init(oracle_state *s, const salec_config *cfg, double *damp_time_o
     return dtime;
 }
 
-/* loop body src/salec2d.c:115-272 */
-void oracle_step(oracle_state *s, const salec_config *cfg)
+/* loop body src/salec2d.c:115-272, in two halves split at the reference's
+ * output point (src/salec2d.c:241-245, after the ghost-ring flux) */
+void oracle_step_to_output(oracle_state *s, const salec_config *cfg)
 {
     grid_t g = mkgrid(s);
     trace_profile(s, cfg, g);
@@ -1747,6 +1748,12 @@ void oracle_step(oracle_state *s, const salec_config *cfg)
     oracle_ghost_fill(s, cfg, SALEC_E_VIB, O_GHT_ALWAYS_COPY);
     oracle_ghost_fill(s, cfg, SALEC_M_ENG, O_GHT_DENSITY);
     calculate_flux_mixed(s, g, 0);
+    (void)cell_receive;
+}
+void oracle_step_from_output(oracle_state *s, const salec_config *cfg)
+{
+    grid_t g = mkgrid(s);
+    double dt = s->dt;
     advect_flux_default(s, cfg, g);
     switch (cfg->damp_type) {
     case SALEC_DAMP_PML: damp_on_cell_pml(s, cfg, g, dt, s->t); break;
@@ -1758,5 +1765,9 @@ void oracle_step(oracle_state *s, const salec_config *cfg)
     oracle_update_state(s, cfg);
     s->t += s->dt;
     s->step++;
-    (void)cell_receive;
+}
+void oracle_step(oracle_state *s, const salec_config *cfg)
+{
+    oracle_step_to_output(s, cfg);
+    oracle_step_from_output(s, cfg);
 }

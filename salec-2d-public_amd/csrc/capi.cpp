@@ -73,6 +73,7 @@ struct salec_ctx {
     long tr_cap = 0;
     double *d_vx = nullptr, *d_vy = nullptr;
     int initialized = 0;
+    int half_open = 0;                   /* a step stopped at its output point */
     unsigned prof_mask = 0;              /* phases timed with HIP events   */
     std::vector<hipEvent_t> ev_pool;     /* created on demand, reused      */
     size_t ev_used = 0;
@@ -710,6 +711,9 @@ static hipEvent_t next_event(salec_ctx *c)
 int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
 {
     if (!c) return SALEC_EINVAL;
+    const bool to_out = flags & SALEC_STEP_TO_OUTPUT, from_out = flags & SALEC_STEP_FROM_OUTPUT;
+    if (to_out && from_out) return SALEC_EINVAL;
+    if (from_out != (bool)c->half_open) return SALEC_ESTATE;
     hipStream_t s = c->stream;
     const Grid &g = c->g;
     const Fields &F = c->F;
@@ -729,6 +733,7 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
         if (e) hipEventRecord(e, s);
     };
 #define PHASE(ph, ...) do { T0(ph); __VA_ARGS__; T1(ph); } while (0)
+    if (!from_out) {
     PHASE(PH_PROFILE, launch_trace_profile(s, g, F, P, c->d_prof, c->prof_chunks);
                       launch_profile_finish(s, g, F, P, c->d_prof, c->prof_chunks));
     PHASE(PH_GF_A, run_program(c, c->gfA));
@@ -748,13 +753,15 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, c->d_mats, c->d_time, c->bx, c->by, c->d_top));
     PHASE(PH_GF_C, run_program(c, c->gfC));
     PHASE(PH_FLUX_RING, launch_flux_ring(s, g, F, c->bx, c->by));
-    {
+    }
+    c->half_open = to_out;
+    if (!to_out) {
         Fields Fn = c->F;
         for (int f : kSwapped) Fn.f[f] = c->Falt.f[f];
         PHASE(PH_REMAP_CLOSE, launch_remap_close(s, g, F, Fn, P, c->d_mats, c->bx, c->by, c->d_top, c->d_time));
         for (int f : kSwapped) std::swap(c->F.f[f], c->Falt.f[f]);
+        PHASE(PH_TIME, launch_advance_time(s, c->d_time));
     }
-    PHASE(PH_TIME, launch_advance_time(s, c->d_time));
 #undef PHASE
     HIPCHK(hipGetLastError());
     if (!(flags & SALEC_STEP_NOSYNC) || st) {
@@ -763,6 +770,104 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
         HIPCHK(hipStreamSynchronize(s));
         if (st) { st->dt = h.dt; st->t = h.t; st->step = (long)h.step; }
     }
+    return SALEC_OK;
+}
+
+/* checkpoint file: a fixed header, every field in field-id order (all
+ * components, padded grid), then the tracer arrays */
+struct CkptHeader {
+    char magic[8];
+    int abi, nx, ny, nmat;
+    long ntracers, step;
+    double t, dt, damp_time;
+};
+
+int salec_checkpoint(salec_ctx *c, const char *path)
+{
+    if (!c || !path) return SALEC_EINVAL;
+    if (c->half_open) return SALEC_ESTATE;
+    CkptHeader h{};
+    std::memcpy(h.magic, "SALECCK1", 8);
+    h.abi = SALEC_ABI_VERSION; h.nx = c->g.nx; h.ny = c->g.ny; h.nmat = c->g.nmat;
+    h.ntracers = c->T.n; h.damp_time = c->P.damp_time;
+    int rc = salec_get_time(c, &h.t, &h.dt, &h.step);
+    if (rc) return rc;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return SALEC_EINVAL;
+    bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1;
+    std::vector<double> buf;
+    for (int f = 0; f < SALEC_NFIELDS && ok; ++f) {
+        int grid, nc;
+        salec_field_info(f, c->g.nmat, &grid, &nc);
+        buf.resize((size_t)salec_grid_points(grid, c->g.nx, c->g.ny) * nc);
+        if ((rc = salec_download(c, f, buf.data()))) { std::fclose(fp); return rc; }
+        ok = std::fwrite(buf.data(), sizeof(double), buf.size(), fp) == buf.size();
+    }
+    if (ok && h.ntracers > 0) {
+        long n = h.ntracers;
+        std::vector<double> pos(2 * n), vel(2 * n);
+        std::vector<float> mp(n), mt(n);
+        std::vector<int> iv(7 * (size_t)n);
+        salec_tracers tr{n, pos.data(), vel.data(), mp.data(), mt.data(), iv.data(), iv.data() + n, iv.data() + 2 * n,
+                         iv.data() + 3 * n, iv.data() + 4 * n, iv.data() + 5 * n, iv.data() + 6 * n};
+        if ((rc = salec_download_tracers(c, &tr))) { std::fclose(fp); return rc; }
+        ok = std::fwrite(pos.data(), sizeof(double), pos.size(), fp) == pos.size() &&
+             std::fwrite(vel.data(), sizeof(double), vel.size(), fp) == vel.size() &&
+             std::fwrite(mp.data(), sizeof(float), n, fp) == (size_t)n &&
+             std::fwrite(mt.data(), sizeof(float), n, fp) == (size_t)n &&
+             std::fwrite(iv.data(), sizeof(int), iv.size(), fp) == iv.size();
+    }
+    if (std::fclose(fp) != 0) ok = false;
+    return ok ? SALEC_OK : SALEC_EINVAL;
+}
+
+int salec_restore(salec_ctx *c, const char *path)
+{
+    if (!c || !path) return SALEC_EINVAL;
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return SALEC_EINVAL;
+    CkptHeader h{};
+    int rc = SALEC_OK;
+    if (std::fread(&h, sizeof h, 1, fp) != 1 || std::memcmp(h.magic, "SALECCK1", 8) != 0 ||
+        h.nx != c->g.nx || h.ny != c->g.ny || h.nmat != c->g.nmat || h.ntracers < 0 || h.ntracers > c->tr_cap) {
+        std::fclose(fp);
+        return SALEC_EINVAL;
+    }
+    std::vector<double> buf;
+    for (int f = 0; f < SALEC_NFIELDS && rc == SALEC_OK; ++f) {
+        int grid, nc;
+        salec_field_info(f, c->g.nmat, &grid, &nc);
+        buf.resize((size_t)salec_grid_points(grid, c->g.nx, c->g.ny) * nc);
+        if (std::fread(buf.data(), sizeof(double), buf.size(), fp) != buf.size()) rc = SALEC_EINVAL;
+        else rc = salec_upload(c, f, buf.data());
+    }
+    if (rc == SALEC_OK) {
+        long n = h.ntracers;
+        std::vector<double> pos(2 * n), vel(2 * n);
+        std::vector<float> mp(n), mt(n);
+        std::vector<int> iv(7 * (size_t)n);
+        bool ok = n == 0 || (std::fread(pos.data(), sizeof(double), pos.size(), fp) == pos.size() &&
+                             std::fread(vel.data(), sizeof(double), vel.size(), fp) == vel.size() &&
+                             std::fread(mp.data(), sizeof(float), n, fp) == (size_t)n &&
+                             std::fread(mt.data(), sizeof(float), n, fp) == (size_t)n &&
+                             std::fread(iv.data(), sizeof(int), iv.size(), fp) == iv.size());
+        if (!ok) rc = SALEC_EINVAL;
+        else {
+            salec_tracers tr{n, pos.data(), vel.data(), mp.data(), mt.data(), iv.data(), iv.data() + n,
+                             iv.data() + 2 * n, iv.data() + 3 * n, iv.data() + 4 * n, iv.data() + 5 * n,
+                             iv.data() + 6 * n};
+            rc = salec_upload_tracers(c, &tr);
+        }
+    }
+    std::fclose(fp);
+    if (rc) return rc;
+    if ((rc = salec_set_damp_time(c, h.damp_time))) return rc;
+    if ((rc = salec_set_time(c, h.t, h.dt))) return rc;
+    DevTime d{};
+    d.dt = h.dt; d.t = h.t; d.step = (double)h.step;
+    HIPCHK(hipMemcpyAsync(c->d_time, &d, sizeof d, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->half_open = 0;
     return SALEC_OK;
 }
 

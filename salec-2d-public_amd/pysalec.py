@@ -36,6 +36,7 @@ TRANSIENT = ["E_VEL", "E_GRAD_VEL", "E_DIV_VEL", "E_Q", "E_STA2S", "E_STRENGTH",
              "V_DIS", "Y_PROFILE"]
 
 SALEC_OK = 0
+SALEC_STEP_NOSYNC, SALEC_STEP_TO_OUTPUT, SALEC_STEP_FROM_OUTPUT = 0x1, 0x2, 0x4
 # strength/damage/tensile/porosity model registry (salec.h SALEC_FN_*)
 (SALEC_FN_SIMPLEROCK, SALEC_FN_SIMPLESHEAR, SALEC_FN_JOHNSONCOOK1, SALEC_FN_JOHNSONCOOK2, SALEC_FN_ROCKSTRENGTHT,
  SALEC_FN_ICESTRENGTH, SALEC_FN_COLLINS, SALEC_FN_IVANOV, SALEC_FN_SIMPLETENSILE, SALEC_FN_NONE, SALEC_FN_WUNNEMANN,
@@ -127,6 +128,8 @@ def lib():
     L.salec_host_laplace9.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
     L.salec_crater_metrics.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.salec_checkpoint.argtypes = [vp, C.c_char_p]
+    L.salec_restore.argtypes = [vp, C.c_char_p]
     L.salec_write_tillotson.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int]
     L.salec_write_sd_table.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int,
                                        C.c_double, C.c_double, C.c_double, C.c_double]
@@ -280,10 +283,24 @@ class Device:
         check(lib().salec_get_time(self.ctx, C.byref(t), C.byref(dt), C.byref(st)), "salec_get_time")
         return t.value, dt.value, st.value
 
-    def step(self, sync=True):
+    def step(self, sync=True, flags=0):
         s = Stats()
-        check(lib().salec_step(self.ctx, 0 if sync else 1, C.byref(s) if sync else None), "salec_step")
+        f = flags | (0 if sync else SALEC_STEP_NOSYNC)
+        check(lib().salec_step(self.ctx, f, C.byref(s) if sync else None), "salec_step")
         return s
+
+    def step_to_output(self):
+        """First part of a step, up to the reference's output point."""
+        return self.step(flags=SALEC_STEP_TO_OUTPUT)
+
+    def step_from_output(self):
+        return self.step(flags=SALEC_STEP_FROM_OUTPUT)
+
+    def checkpoint(self, path):
+        check(lib().salec_checkpoint(self.ctx, path.encode()), "salec_checkpoint")
+
+    def restore(self, path):
+        check(lib().salec_restore(self.ctx, path.encode()), "salec_restore")
 
     def sync(self):
         check(lib().salec_sync(self.ctx), "salec_sync")

@@ -250,3 +250,62 @@ def test_full_size_c1_properties():
     assert (a["M_DEN"][:, 2:-2, 2:-2] >= 0).all() and (a["M_ENG"][:, 2:-2, 2:-2] >= 0).all()
     assert np.abs(a["V_VEL"]).max() > 1000.0      # the impact is under way
     say("  max |v| %.1f m/s, max damage %.3f" % (np.abs(a["V_VEL"]).max(), a["E_DAM"].max()))
+
+
+def test_split_step_at_output_point():
+    """A step split at the reference's output point (before advect_flux)
+    matches the oracle at that point and after the rest of the step; calls
+    out of order are refused."""
+    say("split step c1 32x48")
+    h, o, dev, tr = setup_pair("c1", 32, 48)
+    for _ in range(8):
+        o.step()
+    dev.upload_state(o.state)
+    dev.set_time(o.t, o.dt)
+    L = ps.lib()
+    assert L.salec_step(dev.ctx, ps.SALEC_STEP_FROM_OUTPUT, None) == -6
+    for k in range(6):
+        o.step_to_output()
+        s = dev.step_to_output()
+        assert s.dt == o.dt
+        wf, w, mism = P.compare(dev.state(), o.state, h.nmat)
+        assert w <= P.FIELD_RTOL and not any(mism.values()), ("output point", k, wf, w, mism)
+        assert L.salec_step(dev.ctx, 0, None) == -6       # the open step must be finished first
+        o.step_from_output()
+        s = dev.step_from_output()
+        assert (s.t, s.dt) == (o.t, o.dt)
+        wf, w, mism = P.compare(dev.state(), o.state, h.nmat)
+        assert w <= P.FIELD_RTOL and not any(mism.values()), ("end of step", k, wf, w, mism)
+    dev.close()
+
+
+def test_checkpoint_restore(tmp_path):
+    """Stepping on from a restored checkpoint repeats the original run bit
+    for bit, tracers included."""
+    say("checkpoint/restore c1 40x60")
+    h, o, dev, tr = setup_pair("c1", 40, 60, tracers=True)
+    dev.upload_tracers(tr)
+    dev.set_time(h.time0, h.dt0)
+    for _ in range(10):
+        dev.step(sync=False)
+    path = str(tmp_path / "ck.bin")
+    dev.checkpoint(path)
+    for _ in range(10):
+        dev.step(sync=False)
+    a, ta, tma = dev.state(), dev.download_tracers(tr.n), dev.get_time()
+    dev2 = ps.Device(h.cfg, max_tracers=tr.n)
+    dev2.restore(path)
+    for _ in range(10):
+        dev2.step(sync=False)
+    b, tb, tmb = dev2.state(), dev2.download_tracers(tr.n), dev2.get_time()
+    assert tma == tmb
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    for k in ("pos", "vel", "idx", "idy", "alive", "matid", "tag"):
+        assert np.array_equal(getattr(ta, k), getattr(tb, k)), k
+    small = ps.Device(h.cfg, max_tracers=1)
+    with pytest.raises(RuntimeError):
+        small.restore(path)                               # tracer capacity too small
+    small.close()
+    dev.close()
+    dev2.close()
