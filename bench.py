@@ -92,6 +92,63 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
     return 2.0 * nbytes / (ms.value / reps * 1e-3) / 1e9
 
 
+def ensemble_leg(nruns, steps, seed, wd):
+    """The batched-ensemble figure: nruns crater runs drawn with a fixed seed
+    (mesh scale s ~ U[1, 4] of 256 x 384 cells, rounded to multiples of 8;
+    projectile radius U[8, 40] cells; impact speed U[5, 20] km/s; intact
+    cohesion log-U[1, 50] MPa; intact friction U[0.4, 1.2]; materials and
+    tables as the strength workload, no tracers), stepped from contact.
+    Serial: each run alone, one after another.  Batched: all runs advanced
+    together as captured graphs on their own streams (salec_batch_step)."""
+    rng = np.random.default_rng(seed)
+    devs, cells, t0 = [], 0, time.time()
+    for i in range(nruns):
+        sc = rng.uniform(1.0, 4.0)
+        nx, ny = int(round(256 * sc / 8)) * 8, int(round(384 * sc / 8)) * 8
+        rc, vel = rng.uniform(8.0, 40.0), rng.uniform(5.0e3, 2.0e4)
+        coh, fric = 10.0 ** rng.uniform(6.0, np.log10(5.0e7)), rng.uniform(0.4, 1.2)
+        inp, eos = ps.write_config(os.path.join(wd, "ens%02d" % i), "c1", nx=nx, ny=ny, proj_cells=rc, velocity=vel,
+                                   table_n=(256, 256),
+                                   sections_over={"material.yint0": "[, %.6g, %.6g]" % (coh, coh),
+                                                  "material.yintfri": "[, %.6g, %.6g]" % (fric, fric)})
+        h = ps.HostRun(inp, eos)
+        d = ps.Device(h.cfg, max_tracers=1)
+        d.upload_state(h.state())
+        h.cfg.damp_time = d.init(h.damp_t_in)
+        d.set_damp_time(h.cfg.damp_time)
+        d.set_time(h.time0, h.dt0)
+        cells += nx * ny
+        h.close()
+        devs.append(d)
+    log("ensemble: %d runs, %.1fM cells, set up in %.1fs" % (nruns, cells / 1e6, time.time() - t0))
+    for d in devs:                      # warm every run once
+        d.step(sync=False)
+    for d in devs:
+        d.sync()
+    t0 = time.perf_counter()
+    for d in devs:                      # serial: one run after another
+        for _ in range(steps):
+            d.step(sync=False)
+        d.sync()
+    serial = time.perf_counter() - t0
+    batch = ps.Batch(devs)
+    batch.step(1)                       # records the graphs
+    t0 = time.perf_counter()
+    batch.step(steps)
+    batched = time.perf_counter() - t0
+    bad = [i for i, d in enumerate(devs) if d.check_finite()[0] != 0]
+    batch.close()
+    for d in devs:
+        d.close()
+    out = {"runs": nruns, "cells": cells, "steps": steps, "seed": seed,
+           "serial_cell_steps_per_s": round(cells * steps / serial, 1),
+           "batched_cell_steps_per_s": round(cells * steps / batched, 1),
+           "speedup": round(serial / batched, 3), "nonfinite_runs": bad}
+    log("ensemble: serial %.3e, batched %.3e cell-steps/s (x%.2f)" % (out["serial_cell_steps_per_s"],
+                                                                    out["batched_cell_steps_per_s"], out["speedup"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +160,8 @@ def main():
                     help="untimed steps from contact before measuring (developed flow)")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--ensemble", type=int, default=64, help="runs in the batched-ensemble leg (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver keeps the whole mesh on one device "
@@ -206,6 +265,9 @@ def main():
                                                                         tr.n, cel)}
         log("cpu oracle: %d steps in %.1fs" % (args.cpu_steps, cel))
 
+    dev.close()
+    ens = ensemble_leg(args.ensemble, args.ensemble_steps, 0, wd) if args.ensemble > 0 else None
+
     out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -217,9 +279,8 @@ def main():
                       "start_time": t_adv,
                       "cells": cells, "tracers": int(tr.n), "nmat": nmat,
                       "sim_time_after": t_now},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens}
     print(json.dumps(out), flush=True)
-    dev.close()
 
 
 if __name__ == "__main__":

@@ -249,6 +249,18 @@ int  salec_step(salec_ctx *ctx, unsigned flags, salec_stats *stats);
 int  salec_checkpoint(salec_ctx *ctx, const char *path);
 int  salec_restore(salec_ctx *ctx, const char *path);
 
+/* Ensembles (parameter sweeps of independent runs on one GPU).  No
+ * counterpart in the reference, which runs one configuration per MPI job.
+ * A batch holds contexts created and initialised as usual; batch_step
+ * advances every run by nsteps steps, each with its own dt, replaying each
+ * run's step as a captured HIP graph on its own stream, and returns when
+ * all have finished.  The contexts stay usable on their own; changing a
+ * context's damp time, tracers or profiling re-records its graph. */
+typedef struct salec_batch salec_batch;
+int  salec_batch_create(salec_ctx **ctxs, int n, salec_batch **out);
+int  salec_batch_step(salec_batch *b, int nsteps);
+void salec_batch_destroy(salec_batch *b);
+
 /* Block until all work queued on the context has finished. */
 int  salec_sync(salec_ctx *ctx);
 

@@ -74,6 +74,8 @@ struct salec_ctx {
     double *d_vx = nullptr, *d_vy = nullptr;
     int initialized = 0;
     int half_open = 0;                   /* a step stopped at its output point */
+    int parity = 0;                      /* which buffer set of the remapped fields is current */
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};  /* captured steps, by parity */
     unsigned prof_mask = 0;              /* phases timed with HIP events   */
     std::vector<hipEvent_t> ev_pool;     /* created on demand, reused      */
     size_t ev_used = 0;
@@ -82,6 +84,14 @@ struct salec_ctx {
     long phase_calls[PH_COUNT]{};
     unsigned long long *d_nf = nullptr;
 };
+
+/* Captured step graphs bake in kernel arguments (damp time, tracer count,
+ * profiling events): drop them whenever one of those changes. */
+static void drop_graphs(salec_ctx *c)
+{
+    for (hipGraphExec_t &e : c->gexec)
+        if (e) { hipGraphExecDestroy(e); e = nullptr; }
+}
 
 static int hip_ok(hipError_t e) { return e == hipSuccess ? SALEC_OK : SALEC_EDEVICE; }
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { std::fprintf(stderr, "salec: %s failed: %s\n", #x, hipGetErrorString(e_)); return SALEC_EDEVICE; } } while (0)
@@ -559,6 +569,7 @@ void salec_destroy(salec_ctx *c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (hipGraphExec_t &e : c->gexec) if (e) hipGraphExecDestroy(e);
     for (void *p : c->owned) hipFree(p);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -598,6 +609,7 @@ int salec_download(salec_ctx *c, int field, double *host)
 int salec_upload_tracers(salec_ctx *c, const salec_tracers *tr)
 {
     if (!c || !tr) return SALEC_EINVAL;
+    drop_graphs(c);
     if (tr->n > c->tr_cap) return SALEC_EINVAL;
     long n = tr->n;
     Tracers &T = c->T;
@@ -662,6 +674,7 @@ int salec_get_time(salec_ctx *c, double *t, double *dt, long *step)
 int salec_init(salec_ctx *c, double *damp_time_io)
 {
     if (!c) return SALEC_EINVAL;
+    drop_graphs(c);
     hipStream_t s = c->stream;
     launch_update_state(s, c->g, c->F, c->P, c->d_mats);
     run_program(c, c->gfInit);
@@ -692,6 +705,7 @@ int salec_init(salec_ctx *c, double *damp_time_io)
 int salec_set_damp_time(salec_ctx *c, double damp_time)
 {
     if (!c) return SALEC_EINVAL;
+    drop_graphs(c);
     c->P.damp_time = damp_time;
     c->cfg.damp_time = damp_time;
     c->initialized = 1;
@@ -760,6 +774,7 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
         for (int f : kSwapped) Fn.f[f] = c->Falt.f[f];
         PHASE(PH_REMAP_CLOSE, launch_remap_close(s, g, F, Fn, P, c->d_mats, c->bx, c->by, c->d_top, c->d_time));
         for (int f : kSwapped) std::swap(c->F.f[f], c->Falt.f[f]);
+        c->parity ^= 1;
         PHASE(PH_TIME, launch_advance_time(s, c->d_time));
     }
 #undef PHASE
@@ -871,6 +886,73 @@ int salec_restore(salec_ctx *c, const char *path)
     return SALEC_OK;
 }
 
+/* ------------------------------------------------------------ ensembles */
+/* A batch advances independent runs together.  Each context's step is
+ * captured once as a HIP graph per buffer parity (the remapped fields swap
+ * buffers every step) and replayed on the context's own stream, so the
+ * host issues one launch per run and step and the runs' kernels overlap on
+ * the GPU.  Every run keeps its own dt. */
+struct salec_batch {
+    std::vector<salec_ctx *> ctx;
+};
+
+static int capture_graphs(salec_ctx *c)
+{
+    if (c->prof_mask) return SALEC_ESTATE;   /* phase events are not recorded into graphs */
+    if (c->gexec[0] && c->gexec[1]) return SALEC_OK;
+    drop_graphs(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 2; ++k) {
+        const int p = c->parity;
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = salec_step(c, SALEC_STEP_NOSYNC, nullptr);   /* records; swaps the buffer sets */
+        hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (rc != SALEC_OK || e != hipSuccess) { if (gr) hipGraphDestroy(gr); return rc ? rc : SALEC_EDEVICE; }
+        e = hipGraphInstantiate(&c->gexec[p], gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (e != hipSuccess) { c->gexec[p] = nullptr; return SALEC_EDEVICE; }
+    }
+    return SALEC_OK;   /* two recorded steps: parity and buffers are back where they were */
+}
+
+int salec_batch_create(salec_ctx **ctxs, int n, salec_batch **out)
+{
+    if (!ctxs || n <= 0 || !out) return SALEC_EINVAL;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return SALEC_EINVAL;
+        for (int j = 0; j < i; ++j) if (ctxs[j] == ctxs[i]) return SALEC_EINVAL;
+    }
+    salec_batch *b = new salec_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    *out = b;
+    return SALEC_OK;
+}
+
+int salec_batch_step(salec_batch *b, int nsteps)
+{
+    if (!b || nsteps < 0) return SALEC_EINVAL;
+    for (salec_ctx *c : b->ctx) {
+        if (c->half_open) return SALEC_ESTATE;
+        int rc = capture_graphs(c);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < nsteps; ++k)
+        for (salec_ctx *c : b->ctx) {
+            HIPCHK(hipGraphLaunch(c->gexec[c->parity], c->stream));
+            for (int f : kSwapped) std::swap(c->F.f[f], c->Falt.f[f]);
+            c->parity ^= 1;
+        }
+    for (salec_ctx *c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    return SALEC_OK;
+}
+
+void salec_batch_destroy(salec_batch *b)
+{
+    delete b;
+}
+
 int salec_sync(salec_ctx *c)
 {
     if (!c) return SALEC_EINVAL;
@@ -903,6 +985,7 @@ int salec_check(salec_ctx *c, int *field, long *index)
 
 int salec_set_profiling(salec_ctx *c, unsigned mask)
 {
+    if (c) drop_graphs(c);
     if (!c) return SALEC_EINVAL;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->prof_mask = mask;

@@ -128,6 +128,9 @@ def lib():
     L.salec_host_laplace9.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
     L.salec_crater_metrics.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.salec_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.salec_batch_step.argtypes = [vp, C.c_int]
+    L.salec_batch_destroy.argtypes = [vp]
     L.salec_checkpoint.argtypes = [vp, C.c_char_p]
     L.salec_restore.argtypes = [vp, C.c_char_p]
     L.salec_write_tillotson.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int]
@@ -360,6 +363,30 @@ def crater_metrics(y_profile, col_r, nx):
     d, D = C.c_double(), C.c_double()
     lib().salec_crater_metrics(_dp(y), _dp(r), nx, C.byref(d), C.byref(D))
     return d.value, D.value
+
+
+class Batch:
+    """Independent runs advanced together (salec_batch_*)."""
+
+    def __init__(self, devices):
+        self.devices = list(devices)
+        arr = (C.c_void_p * len(self.devices))(*[d.ctx.value for d in self.devices])
+        self.b = C.c_void_p()
+        check(lib().salec_batch_create(arr, len(self.devices), C.byref(self.b)), "salec_batch_create")
+
+    def step(self, nsteps=1):
+        check(lib().salec_batch_step(self.b, int(nsteps)), "salec_batch_step")
+
+    def close(self):
+        if self.b:
+            lib().salec_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # --------------------------------------------------------------- inputs ---

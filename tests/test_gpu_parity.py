@@ -309,3 +309,42 @@ def test_checkpoint_restore(tmp_path):
     small.close()
     dev.close()
     dev2.close()
+
+
+def test_batch_graph_replays_match_eager_steps():
+    """Three different runs advanced as a batch (captured-graph replays on
+    their own streams, per-run dt) match the same runs stepped eagerly, at
+    every replay, with eager steps interleaved between replays."""
+    say("batch of 3 runs vs eager")
+    cases = [("c0", 24, 36, {}), ("c1", 32, 48, {}), ("c1", 40, 56, {"velocity": 8000.0, "proj_cells": 4})]
+    runs = []
+    for which, nx, ny, kw in cases:
+        h = ps.HostRun(*ps.write_config(os.path.join(tempfile.mkdtemp(prefix="gb_"), which), which, nx=nx, ny=ny, **kw))
+        st = h.state()
+        pair = []
+        for _ in range(2):
+            d = ps.Device(h.cfg, max_tracers=1)
+            d.upload_state(st)
+            d.init(h.damp_t_in)
+            d.set_time(h.time0, h.dt0)
+            pair.append(d)
+        runs.append((h, pair[0], pair[1]))
+    batch = ps.Batch([r[1] for r in runs])
+    for rep, n in enumerate([1, 2, 3, 5]):
+        batch.step(n)
+        for _, _, ref in runs:
+            for _ in range(n):
+                ref.step(sync=False)
+        if rep == 1:   # eager work on the batched contexts between replays
+            for _, d, ref in runs:
+                d.step()
+                ref.step()
+        for i, (h, d, ref) in enumerate(runs):
+            assert d.get_time() == ref.get_time(), (rep, i)
+            a, b = d.state(), ref.state()
+            for k in P.COMPARED:
+                assert np.array_equal(a[k], b[k], equal_nan=True), (rep, i, k)
+    batch.close()
+    for _, d, ref in runs:
+        d.close()
+        ref.close()
