@@ -38,6 +38,11 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+# kernel behind each fused phase (names of the rocprof summaries in profiles/)
+PHASE_KERNEL = {"advect_damp_state": "remap_close", "energy_flux": "energy_flux", "calculate_acc": "acc",
+                "vibration_stress": "stress", "gradvel_dt": "gradvel_dt", "vertex_vof_vel": "vertex_vof_vel"}
+
+
 def phase_bytes(n):
     """Algorithmic (compulsory, unique) bytes per unit for each phase: FP64
     values read + written once per unit; vertex data shared by four cells is
@@ -58,6 +63,31 @@ def phase_bytes(n):
         "energy_flux": ("cell", (5 * n + 44) * d),           # update_energy (4n+15) + flux (4n+33) - shared (3n+4)
         "advect_damp_state": ("cell", (7 * n + 31) * d),     # advect (6n+25) + damping sigma (2) + EOS (n+4)
     }
+
+
+def summarize_pmc(fetch_csv, write_csv, kernel):
+    """HBM bytes per launch of `kernel` from rocprofv3 --pmc counter CSVs.
+    FETCH_SIZE and WRITE_SIZE count in units this reads off the 1 GiB
+    device copy that the same bench command runs (copy_bandwidth): bytes per
+    unit = 2^30 / the counter's value for that copy."""
+    import csv
+
+    def load(path):
+        vals, copy = [], []
+        for r in csv.DictReader(open(path)):
+            v = float(r["Counter_Value"])
+            if "copyBuffer" in r["Kernel_Name"]:
+                copy.append(v)
+            elif kernel in r["Kernel_Name"]:
+                vals.append(v)
+        big = [v for v in copy if v > 0.5 * max(copy)]
+        return sum(vals) / len(vals), float(1 << 30) / (sum(big) / len(big)), len(vals)
+
+    f, fcal, nf = load(fetch_csv)
+    w, wcal, nw = load(write_csv)
+    return {"kernel": kernel, "fetch_bytes": f * fcal, "write_bytes": w * wcal,
+            "traffic_bytes": f * fcal + w * wcal, "fetch_bytes_per_unit": fcal, "write_bytes_per_unit": wcal,
+            "launches": [nf, nw]}
 
 
 def copy_bandwidth(nbytes=1 << 30, reps=10):
@@ -162,7 +192,16 @@ def main():
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--ensemble", type=int, default=64, help="runs in the batched-ensemble leg (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--checkpoint-out", default=None, help="save the advanced state here (salec_checkpoint)")
+    ap.add_argument("--checkpoint-in", default=None,
+                    help="start from this checkpoint instead of contact + advance (for profiler runs)")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "KERNEL"), default=None,
+                    help="print the HBM traffic per launch of KERNEL from two rocprofv3 --pmc runs "
+                         "(FETCH_SIZE, WRITE_SIZE) of a bench command, calibrated on its 1 GiB copy")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        print(json.dumps(summarize_pmc(*args.summarize_pmc)))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver keeps the whole mesh on one device "
               "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
@@ -188,14 +227,24 @@ def main():
     dev.set_time(h.time0, h.dt0)
     log("device init + %d tracers in %.1fs" % (tr.n, time.time() - t0))
     t0 = time.time()
-    for k in range(args.advance):
-        dev.step(sync=False)
-        if k % 2000 == 1999:
-            dev.sync()
-            log("  advanced %d steps" % (k + 1))
-    dev.sync()
+    if args.checkpoint_in:
+        dev.restore(args.checkpoint_in)
+        dev.sync()
+        n_adv = int(dev.get_time()[2])
+        log("restored %s (step %d)" % (args.checkpoint_in, n_adv))
+    else:
+        n_adv = args.advance
+        for k in range(args.advance):
+            dev.step(sync=False)
+            if k % 2000 == 1999:
+                dev.sync()
+                log("  advanced %d steps" % (k + 1))
+        dev.sync()
     t_adv, dt_adv, _ = dev.get_time()
-    log("advanced %d steps to t=%.4f s in %.1fs" % (args.advance, t_adv, time.time() - t0))
+    log("advanced %d steps to t=%.4f s in %.1fs" % (n_adv, t_adv, time.time() - t0))
+    if args.checkpoint_out:
+        dev.checkpoint(args.checkpoint_out)
+        log("checkpoint written to %s" % args.checkpoint_out)
     init_state = dev.state() if args.cpu_steps > 0 else None
     init_tr = dev.download_tracers(tr.n) if args.cpu_steps > 0 else None
 
@@ -239,10 +288,16 @@ def main():
     achieved = bpu * units / (avg_ms * 1e-3) / 1e9
     b_cell = 8 * (6 * nmat + 111)
     step_gbs = (b_cell * cells + 104 * tr.n) / (ms_step * 1e-3) / 1e9
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "%s_c1_%dx%d_traffic.json" % (PHASE_KERNEL.get(dom[0], dom[0]), h.nx, h.ny))
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        if tj.get("phase") == dom[0]:
+            traffic, traffic_src = round(tj["traffic_bytes"]), os.path.relpath(tpath, ROOT)
     copy_gbs = copy_bandwidth()
     log("device copy: %.1f GB/s" % copy_gbs)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom[0], "bytes_per_unit": bpu, "units_per_launch": units, "unit_kind": unit,
                 "avg_launch_ms": round(avg_ms, 5),
                 "step_algorithmic_GBs": round(step_gbs, 2), "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
@@ -275,7 +330,7 @@ def main():
            "config": {"workload": "simple crater with strength: %dx%d cells, vacuum+regolith+basalt "
                                   "(256x256 rho-e tables), ROCK+Ivanov, pml2 30 layers, %d tracers, "
                                   "r=20 cells at 12 km/s, timed after %d steps from contact"
-                                  % (h.nx, h.ny, tr.n, args.advance),
+                                  % (h.nx, h.ny, tr.n, n_adv),
                       "start_time": t_adv,
                       "cells": cells, "tracers": int(tr.n), "nmat": nmat,
                       "sim_time_after": t_now},
