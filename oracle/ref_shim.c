@@ -89,3 +89,9 @@ double ref_laplace2d9(const double *xi, const double *yi, double gamma)
 {
     return Laplace2d9(xi, yi, gamma);
 }
+
+/* interpolate_eos_tp (src/eos/eos_state.c:902-921); DataId -1 density, 0 energy density */
+double ref_interpolate_eos_tp(void *p, double tem, double pre, int data_id)
+{
+    return interpolate_eos_tp((eos_table *)p, tem, pre, data_id);
+}

@@ -24,6 +24,9 @@ struct salec_host {
     char name[SALEC_MAXMAT][80];
     double *till_yeng[SALEC_MAXMAT];
     double *tab_yden[SALEC_MAXMAT], *tab_xtem[SALEC_MAXMAT], *tab_data[SALEC_MAXMAT];
+    /* .sd tables: per-cell and per-energy-row ranges of T and P (k = 0, 1),
+     * used to skip cells that cannot hold a state in sd_locate */
+    double *sd_cmin[SALEC_MAXMAT][2], *sd_cmax[SALEC_MAXMAT][2], *sd_rmin[SALEC_MAXMAT][2], *sd_rmax[SALEC_MAXMAT][2];
     /* Tillotson constants used to invert (T,P) -> (rho,e) at setup */
     double inv_p[SALEC_MAXMAT][12];
     double *inv_yeng[SALEC_MAXMAT];
@@ -163,6 +166,200 @@ static double till_interp_tp(const double *t, const double *yeng, int nden, doub
     return which == -1 ? LM : LE * LM;
 }
 
+
+/* ------------------------------------------------- SieDen table inverse */
+/* vec_cmp, src/geometry/linear.c:175-186: how many of x[0..n) exceed y */
+static int n_above(const double *x, double y, int n)
+{
+    int c = 0;
+    for (int k = 0; k < n; ++k) c += x[k] > y;
+    return c;
+}
+
+/* abilinear, src/geometry/linear.c:220-298.  Local coordinates xl in
+ * [-1, 1]^2 of the point (x[4], y[4]) in the bilinear quad whose corners
+ * (x[0..3], y[0..3]) run (-1,-1), (1,-1), (1,1), (-1,1): the bilinear map
+ * is solved as a quadratic in xl[1]; roots outside the quad are dropped
+ * (zeroed) and of two roots the one with the larger |xl0| + |xl1| is
+ * returned.  Returns the number of roots kept.  The arithmetic follows the
+ * reference's operation order so that the roots agree to the last bit. */
+static int quad_inverse(const double *x, const double *y, double *xl)
+{
+    static const double B[4][5] = {{1.0, -1.0, 1.0, -1.0, 0.0}, {-1.0, 1.0, 1.0, -1.0, 0.0},
+                                   {-1.0, -1.0, 1.0, 1.0, 0.0}, {1.0, 1.0, 1.0, 1.0, -4.0}};
+    double a[4] = {0., 0., 0., 0.}, b[4] = {0., 0., 0., 0.};
+    for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 5; ++j) { a[k] += B[k][j] * x[j]; b[k] += B[k][j] * y[j]; }
+    double qa = -a[0] * b[2] + a[2] * b[0];
+    double qb = -a[0] * b[3] + a[3] * b[0] - a[1] * b[2] + a[2] * b[1];
+    double qc = -a[1] * b[3] + a[3] * b[1];
+    double disc = qb * qb - 4.0 * qa * qc;
+    double r[4] = {0., 0., 0., 0.};
+    int kept = 0;
+    if (disc < 0.) {
+        kept = 0;
+    } else if (fabs(disc) < 1e-24) {
+        r[1] = -qb / (2. * qa);
+        r[0] = -1.0 * (a[2] * r[1] + a[3]) / (a[0] * r[1] + a[1]);
+        kept = 1;
+        if (fabs(r[0]) > 1.0 || fabs(r[1]) > 1.0) { r[0] = 0.0; r[1] = 0.0; kept--; }
+    } else {
+        r[1] = (-qb + sqrt(disc)) / (2. * qa);
+        r[0] = -1.0 * (a[2] * r[1] + a[3]) / (a[0] * r[1] + a[1]);
+        r[3] = (-qb - sqrt(disc)) / (2. * qa);
+        r[2] = -1.0 * (a[2] * r[3] + a[3]) / (a[0] * r[3] + a[1]);
+        kept = 2;
+        if (fabs(r[0]) > 1.0 || fabs(r[1]) > 1.0) { r[0] = 0.0; r[1] = 0.0; kept--; }
+        if (fabs(r[2]) > 1.0 || fabs(r[3]) > 1.0) { r[2] = 0.0; r[3] = 0.0; kept--; }
+    }
+    if ((fabs(r[2]) + fabs(r[3])) > (fabs(r[1]) + fabs(r[0]))) { xl[0] = r[2]; xl[1] = r[3]; }
+    else { xl[0] = r[0]; xl[1] = r[1]; }
+    return kept;
+}
+
+#define SDT(M, i, j, k) ((M)->tab_data[((size_t)(i) * (M)->tab_nden + (j)) * 3 + (k)])
+/* The table cell holding the state (T, P) and its local coordinates in
+ * [-1, 1]^2: the first cell, energy rows outer and density inner, whose
+ * corner temperatures and pressures both straddle the target and whose
+ * bilinear inverse has a root (the search of ANEOSSieDenInterpolateTP and
+ * ANEOSSieDenInitStateRef, src/eos/aneos_sieden.c:136-262).  Returns 0 if
+ * no cell holds the state. */
+typedef struct sd_ranges { const double *cmin[2], *cmax[2], *rmin[2], *rmax[2]; } sd_ranges;
+static int sd_locate(const salec_material *M, const sd_ranges *R, double T, double P, int *ti, int *di, double xl[2])
+{
+    xl[0] = xl[1] = 0.;
+    const double tgt[2] = {T, P};
+    const int nc = M->tab_nden - 1;
+    for (int i = 0; i < M->tab_ntem - 1; ++i) {
+        /* a cell passes the corner test iff max > target >= min over its
+         * corners (finite tables); a row without such a range has none */
+        if (R && !(R->rmax[0][i] > T && R->rmin[0][i] <= T && R->rmax[1][i] > P && R->rmin[1][i] <= P)) continue;
+        for (int j = 0; j < M->tab_nden - 1; ++j) {
+            if (R) {
+                size_t q = (size_t)i * nc + j;
+                int skip = 0;
+                for (int k = 0; k < 2; ++k) skip |= !(R->cmax[k][q] > tgt[k] && R->cmin[k][q] <= tgt[k]);
+                if (skip) continue;
+            }
+            double pc[5] = {SDT(M, i, j, 1), SDT(M, i + 1, j, 1), SDT(M, i + 1, j + 1, 1), SDT(M, i, j + 1, 1), P};
+            double tc[5] = {SDT(M, i, j, 0), SDT(M, i + 1, j, 0), SDT(M, i + 1, j + 1, 0), SDT(M, i, j + 1, 0), T};
+            int np = n_above(pc, P, 4), nt = n_above(tc, T, 4);
+            if (np <= 0 || np >= 4 || nt <= 0 || nt >= 4) continue;
+            if (quad_inverse(tc, pc, xl) != 0) { *ti = i; *di = j; return 1; }
+        }
+    }
+    return 0;
+}
+
+/* the ranges of sd_locate for a finite table; NULL (plain scan) otherwise */
+static int sd_build_ranges(salec_host *h, int m)
+{
+    const salec_material *M = &h->mat[m];
+    int ne = M->tab_ntem - 1, nd = M->tab_nden - 1;
+    for (size_t q = 0; q < (size_t)M->tab_ntem * M->tab_nden * 3; ++q)
+        if (!isfinite(M->tab_data[q])) return SALEC_OK;
+    for (int k = 0; k < 2; ++k) {
+        h->sd_cmin[m][k] = (double *)malloc(sizeof(double) * ne * nd);
+        h->sd_cmax[m][k] = (double *)malloc(sizeof(double) * ne * nd);
+        h->sd_rmin[m][k] = (double *)malloc(sizeof(double) * ne);
+        h->sd_rmax[m][k] = (double *)malloc(sizeof(double) * ne);
+        if (!h->sd_cmin[m][k] || !h->sd_cmax[m][k] || !h->sd_rmin[m][k] || !h->sd_rmax[m][k]) return SALEC_ENOMEM;
+        for (int i = 0; i < ne; ++i) {
+            double rlo = INFINITY, rhi = -INFINITY;
+            for (int j = 0; j < nd; ++j) {
+                double v[4] = {SDT(M, i, j, k), SDT(M, i + 1, j, k), SDT(M, i + 1, j + 1, k), SDT(M, i, j + 1, k)};
+                double lo = v[0], hi = v[0];
+                for (int c = 1; c < 4; ++c) { lo = v[c] < lo ? v[c] : lo; hi = v[c] > hi ? v[c] : hi; }
+                h->sd_cmin[m][k][(size_t)i * nd + j] = lo;
+                h->sd_cmax[m][k][(size_t)i * nd + j] = hi;
+                rlo = lo < rlo ? lo : rlo;
+                rhi = hi > rhi ? hi : rhi;
+            }
+            h->sd_rmin[m][k][i] = rlo;
+            h->sd_rmax[m][k][i] = rhi;
+        }
+    }
+    return SALEC_OK;
+}
+static const sd_ranges *sd_get_ranges(const salec_host *h, int m, sd_ranges *R)
+{
+    if (!h || !h->sd_cmin[m][0]) return NULL;
+    for (int k = 0; k < 2; ++k) {
+        R->cmin[k] = h->sd_cmin[m][k]; R->cmax[k] = h->sd_cmax[m][k];
+        R->rmin[k] = h->sd_rmin[m][k]; R->rmax[k] = h->sd_rmax[m][k];
+    }
+    return R;
+}
+
+/* ANEOSSieDenInterpolateTP, src/eos/aneos_sieden.c:136-198: density and
+ * energy density of the state (T, P). */
+static int sd_interp_tp(const salec_material *M, const sd_ranges *R, double T, double P, double *den, double *eng)
+{
+    int ti, di;
+    double xl[2];
+    if (!sd_locate(M, R, T, P, &ti, &di, xl)) return SALEC_EINVAL;
+    xl[0] = (xl[0] + 1.0) / 2.0;
+    xl[1] = (xl[1] + 1.0) / 2.0;
+    double e = M->tab_xtem[ti] * (1.0 - xl[0]) + M->tab_xtem[ti + 1] * xl[0];
+    double d = M->tab_yden[di] * (1.0 - xl[1]) + M->tab_yden[di + 1] * xl[1];
+    *den = d;
+    *eng = e * d;
+    return SALEC_OK;
+}
+
+/* ANEOSSieDenInitStateRef, src/eos/aneos_sieden.c:200-262: the reference
+ * density at (Pnorm, Tnorm) and the state_reference values derived from
+ * the table around it.  The sound speed keeps the reference's two-term sum
+ * (a stray ';' at :244 drops the other two corners). */
+static int sd_state_ref(salec_material *M, double pnorm, double tnorm)
+{
+    int ti, di;
+    double xl[2];
+    if (!sd_locate(M, NULL, tnorm, pnorm, &ti, &di, xl)) return SALEC_EINVAL;
+    int tj = ti + 1, dj = di + 1;
+    double dnorm = M->tab_yden[di] * (1.0 - xl[1]) / 2.0 + M->tab_yden[dj] * (1.0 + xl[1]) / 2.0;
+    double b0 = 0.25 * (1 - xl[0]) * (1 - xl[1]), b1 = 0.25 * (1 + xl[0]) * (1 - xl[1]);
+    double cs = SDT(M, ti, di, 2) * b0 + SDT(M, ti + 1, di, 2) * b1;
+    /* d/dxl0 of the corner functions, corners (ti,di), (tj,di), (tj,dj), (ti,dj) */
+    double g0 = -0.25 * (1 - xl[1]), g1 = 0.25 * (1 - xl[1]), g2 = 0.25 * (1 + xl[1]), g3 = -0.25 * (1 + xl[1]);
+    double dpde = g0 * SDT(M, ti, di, 1) + g1 * SDT(M, tj, di, 1) + g2 * SDT(M, tj, dj, 1) + g3 * SDT(M, ti, dj, 1);
+    double dtde = g0 * SDT(M, ti, di, 0) + g1 * SDT(M, tj, di, 0) + g2 * SDT(M, tj, dj, 0) + g3 * SDT(M, ti, dj, 0);
+    dpde *= 2.0 / (M->tab_xtem[tj] - M->tab_xtem[ti]);
+    dtde *= 2.0 / (M->tab_xtem[tj] - M->tab_xtem[ti]);
+    M->MeltDen = 0.85 * dnorm;
+    M->VaporDen = 0.04 * dnorm;
+    M->den0 = dnorm;
+    double thermexp = (dpde / dtde) / (cs * cs * dnorm);
+    double c_heat = 1.0 / dtde;
+    M->gamfrac = thermexp / c_heat;
+    return SALEC_OK;
+}
+
+/* interpolate_eos_tp (src/eos/eos_state.c:902-921) for the profile and the
+ * projectile: density and energy density of the state (T, P). */
+static int eos_tp(const salec_host *h, int m, double T, double P, double *den, double *eng)
+{
+    sd_ranges R;
+    if (h->mat[m].eos_type == SALEC_EOS_ANEOS_SD) return sd_interp_tp(&h->mat[m], sd_get_ranges(h, m, &R), T, P, den, eng);
+    *eng = till_interp_tp(h->inv_p[m], h->inv_yeng[m], h->inv_nden[m], T, P, 0);
+    *den = till_interp_tp(h->inv_p[m], h->inv_yeng[m], h->inv_nden[m], T, P, -1);
+    return SALEC_OK;
+}
+
+int salec_host_eos_tp(const salec_material *M, double T, double P, double *den, double *eng)
+{
+    if (!M || !den || !eng) return SALEC_EINVAL;
+    if (M->eos_type == SALEC_EOS_ANEOS_SD) return sd_interp_tp(M, NULL, T, P, den, eng);
+    if (M->eos_type != SALEC_EOS_TILLOTSON) return SALEC_EINVAL;
+    double p[12];
+    p[RHO0] = M->TLRho0; p[CV] = M->TLCv; p[TA] = M->TLA; p[TB] = M->TLB; p[E0] = M->TLE0; p[La] = M->TLa;
+    p[Lb] = M->TLb; p[ALPHA] = M->TLAlpha; p[BETA_T] = M->TLBeta; p[EIV] = M->TLEiv; p[ECV] = M->TLEcv;
+    p[TREF] = M->TLTref;
+    *eng = till_interp_tp(p, M->till_yeng, M->till_nden, T, P, 0);
+    *den = till_interp_tp(p, M->till_yeng, M->till_nden, T, P, -1);
+    return SALEC_OK;
+}
+
 /* ------------------------------------------------------------ EOS files */
 static int load_tillotson(salec_host *h, int m, const char *path)
 {
@@ -203,24 +400,16 @@ static int load_tillotson(salec_host *h, int m, const char *path)
     return SALEC_OK;
 }
 
-/* LoadANEOSSieDen layout, src/eos/aneos_sieden.c:11-73; header line 2 of
- * tables written by salec_write_sd_table carries the Tillotson constants. */
+/* LoadANEOSSieDen layout, src/eos/aneos_sieden.c:11-73: three header lines
+ * (skipped), sizes, Pnorm Tnorm, the energy and density axes, the (T, P, c)
+ * data, then entropy/strength pairs that the solver does not use. */
 static int load_sd_table(salec_host *h, int m, const char *path)
 {
     FILE *fp = fopen(path, "r");
     if (!fp) return SALEC_EINVAL;
     char line[4096];
-    int have_inv = 0;
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < 3; ++l)
         if (!fgets(line, sizeof line, fp)) { fclose(fp); return SALEC_EINVAL; }
-        double *p = h->inv_p[m];
-        int rs = 0;
-        if (sscanf(line, "# tillotson %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %d",
-                   p + 0, p + 1, p + 2, p + 3, p + 4, p + 5, p + 6, p + 7, p + 8, p + 9, p + 10, p + 11, &rs) == 13) {
-            have_inv = 1;
-            h->inv_nden[m] = rs;
-        }
-    }
     int nden, nene;
     double pn, tn;
     if (fscanf(fp, "%d %d", &nden, &nene) != 2 || fscanf(fp, "%le %le", &pn, &tn) != 2) { fclose(fp); return SALEC_EINVAL; }
@@ -242,17 +431,8 @@ static int load_sd_table(salec_host *h, int m, const char *path)
     M->tab_nden = nden; M->tab_ntem = nene;
     h->tab_yden[m] = yd; h->tab_xtem[m] = xt; h->tab_data[m] = dd;
     M->tab_yden = yd; M->tab_xtem = xt; M->tab_data = dd;
-    if (!have_inv) return SALEC_EINVAL;  /* setup inversion needs Tillotson constants */
-    h->inv_yeng[m] = till_cold_energy(h->inv_p[m], h->inv_nden[m]);
-    if (!h->inv_yeng[m]) return SALEC_ENOMEM;
-    /* Dnorm-based thresholds of ANEOSSieDenInitStateRef (src/eos/aneos_sieden.c:239-240);
-     * Dnorm is the reference density of the Tillotson constants the table was filled from. */
-    double Dn = h->inv_p[m][RHO0];
-    M->MeltDen = 0.85 * Dn;
-    M->VaporDen = 0.04 * Dn;
-    M->den0 = Dn;
-    M->gamfrac = (h->inv_p[m][Lb] + h->inv_p[m][La]) / (h->inv_p[m][TA] / Dn);
-    return SALEC_OK;
+    int rc = sd_state_ref(M, pn, tn);
+    return rc ? rc : sd_build_ranges(h, m);
 }
 
 int salec_write_tillotson(const char *path, const double p[12], int rhostep)
@@ -539,6 +719,9 @@ void salec_host_free(salec_host *h)
         if (h->inv_yeng[m] != h->till_yeng[m]) free(h->till_yeng[m]);
         free(h->inv_yeng[m]);
         free(h->tab_yden[m]); free(h->tab_xtem[m]); free(h->tab_data[m]);
+        for (int k = 0; k < 2; ++k) {
+            free(h->sd_cmin[m][k]); free(h->sd_cmax[m][k]); free(h->sd_rmin[m][k]); free(h->sd_rmax[m][k]);
+        }
     }
     for (int k = 0; k < SALEC_NFIELDS; ++k) free(h->f[k]);
     free(h);
@@ -952,25 +1135,28 @@ static int build_profile(salec_host *h, tprof *P)
     double gz = h->gravity[1];
     /* IntegrateGravity + CorrectionDensity1, iterated to the fixed point */
     double *prev = (double *)malloc(sizeof(double) * n);
+    int fail_any = 0;
     for (int it = 0; it < 2000; ++it) {
         for (int k = n - 2; k >= 0; --k) {
             double drg = 0.25 * (P->den[k + 1] + P->den[k]) * (gz + gz);
             P->pre[k] = P->pre[k + 1] + drg * (P->rad[k] - P->rad[k + 1]);
         }
         memcpy(prev, P->den, sizeof(double) * n);
+        int fail = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : fail)
         for (int k = 0; k < n; ++k) {
             const salec_material *r = &h->mat[P->mat[k]];
             double mt = r->Tmelt0 * pow(r->Asimon * P->pre[k] + 1.0, r->Csimon);
             if (r->PolySol > 0) { double p = P->pre[k] * 1.0e-9; mt = r->Tsolc1 + r->Tsolc2 * p + r->Tsolc3 * p * p; }
             double lt = Min(P->tem[k], mt);
             P->tem[k] = lt;
-            int m = P->mat[k];
-            P->eng[k] = till_interp_tp(h->inv_p[m], h->inv_yeng[m], h->inv_nden[m], lt, P->pre[k], 0);
-            P->den[k] = till_interp_tp(h->inv_p[m], h->inv_yeng[m], h->inv_nden[m], lt, P->pre[k], -1);
+            if (eos_tp(h, P->mat[k], lt, P->pre[k], &P->den[k], &P->eng[k]) != SALEC_OK) fail = 1;
         }
+        if (fail) { fail_any = 1; break; }
         if (memcmp(prev, P->den, sizeof(double) * n) == 0) break;
     }
     free(prev);
+    if (fail_any) return SALEC_EINVAL;   /* a profile state lies outside an EOS table */
     for (int k = n - 2; k >= 0; --k) {
         double drg = 0.25 * (P->den[k + 1] + P->den[k]) * (gz + gz);
         P->pre[k] = P->pre[k + 1] + drg * (P->rad[k] - P->rad[k + 1]);
@@ -989,6 +1175,10 @@ int salec_host_setup(salec_host *h)
     int nmat = h->cfg.nmat;
     int CW = h->nx + 4, CH = h->ny + 4, VW = h->nx + 5, VH = h->ny + 5;
     long NC = (long)CW * CH, NV = (long)VW * VH;
+    /* projectile state: the same in every projectile cell */
+    const double p_pty = h->mat[h->p_mat].alpha0;
+    double p_den = 0., p_eng = 0.;
+    if (eos_tp(h, h->p_mat, h->p_tem, h->p_pre * p_pty, &p_den, &p_eng) != SALEC_OK) return SALEC_EINVAL;
     /* init_material_plane_distribution, src/target_profile.c:588-713 */
     for (int j = 0; j < CH; ++j)
         for (int k = 0; k < CW; ++k) {
@@ -1010,9 +1200,9 @@ int salec_host_setup(salec_host *h)
                 int pm = h->p_mat;
                 h->f[SALEC_M_VOF][pm * NC + c] = 1.0;
                 h->f[SALEC_E_PRE][c] = h->p_pre; h->f[SALEC_E_TEM][c] = h->p_tem; h->f[SALEC_E_DAM][c] = h->p_dam;
-                double pty = h->mat[pm].alpha0;
-                h->f[SALEC_M_DEN][pm * NC + c] = till_interp_tp(h->inv_p[pm], h->inv_yeng[pm], h->inv_nden[pm], h->p_tem, h->p_pre * pty, -1) / pty;
-                h->f[SALEC_M_ENG][pm * NC + c] = till_interp_tp(h->inv_p[pm], h->inv_yeng[pm], h->inv_nden[pm], h->p_tem, h->p_pre * pty, 0) / pty;
+                double pty = p_pty;
+                h->f[SALEC_M_DEN][pm * NC + c] = p_den / pty;
+                h->f[SALEC_M_ENG][pm * NC + c] = p_eng / pty;
                 h->f[SALEC_E_VEL][c] = h->p_vel[0]; h->f[SALEC_E_VEL][NC + c] = h->p_vel[1];
                 for (int q = 0; q < 4; ++q) wpt[q * NC + c] = -0.25;
                 h->f[SALEC_M_PTY][pm * NC + c] = pty;

@@ -59,6 +59,13 @@ int  salec_write_sd_table(const char *path, const double p[12], int rhostep,
  * src/construct.c:585-594, uses gamma = 0.5). */
 void salec_host_laplace9(const double *xi, double gamma, double *w);
 
+/* interpolate_eos_tp (src/eos/eos_state.c:902-921) for the density (den)
+ * and energy density (eng) of the state (T, P): TillEOSInterpolateTP for
+ * Tillotson materials, ANEOSSieDenInterpolateTP (a search of the table's
+ * cells for the one holding (T, P), then its bilinear inverse) for .sd
+ * tables.  SALEC_EINVAL if no table cell holds the state. */
+int salec_host_eos_tp(const salec_material *m, double T, double P, double *den, double *eng);
+
 /* Transient crater metrics from the crater profile y_profile (trace_profile,
  * src/ejecta.c:8-63) of one step: depth = -min over the physical columns of
  * the profile; diameter = twice the radius at which the profile, walking

@@ -125,6 +125,8 @@ def lib():
     L.salec_host_count_tracers.restype = C.c_long
     L.salec_host_init_tracers.argtypes = [vp, C.POINTER(Tracers)]
     L.salec_host_set_damp_time.argtypes = [vp, C.c_double]
+    L.salec_host_eos_tp.argtypes = [C.POINTER(Material), C.c_double, C.c_double, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
     L.salec_host_laplace9.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
     L.salec_crater_metrics.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -346,6 +348,14 @@ class Device:
             self.close()
         except Exception:
             pass
+
+
+def eos_tp(mat, tem, pre):
+    """(density, energy density) of the state (T, P) for one material
+    (salec_host_eos_tp); raises if no table cell holds the state."""
+    d, e = C.c_double(), C.c_double()
+    check(lib().salec_host_eos_tp(C.byref(mat), tem, pre, C.byref(d), C.byref(e)), "salec_host_eos_tp")
+    return d.value, e.value
 
 
 def laplace9(xi, gamma=0.5):

@@ -7,7 +7,9 @@ The reference holds no golden vectors of its own.  These come from:
     src/io/InputParser.c) compiled unchanged by `make -C oracle ref`:
       eos_ref.npz      calculate_state / calculate_state_correct on seeded
                        (rho, rho*e) samples of the three test EOS files,
-                       state_reference values and the Tillotson cold table;
+                       interpolate_eos_tp on seeded (T, P) samples (states
+                       inside each table's coverage), state_reference
+                       values and the Tillotson cold table;
       laplace_ref.npz  Laplace2d9 weights (gamma 0.5) of seeded stencils;
       bisearch_ref.npz bisearch on seeded sorted lists;
   * the oracle itself, as a regression record of the composed step:
@@ -50,6 +52,7 @@ def eos_samples(seed, n=4000):
 
 def write_eos_ref(wd):
     out = {}
+    rng_tp = np.random.default_rng(31)
     for ci, (which, fn, ty, m) in enumerate(EOS_CASES):
         inp, eos = ps.write_config(os.path.join(wd, which), which, **SMALL)
         E = refeos.RefEOS(os.path.join(eos, fn), ty)
@@ -60,6 +63,10 @@ def write_eos_ref(wd):
         k = "%s_%s" % (which, fn.split(".")[0])
         out[k + "_den"], out[k + "_eng"], out[k + "_plim"] = den, eng, plim
         out[k + "_state"], out[k + "_state_correct"] = res, resc
+        T = rng_tp.uniform(200.0, 2000.0, 400)
+        P = 10.0 ** rng_tp.uniform(3.0, 10.0, 400)
+        tp = np.array([E.eos_tp(a, b) for a, b in zip(T, P)])
+        out[k + "_tp_T"], out[k + "_tp_P"], out[k + "_tp"] = T, P, tp
         sr = E.state_reference()
         out[k + "_stateref"] = np.array([sr[n] for n in sorted(sr)])
         out[k + "_stateref_names"] = np.array(sorted(sr))

@@ -41,6 +41,8 @@ def rlib():
     R.ref_sd_dnorm.restype = d
     R.ref_bisearch.argtypes = [d, dp, C.c_int]
     R.ref_bisearch.restype = C.c_int
+    R.ref_interpolate_eos_tp.argtypes = [vp, d, d, C.c_int]
+    R.ref_interpolate_eos_tp.restype = d
     R.ref_laplace2d9.argtypes = [dp, dp, d]
     R.ref_laplace2d9.restype = d
     _R = R
@@ -81,6 +83,10 @@ class RefEOS:
         p, c = C.c_double(), C.c_double()
         self.R.ref_till_pres(self.p, e, rho, C.byref(p), C.byref(c))
         return p.value, c.value
+
+    def eos_tp(self, tem, pre):
+        """(density, energy density) of the state (T, P), interpolate_eos_tp."""
+        return (self.R.ref_interpolate_eos_tp(self.p, tem, pre, -1), self.R.ref_interpolate_eos_tp(self.p, tem, pre, 0))
 
     def till_temp(self, e, rho):
         return self.R.ref_till_temp(self.p, e, rho)

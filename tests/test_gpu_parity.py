@@ -154,6 +154,12 @@ VARIANTS = {
     "damp_off": ("c0", dict(damping="off")),
     "outflow_right_freeslip_bottom": ("c0", dict(sections_over={"boundary.right": "outflow",
                                                                "boundary.bottom": "freeslip"})),
+    # origin inside the mesh (Ox = 0.5), so the left boundary is not forced to the axis mirror
+    "origin_inside_outflow_left_fixed_top": ("c1", dict(sections_over={"mesh.Ox": "0.5", "boundary.left": "outflow",
+                                                                "boundary.top": "fixed", "boundary.bottom": "outflow"})),
+    "origin_inside_fixed_left_outflow_right": ("c0", dict(sections_over={"mesh.Ox": "0.5", "boundary.left": "fixed",
+                                                                  "boundary.right": "outflow",
+                                                                  "boundary.top": "freeslip"})),
     # SimpleRock is registry entry 0, which str2func refuses (src/strength.c:590)
     "simpleshear_damage": ("c1", dict(sections_over={"material.damage": "[, SimpleShear, Ivanov]"})),
     "johnsoncook2_collins": ("c1", dict(sections_over={"material.yshear": "[, JohnsonCook2, JohnsonCook1]",
@@ -348,3 +354,31 @@ def test_batch_graph_replays_match_eager_steps():
     for _, d, ref in runs:
         d.close()
         ref.close()
+
+
+@pytest.mark.parametrize("which,nx,ny,steps", [("c0", 48, 72, 2000), ("c1", 48, 72, 1200)])
+def test_crater_metrics_end_of_run(which, nx, ny, steps):
+    """End-of-run transient crater: depth (minus the deepest point of the
+    crater profile) and diameter (twice the radius where the profile
+    returns to z = 0), both at the time of maximum depth, within 0.5 % of
+    the oracle's after the full run (lockstep, no reseeding)."""
+    say("crater metrics %s %dx%d, %d steps" % (which, nx, ny, steps))
+    h, o, dev, tr = setup_pair(which, nx, ny)
+    dev.upload_state(o.state)
+    dev.set_time(o.t, o.dt)
+    col_r = h.field("E_POS")[0, 2].copy()
+    best = {"gpu": (-np.inf, 0.0), "oracle": (-np.inf, 0.0)}
+    for k in range(steps):
+        o.step()
+        dev.step(sync=False)
+        if k % 20 == 19 or k == steps - 1:
+            for key, yp in (("gpu", dev.download("Y_PROFILE")[0, 0]), ("oracle", o.state["Y_PROFILE"][0, 0])):
+                d, D = ps.crater_metrics(yp, col_r, nx)
+                if d > best[key][0]:
+                    best[key] = (d, D)
+    (dg, Dg), (do, Do) = best["gpu"], best["oracle"]
+    say("  depth %.2f / %.2f m, diameter %.2f / %.2f m" % (dg, do, Dg, Do))
+    assert do > 2 * 10.0 and Do > 0.0           # a crater formed
+    assert abs(dg - do) <= 0.005 * abs(do)
+    assert abs(Dg - Do) <= 0.005 * abs(Do)
+    dev.close()

@@ -210,3 +210,69 @@ def test_create_validates_before_touching_a_device(tmp_path):
         have_gpu = False
     if not have_gpu:
         assert L.salec_create(C.byref(h.cfg), C.byref(ctx)) == -3
+
+
+EOS_FILES = [("c0", "granite.tillotson", 1), ("c1", "regolith.aneos.sd", 1), ("c1", "basalt.aneos.sd", 2)]
+
+
+@pytest.fixture(scope="module")
+def eos_runs(tmp_path_factory):
+    d = tmp_path_factory.mktemp("eostp")
+    return {w: ps.HostRun(*ps.write_config(str(d / w), w, nx=24, ny=36)) for w in ("c0", "c1")}
+
+
+@pytest.mark.parametrize("case", EOS_FILES, ids=[c[1] for c in EOS_FILES])
+def test_eos_tp_inversion_golden(eos_runs, case):
+    """The host's (T, P) -> (rho, rho e) inversion used for the initial
+    profile and the projectile (TillEOSInterpolateTP, and the table search of
+    ANEOSSieDenInterpolateTP) against the reference's own EOS code:
+    bit-exact wherever a table cell holds the state.  Outside a table's
+    coverage the reference reads past its arrays; the host reports an
+    error instead."""
+    which, fn, m = case
+    g = np.load(os.path.join(GOLD, "eos_ref.npz"))
+    k = "%s_%s" % (which, fn.split(".")[0])
+    mat = eos_runs[which].cfg.mat[m]
+    checked = 0
+    for T, Pr, ref in zip(g[k + "_tp_T"], g[k + "_tp_P"], g[k + "_tp"]):
+        try:
+            got = ps.eos_tp(mat, T, Pr)
+        except RuntimeError:
+            assert fn.endswith(".sd")
+            continue
+        assert np.array_equal(got, ref, equal_nan=True), (T, Pr, got, ref)
+        checked += 1
+    assert checked > 300
+
+
+@pytest.mark.parametrize("case", EOS_FILES, ids=[c[1] for c in EOS_FILES])
+def test_state_reference_golden(eos_runs, case):
+    """Reference density, melt/vapour thresholds and gamma fraction as the
+    reference's loaders set them (TillInitStateRef, ANEOSSieDenInitStateRef
+    including its bilinear inversion at (Pnorm, Tnorm))."""
+    which, fn, m = case
+    g = np.load(os.path.join(GOLD, "eos_ref.npz"))
+    k = "%s_%s" % (which, fn.split(".")[0])
+    ref = dict(zip(g[k + "_stateref_names"], g[k + "_stateref"]))
+    mat = eos_runs[which].cfg.mat[m]
+    for name in ("den0", "MeltDen", "VaporDen", "gamfrac"):
+        assert getattr(mat, name) == ref[name], name
+
+
+@pytest.mark.parametrize("which", ["c0", "c1"])
+def test_target_profile_states_lie_on_the_eos(eos_runs, which):
+    """Every target cell's (m_den, m_eng) is the EOS state of its stored
+    temperature and lithostatic pressure (the fixed point of
+    IntegrateGravity / CorrectDensity, src/target_profile.c:1177-1187)."""
+    h = eos_runs[which]
+    T, Pr, D, E = h.field("E_TEM")[0], h.field("E_PRE")[0], h.field("M_DEN"), h.field("M_ENG")
+    vof, ey = h.field("M_VOF"), h.field("E_POS")[1]
+    n = 0
+    for j in range(2, vof.shape[1] - 2, 3):
+        for k in range(2, vof.shape[2] - 2, 3):
+            m = int(np.argmax(vof[:, j, k]))
+            if m == 0 or ey[j, k] > -1.0 or vof[m, j, k] < 1.0:
+                continue   # vacuum, the projectile above the surface, mixed cells
+            assert ps.eos_tp(h.cfg.mat[m], T[j, k], Pr[j, k]) == (D[m, j, k], E[m, j, k]), (j, k)
+            n += 1
+    assert n > 20
