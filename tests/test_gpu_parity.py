@@ -223,8 +223,11 @@ def test_device_determinism_and_reinit():
 
 def test_full_size_c1_properties():
     """The 1000 x 1500 benchmark workload: 60 steps from contact stay finite,
-    volume fractions sum to one in every physical cell, the material masses
-    stay positive, and a second context reproduces the state bit for bit."""
+    volume fractions sum to one in every physical cell to within the
+    volume-fraction cutoff (advect_flux_default drops a material whose
+    fraction falls below TOLVOF = 1e-6 without renormalising the rest, so a
+    cell may lose up to (nmat - 1) * 1e-6), the material masses stay
+    positive, and a second context reproduces the state bit for bit."""
     say("full size c1 1000x1500: setup")
     h = ps.HostRun(*ps.write_config(os.path.join(tempfile.mkdtemp(prefix="gf_"), "c1"), "c1"))
     st0 = h.state()
@@ -252,7 +255,7 @@ def test_full_size_c1_properties():
     for k in a:
         assert np.array_equal(a[k], b[k], equal_nan=True), k
     vof = a["M_VOF"][:, 2:-2, 2:-2]
-    assert np.abs(vof.sum(axis=0) - 1.0).max() <= 1e-12
+    assert np.abs(vof.sum(axis=0) - 1.0).max() <= (h.cfg.nmat - 1) * 1e-6
     assert (a["M_DEN"][:, 2:-2, 2:-2] >= 0).all() and (a["M_ENG"][:, 2:-2, 2:-2] >= 0).all()
     assert np.abs(a["V_VEL"]).max() > 1000.0      # the impact is under way
     say("  max |v| %.1f m/s, max damage %.3f" % (np.abs(a["V_VEL"]).max(), a["E_DAM"].max()))
