@@ -179,6 +179,55 @@ def ensemble_leg(nruns, steps, seed, wd):
     return out
 
 
+def sensitivity_leg(nx, ny, steps, wd):
+    """The differentiated step: the strength workload's physics on nx x ny
+    cells over a window of `steps` steps from contact.  Forward: the plain
+    steps.  Sensitivity: the same steps run on dual numbers carrying the
+    derivatives of the crater-depth proxy with respect to intact cohesion,
+    intact friction and impact speed (salec_sens_*).  Both are timed over the
+    step loop only, from the same uploaded state."""
+    inp, eos = ps.write_config(os.path.join(wd, "sens"), "c1", nx=nx, ny=ny)
+    h = ps.HostRun(inp, eos)
+    cs = ps.CraterSensitivity(h, steps)
+
+    def fresh():
+        d = ps.Device(h.cfg)
+        d.upload_state(cs.st0)
+        h.cfg.damp_time = d.init(h.damp_t_in)
+        d.set_time(h.time0, h.dt0)
+        return d
+
+    d = fresh()                          # load the dual-number kernels once
+    d.sens_begin(7, cs.mask, cs.row, cs.speed0)
+    d.sens_step(2)
+    d.sens_end()
+    d.close()
+    d = fresh()
+    d.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.step(sync=False)
+    d.sync()
+    fwd = time.perf_counter() - t0
+    d.close()
+    d = fresh()
+    d.sens_begin(7, cs.mask, cs.row, cs.speed0)
+    t0 = time.perf_counter()
+    d.sens_step(steps)
+    sen = time.perf_counter() - t0
+    loss, grad = d.sens_result()
+    d.sens_end()
+    d.close()
+    h.close()
+    out = {"workload": "strength-workload physics %dx%d, %d steps from contact" % (nx, ny, steps),
+           "params": ["intact cohesion", "intact friction", "impact speed"], "mode": "forward (3 tangent directions)",
+           "forward_ms_per_step": round(fwd / steps * 1e3, 4), "sensitivity_ms_per_step": round(sen / steps * 1e3, 4),
+           "time_ratio": round(sen / fwd, 3), "loss": loss, "grad": grad}
+    log("sensitivity: forward %.3f ms/step, with 3 tangents %.3f ms/step (x%.2f)" % (
+        out["forward_ms_per_step"], out["sensitivity_ms_per_step"], out["time_ratio"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +241,8 @@ def main():
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--ensemble", type=int, default=64, help="runs in the batched-ensemble leg (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--sens-steps", type=int, default=400,
+                    help="window of the sensitivity leg (512x768 cells; 0 = skip)")
     ap.add_argument("--checkpoint-out", default=None, help="save the advanced state here (salec_checkpoint)")
     ap.add_argument("--checkpoint-in", default=None,
                     help="start from this checkpoint instead of contact + advance (for profiler runs)")
@@ -322,6 +373,7 @@ def main():
 
     dev.close()
     ens = ensemble_leg(args.ensemble, args.ensemble_steps, 0, wd) if args.ensemble > 0 else None
+    sens = sensitivity_leg(512, 768, args.sens_steps, wd) if args.sens_steps > 0 else None
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -334,7 +386,7 @@ def main():
                       "start_time": t_adv,
                       "cells": cells, "tracers": int(tr.n), "nmat": nmat,
                       "sim_time_after": t_now},
-           "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens}
+           "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens, "sensitivity": sens}
     print(json.dumps(out), flush=True)
 
 

@@ -261,6 +261,35 @@ int  salec_batch_create(salec_ctx **ctxs, int n, salec_batch **out);
 int  salec_batch_step(salec_batch *b, int nsteps);
 void salec_batch_destroy(salec_batch *b);
 
+/* ---- parameter sensitivities (forward mode) -------------------------------
+ * Derivatives of a crater-depth proxy with respect to up to three scalar
+ * parameters, computed alongside the steps by the same kernels run on dual
+ * numbers: every value is computed exactly as in salec_step, and every
+ * derivative is that of the computed value with each branch (limiters,
+ * yield, cutoffs, table cells, the dt minimum) taken as the values take it.
+ * The reference has no differentiated path; these entry points stand where
+ * its step loop (src/salec2d.c:115-272) would be differentiated.
+ *
+ * Parameters (tangent directions, in grad[] order):
+ *   SALEC_SENS_COHESION  intact cohesion Yint0 of the materials in mat_mask
+ *   SALEC_SENS_FRICTION  intact friction coefficient Yfricint of those materials
+ *   SALEC_SENS_VELOCITY  impact speed: the velocity field at salec_sens_begin
+ *                        scaled by U / impact_speed (at the start of a run the
+ *                        only motion is the projectile's, so this is d/dU)
+ * Loss: L = sum over the vertices k of vertex row surface_row of
+ *       (sum over the sensitivity steps of v_dis_z(k, surface_row))^2.
+ * params = 0 accumulates L alone with the plain step (for finite differences).
+ * Tracers are not advanced by salec_sens_step.  While a run with params != 0
+ * is open, salec_step and the batch calls return SALEC_ESTATE. */
+#define SALEC_SENS_COHESION 0x1u
+#define SALEC_SENS_FRICTION 0x2u
+#define SALEC_SENS_VELOCITY 0x4u
+int  salec_sens_begin(salec_ctx *ctx, unsigned params, unsigned mat_mask, int surface_row, double impact_speed);
+int  salec_sens_step(salec_ctx *ctx, int nsteps);
+/* loss and dL/d(cohesion, friction, speed) so far (zero for unset params) */
+int  salec_sens_result(salec_ctx *ctx, double *loss, double grad[3]);
+int  salec_sens_end(salec_ctx *ctx);
+
 /* Block until all work queued on the context has finished. */
 int  salec_sync(salec_ctx *ctx);
 

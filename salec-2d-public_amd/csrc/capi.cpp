@@ -47,6 +47,17 @@ const char *kPhaseNames[PH_COUNT] = {
     "calculate_v_dis", "restrict_v_dis", "tracers", "energy_flux", "ghost_den_vib_eng", "flux_ghost_ring",
     "advect_damp_state", "advance_time"};
 
+/* an open sensitivity run: tangent planes, Dual scratch, the loss sums */
+struct SensState {
+    unsigned params = 0;
+    int row = 0;
+    TFields T{}, Talt{};                 /* tangent planes; Talt: second set of the remapped fields */
+    Dual *bx = nullptr, *by = nullptr, *top = nullptr, *part = nullptr, *acc = nullptr;
+    double *acc_p = nullptr;             /* loss sums of a params = 0 run */
+    double *stage = nullptr;             /* two-phase ghost staging for the tangents */
+    std::vector<void *> owned;
+};
+
 } // namespace
 
 struct salec_ctx {
@@ -83,6 +94,8 @@ struct salec_ctx {
     double phase_ms[PH_COUNT]{};
     long phase_calls[PH_COUNT]{};
     unsigned long long *d_nf = nullptr;
+    std::vector<DevMat> dm_host;         /* host copy of d_mats */
+    SensState *sens = nullptr;
 };
 
 /* Captured step graphs bake in kernel arguments (damp time, tracer count,
@@ -530,6 +543,7 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     }
     TRY(dalloc(c, &c->d_mats, SALEC_MAXMAT));
     TRY(hip_ok(hipMemcpy(c->d_mats, dm.data(), sizeof(DevMat) * SALEC_MAXMAT, hipMemcpyHostToDevice)));
+    c->dm_host = dm;
     /* ghost programs at the fill points of the loop body */
     TRY(build_program(c, c->gfA, {{SALEC_M_VOF, VT_ALWAYS_COPY}, {SALEC_E_DVEL, VT_VELOCITY_INC}, {SALEC_E_DEN, VT_DENSITY}}));
     TRY(build_program(c, c->gfE, {{SALEC_E_VEL, VT_VELOCITY}}));
@@ -565,10 +579,13 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     return SALEC_OK;
 }
 
+static void sens_free(salec_ctx *c);
+
 void salec_destroy(salec_ctx *c)
 {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    sens_free(c);
     for (hipGraphExec_t &e : c->gexec) if (e) hipGraphExecDestroy(e);
     for (void *p : c->owned) hipFree(p);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -725,6 +742,7 @@ static hipEvent_t next_event(salec_ctx *c)
 int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
 {
     if (!c) return SALEC_EINVAL;
+    if (c->sens && c->sens->params) return SALEC_ESTATE;
     const bool to_out = flags & SALEC_STEP_TO_OUTPUT, from_out = flags & SALEC_STEP_FROM_OUTPUT;
     if (to_out && from_out) return SALEC_EINVAL;
     if (from_out != (bool)c->half_open) return SALEC_ESTATE;
@@ -786,6 +804,199 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
         if (st) { st->dt = h.dt; st->t = h.t; st->step = (long)h.step; }
     }
     return SALEC_OK;
+}
+
+/* ---------------------------------------------------- sensitivities -- */
+static void sens_free(salec_ctx *c)
+{
+    SensState *S = c->sens;
+    if (!S) return;
+    for (void *p : S->owned) hipFree(p);
+    delete S;
+    c->sens = nullptr;
+}
+
+extern "C++" {
+template <class T>
+static int salloc(SensState *S, T **p, size_t count)
+{
+    void *q = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) return SALEC_ENOMEM;
+    if (hipMemset(q, 0, count * sizeof(T)) != hipSuccess) { hipFree(q); return SALEC_EDEVICE; }
+    S->owned.push_back(q);
+    *p = (T *)q;
+    return SALEC_OK;
+}
+}
+
+static int upload_seeds(salec_ctx *c, unsigned params, unsigned mat_mask)
+{
+    std::vector<DevMat> dm = c->dm_host;
+    for (int m = 1; m < c->g.nmat; ++m) {
+        bool on = (mat_mask >> m) & 1u;
+        dm[m].sens_y0 = on && (params & SALEC_SENS_COHESION) ? 1 : 0;
+        dm[m].sens_fr = on && (params & SALEC_SENS_FRICTION) ? 2 : 0;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_mats, dm.data(), sizeof(DevMat) * SALEC_MAXMAT, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SALEC_OK;
+}
+
+/* a ghost/restriction program on the tangent planes, then on the values */
+static void run_program_sens(salec_ctx *c, const Program &P)
+{
+    SensState *S = c->sens;
+    TanSlots ts{};
+    for (int s = 0; s < P.nslots; ++s) {
+        int f = P.field[s];
+        ts.pbase[s] = c->F.f[f];
+        ts.tbase[s] = S->T.t[f];
+        ts.plane[s] = P.slots.plane[s];
+        ts.tst[s] = S->T.st[f];
+        ts.ncomp[s] = P.slots.ncomp[s];
+    }
+    launch_ghost_tan(c->stream, ts, P.d_ops, P.nops, S->stage, P.two_phase);
+    run_program(c, P);
+}
+
+/* one loop iteration on Dual values (salec_step's sequence without tracers) */
+static void sens_step_once(salec_ctx *c)
+{
+    SensState *S = c->sens;
+    hipStream_t s = c->stream;
+    const Grid &g = c->g;
+    const Params &P = c->P;
+    const TFields &T = S->T;
+    launch_trace_profile(s, g, c->F, P, c->d_prof, c->prof_chunks);
+    launch_profile_finish(s, g, c->F, P, c->d_prof, c->prof_chunks);
+    run_program_sens(c, c->gfA);
+    launch_vertex_vof_vel(s, g, c->F, T, P);
+    run_program_sens(c, c->rstV);
+    launch_update_e_vel(s, g, c->F, T);
+    run_program_sens(c, c->gfE);
+    launch_check_v_vel(s, g, c->F, T);
+    launch_gradvel_dt(s, g, c->F, T, P, c->d_time, S->part, c->gv_blocks);
+    launch_stress(s, g, c->F, T, P, c->d_mats, c->d_time);
+    run_program_sens(c, c->gfB);
+    launch_acc(s, g, c->F, T, P, c->d_time);
+    run_program_sens(c, c->rstA);
+    launch_v_dis(s, g, c->F, T, c->d_time);
+    run_program_sens(c, c->rstD);
+    launch_energy_flux(s, g, c->F, T, c->d_mats, c->d_time, S->bx, S->by, S->top);
+    run_program_sens(c, c->gfC);
+    launch_flux_ring(s, g, c->F, T, S->bx, S->by);
+    Fields Fn = c->F;
+    TFields Tn = S->T;
+    for (int f : kSwapped) { Fn.f[f] = c->Falt.f[f]; Tn.t[f] = S->Talt.t[f]; }
+    launch_remap_close(s, g, c->F, T, Fn, Tn, P, c->d_mats, S->bx, S->by, S->top, c->d_time);
+    for (int f : kSwapped) { std::swap(c->F.f[f], c->Falt.f[f]); std::swap(S->T.t[f], S->Talt.t[f]); }
+    c->parity ^= 1;
+    launch_advance_time(s, c->d_time);
+    launch_sens_accum(s, g, c->F, S->T, S->acc, S->row);
+}
+
+int salec_sens_begin(salec_ctx *c, unsigned params, unsigned mat_mask, int surface_row, double impact_speed)
+{
+    if (!c || (params & ~7u)) return SALEC_EINVAL;
+    if (c->half_open || c->sens) return SALEC_ESTATE;
+    if (surface_row < 2 || surface_row > c->g.VH - 3) return SALEC_EINVAL;
+    if ((params & SALEC_SENS_VELOCITY) && !(impact_speed > 0.)) return SALEC_EINVAL;
+    const Grid &g = c->g;
+    SensState *S = new SensState();
+    c->sens = S;
+    S->params = params;
+    S->row = surface_row;
+    int rc = SALEC_OK;
+#define TRY(x) do { rc = (x); if (rc != SALEC_OK) { sens_free(c); return rc; } } while (0)
+    if (!params) {
+        TRY(salloc(S, &S->acc_p, (size_t)g.VW));
+        return SALEC_OK;
+    }
+    for (int f = 0; f < SALEC_NFIELDS; ++f) {
+        if (!tangent_field(f)) continue;
+        int grid, nc;
+        salec_field_info(f, g.nmat, &grid, &nc);
+        long n = salec_grid_points(grid, g.nx, g.ny) * nc;
+        S->T.st[f] = S->Talt.st[f] = n;
+        TRY(salloc(S, &S->T.t[f], (size_t)NTAN * n));
+    }
+    for (int f : kSwapped)
+        if (tangent_field(f)) TRY(salloc(S, &S->Talt.t[f], (size_t)NTAN * S->T.st[f]));
+    TRY(salloc(S, &S->bx, (size_t)g.nmat * g.NBX));
+    TRY(salloc(S, &S->by, (size_t)g.nmat * g.NBY));
+    TRY(salloc(S, &S->top, (size_t)2 * g.nmat * g.CW));
+    TRY(salloc(S, &S->part, (size_t)c->gv_blocks));
+    TRY(salloc(S, &S->acc, (size_t)g.VW));
+    if (c->stage_len > 0) TRY(salloc(S, &S->stage, (size_t)NTAN * c->stage_len));
+    /* the derivative of dt starts at zero; the velocity direction starts
+     * as the velocity field per unit of impact speed */
+    TRY(hip_ok(hipMemsetAsync(&c->d_time->dtd[0], 0, sizeof(double) * NTAN, c->stream)));
+    if (params & SALEC_SENS_VELOCITY) {
+        launch_scale_into(c->stream, c->F.f[SALEC_V_VEL], S->T.t[SALEC_V_VEL] + 2 * S->T.st[SALEC_V_VEL],
+                          S->T.st[SALEC_V_VEL], 1.0 / impact_speed);
+        launch_scale_into(c->stream, c->F.f[SALEC_E_VEL], S->T.t[SALEC_E_VEL] + 2 * S->T.st[SALEC_E_VEL],
+                          S->T.st[SALEC_E_VEL], 1.0 / impact_speed);
+    }
+    TRY(upload_seeds(c, params, mat_mask));
+    TRY(hip_ok(hipStreamSynchronize(c->stream)));
+#undef TRY
+    return SALEC_OK;
+}
+
+int salec_sens_step(salec_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) return SALEC_EINVAL;
+    if (!c->sens || c->half_open) return SALEC_ESTATE;
+    SensState *S = c->sens;
+    for (int k = 0; k < nsteps; ++k) {
+        if (S->params) {
+            sens_step_once(c);
+        } else {
+            int rc = salec_step(c, SALEC_STEP_NOSYNC, nullptr);
+            if (rc) return rc;
+            launch_sens_accum(c->stream, c->g, c->F, S->acc_p, S->row);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SALEC_OK;
+}
+
+int salec_sens_result(salec_ctx *c, double *loss, double grad[3])
+{
+    if (!c) return SALEC_EINVAL;
+    if (!c->sens) return SALEC_ESTATE;
+    SensState *S = c->sens;
+    const int W = c->g.VW;
+    double L = 0., G[NTAN] = {0., 0., 0.};
+    if (S->params) {
+        std::vector<Dual> a(W);
+        HIPCHK(hipMemcpyAsync(a.data(), S->acc, sizeof(Dual) * W, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int k = 2; k <= W - 3; ++k) {
+            L += a[k].v * a[k].v;
+            for (int q = 0; q < NTAN; ++q) G[q] += 2.0 * a[k].v * a[k].d[q];
+        }
+    } else {
+        std::vector<double> a(W);
+        HIPCHK(hipMemcpyAsync(a.data(), S->acc_p, sizeof(double) * W, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int k = 2; k <= W - 3; ++k) L += a[k] * a[k];
+    }
+    if (loss) *loss = L;
+    if (grad) for (int q = 0; q < NTAN; ++q) grad[q] = G[q];
+    return SALEC_OK;
+}
+
+int salec_sens_end(salec_ctx *c)
+{
+    if (!c) return SALEC_EINVAL;
+    if (!c->sens) return SALEC_ESTATE;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = c->sens->params ? upload_seeds(c, 0, 0) : SALEC_OK;
+    sens_free(c);
+    return rc;
 }
 
 /* checkpoint file: a fixed header, every field in field-id order (all
@@ -934,7 +1145,7 @@ int salec_batch_step(salec_batch *b, int nsteps)
 {
     if (!b || nsteps < 0) return SALEC_EINVAL;
     for (salec_ctx *c : b->ctx) {
-        if (c->half_open) return SALEC_ESTATE;
+        if (c->half_open || (c->sens && c->sens->params)) return SALEC_ESTATE;
         int rc = capture_graphs(c);
         if (rc) return rc;
     }

@@ -14,8 +14,29 @@
 
 namespace salec {
 
-#define CPL(fld, c, i) (F.f[SALEC_##fld][(long)(c) * g.NC + (i)])
-#define VPL(fld, c, i) (F.f[SALEC_##fld][(long)(c) * g.NV + (i)])
+/* Field element access.  The step kernels are templates on the scalar type
+ * S (double, or Dual for sensitivity runs, see dual.h) and see the tangent
+ * planes T next to the field buffers F; kernels that only ever run in
+ * double open a PRIMAL_SCOPE. */
+#define CPL(fld, c, i) fa<S, SALEC_##fld>(F.f, T, (long)(c) * g.NC + (i))
+#define VPL(fld, c, i) fa<S, SALEC_##fld>(F.f, T, (long)(c) * g.NV + (i))
+#define PRIMAL_SCOPE using S = double; const NoTan T{}; (void)T
+
+/* dt of the step as the scalar type S (value and its derivatives) */
+template <class S> __device__ __forceinline__ S tm_dt(const DevTime *tm);
+template <> __device__ __forceinline__ double tm_dt<double>(const DevTime *tm) { return tm->dt; }
+template <> __device__ __forceinline__ Dual tm_dt<Dual>(const DevTime *tm)
+{
+    Dual r(tm->dt);
+    for (int k = 0; k < NTAN; ++k) r.d[k] = tm->dtd[k];
+    return r;
+}
+__device__ __forceinline__ void tm_set_dt(DevTime *tm, double x) { tm->dt = x; }
+__device__ __forceinline__ void tm_set_dt(DevTime *tm, const Dual &x)
+{
+    tm->dt = x.v;
+    for (int k = 0; k < NTAN; ++k) tm->dtd[k] = x.d[k];
+}
 
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
@@ -73,6 +94,7 @@ template <class L> static void nm_dispatch(int nmat, L launch)
  * the ejecta flag.  Pass 1: thread per (column, row chunk). */
 __global__ void k_trace_profile(Grid g, Fields F, Params P, double *part, int nchunk)
 {
+    PRIMAL_SCOPE;
     int col = blockIdx.x * blockDim.x + threadIdx.x;
     int ch = blockIdx.y;
     if (col >= g.CW || ch >= nchunk) return;
@@ -176,6 +198,86 @@ void launch_ghost(hipStream_t s, const GhostSlots &sl, const GhostOp *ops, int n
     }
 }
 
+/* The same programs on the tangent planes.  Every mode is linear in the
+ * value except |x|, whose derivative is sign(x) times the tangent, so the
+ * primal source value is read as well; these run before the primal fill of
+ * the same program, while every source still holds its value from the start
+ * of the fill.  Tangent k of component c of slot s is at
+ * tbase[s] + k * tst[s] + c * plane[s]. */
+__device__ __forceinline__ double gmode_tan(double t, double x, int m)
+{
+    switch (m) {
+    case GM_COPY: return t;
+    case GM_NEG: return -t;
+    case GM_ABS: return x < 0. ? -t : t;
+    case GM_NABS: return x < 0. ? t : -t;
+    default: return 0.0;
+    }
+}
+__global__ void k_ghost_tan(TanSlots sl, const GhostOp *ops, int nops, double *stage, int phase)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nops) return;
+    GhostOp o = ops[i];
+    const double *pb = sl.pbase[o.slot];
+    double *tb = sl.tbase[o.slot];
+    if (!tb) return;                     /* a field without tangent planes */
+    long pl = sl.plane[o.slot], ts = sl.tst[o.slot];
+    int nc = sl.ncomp[o.slot];
+    for (int k = 0; k < NTAN; ++k)
+        for (int c = 0; c < nc; ++c) {
+            long q = ((long)k * nc + c) * nops + i;
+            if (phase == 2) { tb[k * ts + c * pl + o.dst] = stage[q]; continue; }
+            int m = c == 0 ? o.m0 : (c == 1 ? o.m1 : o.mr);
+            double t = gmode_tan(tb[k * ts + c * pl + o.src], pb[c * pl + o.src], m);
+            if (phase == 0) tb[k * ts + c * pl + o.dst] = t;
+            else stage[q] = t;
+        }
+}
+void launch_ghost_tan(hipStream_t s, const TanSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase)
+{
+    if (nops <= 0) return;
+    if (!two_phase) {
+        k_ghost_tan<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, 0);
+    } else {
+        k_ghost_tan<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, 1);
+        k_ghost_tan<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, 2);
+    }
+}
+
+/* ------------------------------------------------ sensitivity helpers -- */
+/* acc[k] += v_dis_z of vertex (k, row) for the vertices of one mesh row:
+ * the per-vertex sum over steps of the vertical displacement */
+template <class S>
+__global__ void k_sens_accum(Grid g, Fields F, TF<S> T, S *acc, int row)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 2 || k > g.VW - 3) return;
+    long v = k + (long)g.VW * row;
+    S a = acc[k];
+    a += VPL(V_DIS, 1, v);
+    acc[k] = a;
+}
+void launch_sens_accum(hipStream_t s, const Grid &g, const Fields &F, double *acc, int row)
+{
+    k_sens_accum<double><<<nblk(g.VW, 256), 256, 0, s>>>(g, F, NoTan{}, acc, row);
+}
+void launch_sens_accum(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, Dual *acc, int row)
+{
+    k_sens_accum<Dual><<<nblk(g.VW, 256), 256, 0, s>>>(g, F, T, acc, row);
+}
+/* t[i] = p[i] * f: a tangent plane proportional to its field */
+__global__ void k_scale_into(const double *p, double *t, long n, double f)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) t[i] = p[i] * f;
+}
+void launch_scale_into(hipStream_t s, const double *p, double *t, long n, double f)
+{
+    if (n <= 0) return;
+    k_scale_into<<<nblk(n, 256), 256, 0, s>>>(p, t, n, f);
+}
+
 /* vertex index of an active vertex (distance >= 1 from the padded edge) */
 __device__ __forceinline__ bool active_vertex(const Grid &g, long t, int &k, int &j)
 {
@@ -200,35 +302,37 @@ __device__ __forceinline__ bool physical_cell(const Grid &g, long t, int &k, int
  * e_dvel and e_den ghost fills.  The reference runs the internal vertices
  * before those fills; internal vertices touch only cells at distance >= 3,
  * which the fills never write, so the results are identical. */
-__global__ void k_vertex_vof_vel(Grid g, Fields F, Params P)
+template <class S>
+__global__ void k_vertex_vof_vel(Grid g, Fields F, TF<S> T, Params P)
 {
     int k, j;
     if (!active_vertex(g, xcd_thread(), k, j)) return;
     long v = k + (long)g.VW * j;
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
     for (int m = 0; m < g.nmat; ++m) {
-        double a = 0.0;
+        S a = 0.0;
         for (int i = 0; i < 4; ++i) a += 0.25 * CPL(M_VOF, m, e[i]);
         VPL(V_VOF, m, v) = a;
     }
     const int voff[4] = {2, 3, 0, 1};
-    double inc0 = 0., inc1 = 0., old0 = VPL(V_VEL, 0, v), old1 = VPL(V_VEL, 1, v);
-    double vvol = 0., vmass = 0.;
+    S inc0 = 0., inc1 = 0., old0 = VPL(V_VEL, 0, v), old1 = VPL(V_VEL, 1, v);
+    double vvol = 0.;
+    S vmass = 0.;
     int vc = 0;
     for (int i = 0; i < 4; ++i) {
         if (CPL(E_WPT, voff[i], e[i]) > 0.) vc++;
-        double den = CPL(E_DEN, 0, e[i]);
+        S den = CPL(E_DEN, 0, e[i]);
         if (den < TOLRHO) continue;
         double lv = VPL(V_VOL, i + 1, v);
         vvol += lv;
         vmass += den * lv;
-        double pa = den * lv;
+        S pa = den * lv;
         inc0 += pa * CPL(E_DVEL, 0, e[i]);
         inc1 += pa * CPL(E_DVEL, 1, e[i]);
     }
-    double v0, v1;
+    S v0, v1;
     if (vmass > TOLRHO * vvol) {
-        double pa = 1.0 / vmass, pb = VPL(V_MASS, 0, v) / vmass;
+        S pa = 1.0 / vmass, pb = VPL(V_MASS, 0, v) / vmass;
         v0 = pa * inc0 + pb * old0;
         v1 = pa * inc1 + pb * old1;
         VPL(V_MASS, 0, v) = vmass;
@@ -237,47 +341,53 @@ __global__ void k_vertex_vof_vel(Grid g, Fields F, Params P)
         v0 = 0.0; v1 = 0.0;
     }
     VPL(V_MASS, 1, v) = 1.0 * vc;
-    double r = 0.;
+    S r = 0.;
     r += v0 * v0; r += v1 * v1;
-    double vn = sqrt(r);
-    if (vn > P.velocity_max) { double px = P.velocity_max / vn; v0 *= px; v1 *= px; }
+    S vn = sqrt(r);
+    if (vn > P.velocity_max) { S px = P.velocity_max / vn; v0 *= px; v1 *= px; }
     if (vn < P.velocity_min * 1) { v0 = 0.0; v1 = 0.0; }
     VPL(V_VEL, 0, v) = v0; VPL(V_VEL, 1, v) = v1;
 }
 void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const Params &P)
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
-    k_vertex_vof_vel<<<nblk8(n, 256), 256, 0, s>>>(g, F, P);
+    k_vertex_vof_vel<double><<<nblk8(n, 256), 256, 0, s>>>(g, F, NoTan{}, P);
+}
+void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    k_vertex_vof_vel<Dual><<<nblk8(n, 256), 256, 0, s>>>(g, F, T, P);
 }
 
 /* ------------------------------------------------------- update_e_vel -- */
 /* src/sale2d.c:838-927 */
-__global__ void k_update_e_vel(Grid g, Fields F)
+template <class S>
+__global__ void k_update_e_vel(Grid g, Fields F, TF<S> T)
 {
     int k, j;
     if (!physical_cell(g, xcd_thread(), k, j)) return;
     long c = k + (long)g.CW * j;
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
-    double den = CPL(E_DEN, 0, c);
-    double e0 = 0.0, e1 = 0.0;
+    S den = CPL(E_DEN, 0, c);
+    S e0 = 0.0, e1 = 0.0;
     if (den >= TOLRHO) {
         int vac = 0;
-        double w[4];
+        S w[4];
         for (int kv = 0; kv < 4; ++kv) {
             w[kv] = CPL(E_WPT, kv, c);
-            double vm1 = VPL(V_MASS, 1, vv[kv]);
+            S vm1 = VPL(V_MASS, 1, vv[kv]);
             if (vm1 >= 4.0 || (vm1 >= 3.0 && w[kv] < 0.)) vac++;
         }
         if (vac != 4) {
-            double sm = 0.;
+            S sm = 0.;
             for (int kv = 0; kv < 4; ++kv) {
                 if (w[kv] > 0.0) continue;
-                double m = VPL(V_MASS, 0, vv[kv]);
+                S m = VPL(V_MASS, 0, vv[kv]);
                 e0 += m * VPL(V_VEL, 0, vv[kv]);
                 e1 += m * VPL(V_VEL, 1, vv[kv]);
                 sm += m;
             }
-            if (sm > 0.) { double px = 1.0 / sm; e0 *= px; e1 *= px; }
+            if (sm > 0.) { S px = 1.0 / sm; e0 *= px; e1 *= px; }
             else { e0 = 0.0; e1 = 0.0; }
         }
     }
@@ -285,12 +395,17 @@ __global__ void k_update_e_vel(Grid g, Fields F)
 }
 void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F)
 {
-    k_update_e_vel<<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F);
+    k_update_e_vel<double><<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, NoTan{});
+}
+void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T)
+{
+    k_update_e_vel<Dual><<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, T);
 }
 
 /* -------------------------------------------------------- check_v_vel -- */
 /* src/sale2d.c:1051-1141 */
-__global__ void k_check_v_vel(Grid g, Fields F)
+template <class S>
+__global__ void k_check_v_vel(Grid g, Fields F, TF<S> T)
 {
     int k, j;
     if (!active_vertex(g, xcd_thread(), k, j)) return;
@@ -298,28 +413,33 @@ __global__ void k_check_v_vel(Grid g, Fields F)
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
     const int voff[4] = {2, 3, 0, 1};
     int vcell = 0, vcor = 0;
-    double d[4];
+    S d[4];
     for (int i = 0; i < 4; ++i) {
         d[i] = CPL(E_DEN, 0, e[i]);
         if (d[i] < TOLRHO) vcell++;
         if (CPL(E_WPT, voff[i], e[i]) > 0.) vcor++;
     }
     if (vcell >= 4 || vcor < 4) return;
-    double a0 = 0., a1 = 0., ad = 0.;
+    S a0 = 0., a1 = 0., ad = 0.;
     for (int i = 0; i < 4; ++i) {
         if (d[i] < TOLRHO) continue;
         ad += d[i];
         a0 += d[i] * CPL(E_VEL, 0, e[i]);
         a1 += d[i] * CPL(E_VEL, 1, e[i]);
     }
-    double pa = 1.0 / ad;
+    S pa = 1.0 / ad;
     VPL(V_VEL, 0, v) = pa * a0;
     VPL(V_VEL, 1, v) = pa * a1;
 }
 void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F)
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
-    k_check_v_vel<<<nblk8(n, 256), 256, 0, s>>>(g, F);
+    k_check_v_vel<double><<<nblk8(n, 256), 256, 0, s>>>(g, F, NoTan{});
+}
+void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    k_check_v_vel<Dual><<<nblk8(n, 256), 256, 0, s>>>(g, F, T);
 }
 
 /* ------------------------------------------- update_e_gradvel + dt min -- */
@@ -329,26 +449,28 @@ void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F)
  * exact and order-independent, so dt is bit-reproducible.
  * dt = Min(max_dt, MIN(1.1*dt_prev, DTF*min_cells(...)))  (src/salec2d.c:167) */
 #define GV_BS 256
-__device__ __forceinline__ double wave_min(double v)
+template <class S>
+__device__ __forceinline__ S wave_min(S v)
 {
     for (int off = 32; off > 0; off >>= 1) {
-        double o = __shfl_xor(v, off, 64);
+        S o = shfl_xor(v, off);
         v = o < v ? o : v;
     }
     return v;
 }
-__global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, Params P, double *part)
+template <class S>
+__global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T, Params P, S *part)
 {
-    __shared__ double red[GV_BS / 64];
+    __shared__ S red[GV_BS / 64];
     int k, j;
-    double cand = HUGE_VAL;  /* identity of MIN over finite values */
+    S cand = HUGE_VAL;  /* identity of MIN over finite values */
     bool has = false;
     if (physical_cell(g, xcd_thread(), k, j)) {
         long c = k + (long)g.CW * j;
         long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
-        double vx[4], vy[4];
+        S vx[4], vy[4];
         for (int i = 0; i < 4; ++i) { vx[i] = VPL(V_VEL, 0, vv[i]); vy[i] = VPL(V_VEL, 1, vv[i]); }
-        double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+        S g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
         for (int i = 0; i < 4; ++i) g0 += CPL(E_GRAD, i, c) * vx[i];
         for (int i = 0; i < 4; ++i) g2 += CPL(E_GRAD, i, c) * vy[i];
         for (int i = 0; i < 4; ++i) g1 += CPL(E_GRAD, 4 + i, c) * vx[i];
@@ -356,48 +478,50 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, Params P
         for (int i = 0; i < 4; ++i) g4 += CPL(E_GRAD, 8 + i, c) * vx[i];
         CPL(E_GRAD_VEL, 0, c) = g0; CPL(E_GRAD_VEL, 1, c) = g1; CPL(E_GRAD_VEL, 2, c) = g2;
         CPL(E_GRAD_VEL, 3, c) = g3; CPL(E_GRAD_VEL, 4, c) = g4;
-        double div = g0 + g3 + g4;
+        S div = g0 + g3 + g4;
         CPL(E_DIV_VEL, 0, c) = div;
-        double v0 = CPL(M_VOF, 0, c), csd = CPL(E_CSD, 0, c);
-        double q;
+        S v0 = CPL(M_VOF, 0, c), csd = CPL(E_CSD, 0, c);
+        S q;
         if (v0 > TOLVOF || CPL(E_PRE, 0, c) <= 0.0) {
             q = 0.;
         } else {
             double ea = CPL(E_VOL, 0, c) / CPL(E_POS, 0, c);
             ea = fabs(ea);
-            double ql = P.avis1 * csd * sqrt(ea);
-            double qq = P.avis2 * div * ea;
-            q = (0. < div ? 0. : div) * CPL(E_DEN, 0, c) * (qq - ql);
+            S ql = P.avis1 * csd * sqrt(ea);
+            S qq = P.avis2 * div * ea;
+            q = (0. < div ? S(0.) : div) * CPL(E_DEN, 0, c) * (qq - ql);
             if (q < TOLQPRE) q = 0.;
         }
         CPL(E_Q, 0, c) = q;
         if (!(v0 > 1.0 - TOLVOF)) {
             double cyl = 1.0;
-            double dr = cyl * CPL(E_POS, 2, c) / (fabs(CPL(E_VEL, 0, c)) + csd);
-            double dz = 1.0 * CPL(E_POS, 3, c) / (fabs(CPL(E_VEL, 1, c)) + csd);
+            S ev0 = CPL(E_VEL, 0, c), ev1 = CPL(E_VEL, 1, c);
+            S dr = cyl * CPL(E_POS, 2, c) / (fabs(ev0) + csd);
+            S dz = 1.0 * CPL(E_POS, 3, c) / (fabs(ev1) + csd);
             cand = DTF * (dr < dz ? dr : dz);
             has = true;
         }
     }
     (void)has;
-    double m = wave_min(cand);
+    S m = wave_min(cand);
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double b = red[0];
+        S b = red[0];
         for (int w = 1; w < GV_BS / 64; ++w) b = red[w] < b ? red[w] : b;
         part[blockIdx.x] = b;
     }
 }
 /* dt = Min(max_dt, MIN(1.1*dt_prev, min over the block partials)) */
 #define DT_BS 1024
-__global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, const double *part, int nparts)
+template <class S>
+__global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, const S *part, int nparts)
 {
-    __shared__ double red[DT_BS / 64];
-    double r = HUGE_VAL;
+    __shared__ S red[DT_BS / 64];
+    S r = HUGE_VAL;
     for (int b = threadIdx.x; b < nparts; b += DT_BS) {
-        double v = part[b];
+        S v = part[b];
         r = v < r ? v : r;
     }
     r = wave_min(r);
@@ -405,35 +529,42 @@ __global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, cons
     if (lane == 0) red[wid] = r;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double b = red[0];
+        S b = red[0];
         for (int w = 1; w < DT_BS / 64; ++w) b = red[w] < b ? red[w] : b;
-        double rtn = 1.1 * tm->dt;
+        S rtn = 1.1 * tm_dt<S>(tm);
         rtn = b < rtn ? b : rtn;       /* MIN(rtn_dt, dt_rz) over all cells */
-        tm->dt = dMin(P.max_dt, rtn);
+        tm_set_dt(tm, dMin(P.max_dt, rtn));
     }
 }
 int gradvel_dt_blocks(const Grid &g) { return nblk8((long)g.nx * g.ny, GV_BS); }
 void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const Params &P, DevTime *tm,
                        double *part, int nblocks)
 {
-    k_gradvel_dt<<<nblocks, GV_BS, 0, s>>>(g, F, P, part);
-    k_dt_finish<<<1, DT_BS, 0, s>>>(P, tm, part, nblocks);
+    k_gradvel_dt<double><<<nblocks, GV_BS, 0, s>>>(g, F, NoTan{}, P, part);
+    k_dt_finish<double><<<1, DT_BS, 0, s>>>(P, tm, part, nblocks);
+}
+void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, DevTime *tm,
+                       Dual *part, int nblocks)
+{
+    k_gradvel_dt<Dual><<<nblocks, GV_BS, 0, s>>>(g, F, T, P, part);
+    k_dt_finish<Dual><<<1, DT_BS, 0, s>>>(P, tm, part, nblocks);
 }
 
 /* ------------------------------------ block_vibration + update_e_stress -- */
 /* block_vibration (src/strength.c:625-695) then update_e_stress
  * (src/strength.c:7-349, avg_strength mixing) for one physical cell. */
-template <int NM>
-__global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, const DevMat *mats, const DevTime *tm)
+template <class S, int NM>
+__global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, TF<S> T, Params P, const DevMat *mats, const DevTime *tm)
 {
     int k, j;
     if (!physical_cell(g, xcd_thread(), k, j)) return;
-    const double dt = tm->dt, time = tm->t;
+    const S dt = tm_dt<S>(tm);
+    const double time = tm->t;
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g);
-    double v0 = CPL(M_VOF, 0, c);
-    double dam = CPL(E_DAM, 0, c);
-    double vib = CPL(E_VIB, 0, c);
+    S v0 = CPL(M_VOF, 0, c);
+    S dam = CPL(E_DAM, 0, c);
+    S vib = CPL(E_VIB, 0, c);
     if (P.acoustic_fluid) {
         if (v0 > TOLVOF || dam < 0.5) {
             vib = 0.;
@@ -447,10 +578,10 @@ __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, cons
             if (Td <= 0. || To <= 0.) {
                 vib = 0.;
             } else {
-                double ev0 = CPL(E_VEL, 0, c), ev1 = CPL(E_VEL, 1, c);
-                double l = 0.;
+                S ev0 = CPL(E_VEL, 0, c), ev1 = CPL(E_VEL, 1, c);
+                S l = 0.;
                 l += ev0 * ev0; l += ev1 * ev1;
-                double nv = Cv * sqrt(l), nv2 = nv * nv;
+                S nv = Cv * sqrt(l), nv2 = nv * nv;
                 if ((time <= To) && (nv2 >= vib) && (nv >= 1.0) && (CPL(E_Q, 0, c) > 0.))
                     vib = dMin(nv2, VM * VM);
                 else
@@ -464,11 +595,11 @@ __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, cons
         CPL(E_STE2S, 0, c) = 0.; CPL(E_STA2S, 0, c) = 0.; CPL(E_STRENGTH, 0, c) = 0.;
         return;
     }
-    double ste[4];
+    S ste[4];
     for (int i = 0; i < 4; ++i) ste[i] = CPL(E_STE, i, c);
-    double vref = 0., mref = 0.;
+    S vref = 0., mref = 0.;
     for (int m = 1; m < n; ++m) {
-        double vf = CPL(M_VOF, m, c);
+        S vf = CPL(M_VOF, m, c);
         if (vf < TOLVOF) continue;
         vref += vf * mats[m].VaporDen;
         mref += vf * mats[m].MeltDen;
@@ -476,10 +607,10 @@ __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, cons
     double mp = 1.0;
     mp = dMax(mp, 1.0);
     vref /= mp; mref /= mp;
-    double eden = CPL(E_DEN, 0, c);
-    double s2o = CPL(E_STE2S, 0, c);
-    double ste2s = s2o, sta2s = CPL(E_STA2S, 0, c), tps = CPL(E_TPS, 0, c);
-    double strength = CPL(E_STRENGTH, 0, c);
+    S eden = CPL(E_DEN, 0, c);
+    S s2o = CPL(E_STE2S, 0, c);
+    S ste2s = s2o, sta2s = CPL(E_STA2S, 0, c), tps = CPL(E_TPS, 0, c);
+    S strength = CPL(E_STRENGTH, 0, c);
     int vapor = 0, melt = 0;
     if (eden <= vref) {
         for (int i = 0; i < 4; ++i) ste[i] = 0.0;
@@ -487,78 +618,78 @@ __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, cons
     } else if (eden <= mref) {
         for (int i = 0; i < 4; ++i) ste[i] = 0.0;
         ste2s = 0.; melt = 1;
-        double si[4] = {CPL(E_GRAD_VEL, 0, c), 0.5 * CPL(E_GRAD_VEL, 1, c) + 0.5 * CPL(E_GRAD_VEL, 2, c),
-                        CPL(E_GRAD_VEL, 3, c), CPL(E_GRAD_VEL, 4, c)};
-        double tr = si[0] + si[2] + si[3];
+        S si[4] = {CPL(E_GRAD_VEL, 0, c), 0.5 * CPL(E_GRAD_VEL, 1, c) + 0.5 * CPL(E_GRAD_VEL, 2, c),
+                   CPL(E_GRAD_VEL, 3, c), CPL(E_GRAD_VEL, 4, c)};
+        S tr = si[0] + si[2] + si[3];
         si[0] -= tr / 3.0; si[2] -= tr / 3.0; si[3] -= tr / 3.0;
         sta2s = sqrt(second_inv(si));
         tps = dMin(tps + sta2s * dt, 10.);
         dam = 1.; strength = 0.;
     } else {
-        double gv[5];
+        S gv[5];
         for (int i = 0; i < 5; ++i) gv[i] = CPL(E_GRAD_VEL, i, c);
-        double si[4] = {gv[0], 0.5 * gv[1] + 0.5 * gv[2], gv[3], gv[4]};
-        double tr = si[0] + si[2] + si[3];
+        S si[4] = {gv[0], 0.5 * gv[1] + 0.5 * gv[2], gv[3], gv[4]};
+        S tr = si[0] + si[2] + si[3];
         si[0] -= tr / 3.0; si[2] -= tr / 3.0; si[3] -= tr / 3.0;
-        double s2w = (gv[2] - gv[1]) * dt;
+        S s2w = (gv[2] - gv[1]) * dt;
         s2w = dWind(s2w, -1., 1.);
-        double c2w = sqrt(1.0 - s2w * s2w) - 1.0;
-        double ss2 = (ste[0] - ste[2]) / 2.0;
-        double rot[4] = {ss2 * c2w + ste[1] * s2w, ste[1] * c2w - ss2 * s2w, -ss2 * c2w - ste[1] * s2w, 0.0};
+        S c2w = sqrt(1.0 - s2w * s2w) - 1.0;
+        S ss2 = (ste[0] - ste[2]) / 2.0;
+        S rot[4] = {ss2 * c2w + ste[1] * s2w, ste[1] * c2w - ss2 * s2w, -ss2 * c2w - ste[1] * s2w, 0.0};
         for (int i = 0; i < 4; ++i) ste[i] += 1.0 * rot[i];
-        double pa = 2.0 * CPL(E_MU, 0, c) * dt;
+        S pa = 2.0 * CPL(E_MU, 0, c) * dt;
         for (int i = 0; i < 4; ++i) ste[i] += pa * si[i];
         ste2s = sqrt(second_inv(ste));
         sta2s = sqrt(second_inv(si));
     }
-    double pre = CPL(E_PRE, 0, c), tem = CPL(E_TEM, 0, c), csd = CPL(E_CSD, 0, c);
+    S pre = CPL(E_PRE, 0, c), tem = CPL(E_TEM, 0, c), csd = CPL(E_CSD, 0, c);
     if (vapor) {
         dam = 0.;
     } else if (melt) {
         dam = 1.0;
     } else {
-        double lss = 0.0, lts = 0.;
+        S lss = 0.0, lts = 0.;
         for (int m = 1; m < n; ++m) {
-            double vf = CPL(M_VOF, m, c);
+            S vf = CPL(M_VOF, m, c);
             if (vf < TOLVOF) continue;
             const DevMat &r = mats[m];
-            double my = strength_fn(r.yfunc, pre, dam, tps, sta2s, tem, r);
+            S my = strength_fn<S>(r.yfunc, pre, dam, tps, sta2s, tem, r);
             if (P.acoustic_fluid && pre >= 0.) {
-                double vp = dMin(eden * csd * sqrt(vib), pre);
-                double mt = SimonMelt(pre, r);
-                double sf = OhnakaSoft(tem, r, mt);
+                S vp = dMin(eden * csd * sqrt(vib), pre);
+                S mt = SimonMelt(pre, r);
+                S sf = OhnakaSoft(tem, r, mt);
                 sf = dWind(sf, 0.0, 1.0);
-                double lp = pre - vp * dam * dam * sqrt(sf);
-                double myv = strength_fn(r.yfunc, lp, dam, tps, sta2s, tem, r) + eden * r.Acvis * sta2s;
+                S lp = pre - vp * dam * dam * sqrt(sf);
+                S myv = strength_fn<S>(r.yfunc, lp, dam, tps, sta2s, tem, r) + eden * r.Acvis * sta2s;
                 my = dMin(myv, my);
             }
             lss += vf * dMax(my, 0.0);
-            if (P.tensile_failure) lts += vf * strength_fn(r.tfunc, pre, dam, tps, sta2s, tem, r);
+            if (P.tensile_failure) lts += vf * strength_fn<S>(r.tfunc, pre, dam, tps, sta2s, tem, r);
         }
         strength = lss;
         if (lss < ste2s) {
             sta2s = sta2s / dMax(0.5, (ste2s - s2o) / (ste2s - lss));
-            double sfac = lss / ste2s;
+            S sfac = lss / ste2s;
             for (int i = 0; i < 4; ++i) ste[i] *= sfac;
             ste2s = lss;
-            double dsd = 0.;
+            S dsd = 0.;
             for (int m = 1; m < n; ++m)
-                dsd += CPL(M_VOF, m, c) * strength_fn(mats[m].dfunc, pre, dam, 0., 0., 0., mats[m]);
+                dsd += CPL(M_VOF, m, c) * strength_fn<S>(mats[m].dfunc, pre, dam, 0., 0., 0., mats[m]);
             dam = dWind(dam + sta2s * dsd * dt, 0.0, 1.0);
             tps = dMin(tps + sta2s * dt, 10.);
         }
         if (P.tensile_failure) {
-            double Sxx = ste[0], Szz = ste[2], Sxz = ste[1], Sth = ste[3];
-            double Ts = sqrt(Sxz * Sxz + 0.25 * (Sxx - Szz) * (Sxx - Szz));
-            double Tmax = dMax(0.5 * (Sxx + Szz) - Ts, 0.5 * (Sxx + Szz) + Ts);
+            S Sxx = ste[0], Szz = ste[2], Sxz = ste[1], Sth = ste[3];
+            S Ts = sqrt(Sxz * Sxz + 0.25 * (Sxx - Szz) * (Sxx - Szz));
+            S Tmax = dMax(0.5 * (Sxx + Szz) - Ts, 0.5 * (Sxx + Szz) + Ts);
             Tmax = dMax(Tmax, Sth);
             if ((Tmax - pre) > lts) {
-                double dinc = 0.4 * csd / dMin(CPL(E_POS, 1, c), CPL(E_POS, 2, c));
-                double dold = dam;
-                double t3 = pow(dold, 1.0 / 3.);
+                S dinc = 0.4 * csd / dMin(CPL(E_POS, 1, c), CPL(E_POS, 2, c));
+                S dold = dam;
+                S t3 = pow(dold, 1.0 / 3.);
                 t3 += dinc;
                 dam = dWind(pow(t3, 3.0), 0., 1.);
-                double sfac = dold < 0.98 ? (1.0 - dam) / (1.0 - dold) : 0.0;
+                S sfac = dold < 0.98 ? (1.0 - dam) / (1.0 - dold) : S(0.0);
                 for (int i = 0; i < 4; ++i) ste[i] *= sfac;
                 ste2s *= sfac;
             }
@@ -571,7 +702,14 @@ __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, Params P, cons
 void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_stress<decltype(I)::value><<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, P, mats, tm);
+        k_stress<double, decltype(I)::value><<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, NoTan{}, P, mats, tm);
+    });
+}
+void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, const DevMat *mats,
+                   const DevTime *tm)
+{
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_stress<Dual, decltype(I)::value><<<nblk8((long)g.nx * g.ny, 256), 256, 0, s>>>(g, F, T, P, mats, tm);
     });
 }
 
@@ -580,16 +718,18 @@ void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &
  * calculate_acc_default (src/velocity.c:9-155) otherwise, over all active
  * vertices after the e_pre/e_ste/e_q ghost fills (internal vertices never
  * read the cells those fills write). */
-template <bool PML>
-__global__ void k_acc(Grid g, Fields F, Params P, const DevTime *tm)
+template <class S, bool PML>
+__global__ void k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
 {
     int k, j;
     if (!active_vertex(g, xcd_thread(), k, j)) return;
-    const double dt = tm->dt;
+    const S dt = tm_dt<S>(tm);
     long v = k + (long)g.VW * j;
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
-    double a0 = 0.0, a1 = 0.0, vm = 0.0, vv = 0.0;
-    double vs0 = 0., vs1 = 0., al0 = 0., al1 = 0.;
+    S a0 = 0.0, a1 = 0.0, vm = 0.0;
+    double vv = 0.0;
+    double vs0 = 0., vs1 = 0.;
+    S al0 = 0., al1 = 0.;
     bool damp = false;
     if (PML) {
         for (int i = 0; i < 4; ++i) { vs0 += 0.25 * CPL(E_SIGMA, 0, e[i]); vs1 += 0.25 * CPL(E_SIGMA, 1, e[i]); }
@@ -598,19 +738,19 @@ __global__ void k_acc(Grid g, Fields F, Params P, const DevTime *tm)
     }
     for (int i = 0; i < 4; ++i) {
         double su0 = VPL(V_SUF, 2 * i, v), su1 = VPL(V_SUF, 2 * i + 1, v), sv = VPL(V_SUBVOL, i, v);
-        double lp = CPL(E_PRE, 0, e[i]) + CPL(E_Q, 0, e[i]);
-        double s0 = CPL(E_STE, 0, e[i]), s1 = CPL(E_STE, 1, e[i]), s2 = CPL(E_STE, 2, e[i]), s3 = CPL(E_STE, 3, e[i]);
+        S lp = CPL(E_PRE, 0, e[i]) + CPL(E_Q, 0, e[i]);
+        S s0 = CPL(E_STE, 0, e[i]), s1 = CPL(E_STE, 1, e[i]), s2 = CPL(E_STE, 2, e[i]), s3 = CPL(E_STE, 3, e[i]);
         if (PML) {
             double lpr = CPL(E_DAMPREF, 2, e[i]);
-            double Sxx = s0 - lp, Szz = s2 - lp, Sxz = s1, Sth = s3 - lp;
-            double f0 = 0, f1 = 0;
+            S Sxx = s0 - lp, Szz = s2 - lp, Sxz = s1, Sth = s3 - lp;
+            S f0 = 0, f1 = 0;
             f0 += su0 * Sxx; f0 += su1 * Sxz;
             f1 += su0 * Sxz; f1 += su1 * Szz;
             f0 += sv * Sth;
             a0 += -1.0 * f0; a1 += -1.0 * f1;
             if (damp) {
                 Sxx += lpr; Szz += lpr; Sth += lpr;
-                double b0 = 0, b1 = 0;
+                S b0 = 0, b1 = 0;
                 b0 += su0 * (vs1 * Sxx); b0 += su1 * (vs0 * Sxz);
                 b1 += su0 * (vs1 * Sxz); b1 += su1 * (vs0 * Szz);
                 al0 += b0 * dt;
@@ -618,10 +758,10 @@ __global__ void k_acc(Grid g, Fields F, Params P, const DevTime *tm)
                 al0 += (vs0 + vs1) * sv * Sth * dt;
             }
         } else {
-            double fp0 = 0., fp1 = 0.;
+            S fp0 = 0., fp1 = 0.;
             fp0 += lp * su0; fp1 += lp * su1;
             fp0 += sv * lp;
-            double f0 = 0, f1 = 0;
+            S f0 = 0, f1 = 0;
             f0 += su0 * s0; f0 += su1 * s1;
             f1 += su0 * s1; f1 += su1 * s2;
             f0 += sv * s3;
@@ -641,9 +781,9 @@ __global__ void k_acc(Grid g, Fields F, Params P, const DevTime *tm)
         VPL(V_ALPHA, 0, v) = al0; VPL(V_ALPHA, 1, v) = al1;
     }
     int xd = min(k, g.VW - 1 - k), yd = min(j, g.VH - 1 - j);
-    double n0 = 0., n1 = 0.;
+    S n0 = 0., n1 = 0.;
     if (!(xd <= 1 || yd <= 1)) {
-        double L0 = 0, L1 = 0;
+        S L0 = 0, L1 = 0;
         int q = 0;
         for (int dj = -1; dj <= 1; ++dj)
             for (int dk = -1; dk <= 1; ++dk, ++q) {
@@ -658,7 +798,7 @@ __global__ void k_acc(Grid g, Fields F, Params P, const DevTime *tm)
     }
     if (VPL(V_MASS, 1, v) >= 1.) { n0 = 0.0; n1 = 0.0; }
     if (vm > TOLRHO * vv) {
-        double px = 1.0 / vm;
+        S px = 1.0 / vm;
         a0 *= px; a1 *= px;
         a0 += 1.0 * VPL(V_BF, 0, v); a1 += 1.0 * VPL(V_BF, 1, v);
         a0 += P.anc * n0; a1 += P.anc * n1;
@@ -675,22 +815,31 @@ void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, 
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
     if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3)
-        k_acc<true><<<nblk8(n, 256), 256, 0, s>>>(g, F, P, tm);
+        k_acc<double, true><<<nblk8(n, 256), 256, 0, s>>>(g, F, NoTan{}, P, tm);
     else
-        k_acc<false><<<nblk8(n, 256), 256, 0, s>>>(g, F, P, tm);
+        k_acc<double, false><<<nblk8(n, 256), 256, 0, s>>>(g, F, NoTan{}, P, tm);
+}
+void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, const DevTime *tm)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3)
+        k_acc<Dual, true><<<nblk8(n, 256), 256, 0, s>>>(g, F, T, P, tm);
+    else
+        k_acc<Dual, false><<<nblk8(n, 256), 256, 0, s>>>(g, F, T, P, tm);
 }
 
 /* ------------------------------------------------------ calculate_v_dis -- */
 /* src/sale2d.c:1238-1268 */
-__global__ void k_v_dis(Grid g, Fields F, const DevTime *tm)
+template <class S>
+__global__ void k_v_dis(Grid g, Fields F, TF<S> T, const DevTime *tm)
 {
     int k, j;
     if (!active_vertex(g, xcd_thread(), k, j)) return;
-    const double dt = tm->dt;
+    const S dt = tm_dt<S>(tm);
     long v = k + (long)g.VW * j;
-    double pb = 0.5 * dt * dt;
+    S pb = 0.5 * dt * dt;
     for (int c = 0; c < 2; ++c) {
-        double u = VPL(V_VEL, c, v), a = VPL(V_ACC, c, v);
+        S u = VPL(V_VEL, c, v), a = VPL(V_ACC, c, v);
         VPL(V_DIS, c, v) = dt * u + pb * a;
         VPL(V_VEL, c, v) = u + dt * a;
     }
@@ -698,29 +847,34 @@ __global__ void k_v_dis(Grid g, Fields F, const DevTime *tm)
 void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const DevTime *tm)
 {
     long n = (long)(g.VW - 2) * (g.VH - 2);
-    k_v_dis<<<nblk8(n, 256), 256, 0, s>>>(g, F, tm);
+    k_v_dis<double><<<nblk8(n, 256), 256, 0, s>>>(g, F, NoTan{}, tm);
+}
+void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const DevTime *tm)
+{
+    long n = (long)(g.VW - 2) * (g.VH - 2);
+    k_v_dis<Dual><<<nblk8(n, 256), 256, 0, s>>>(g, F, T, tm);
 }
 
 /* -------------------------------------------------------- update_energy -- */
 /* src/sale2d.c:564-712 (porosity model off) */
-template <int NM>
-__device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, const DevMat *mats, double dt, int k, int j)
+template <class S, int NM, class TT>
+__device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, const TT &T, const DevMat *mats, S dt, int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g);
     if (CPL(M_VOF, 0, c) > TOLVOF || CPL(E_DEN, 0, c) < TOLRHO) return;
-    double lp = CPL(E_PRE, 0, c) + CPL(E_Q, 0, c);
-    double div = CPL(E_DIV_VEL, 0, c);
-    double dEp = -lp * div;
-    double dEs = CPL(E_STE, 0, c) * CPL(E_GRAD_VEL, 0, c)
-               + CPL(E_STE, 1, c) * (CPL(E_GRAD_VEL, 1, c) + CPL(E_GRAD_VEL, 2, c))
-               + CPL(E_STE, 2, c) * CPL(E_GRAD_VEL, 3, c) + CPL(E_STE, 3, c) * CPL(E_GRAD_VEL, 4, c);
-    double eg = 0.;
+    S lp = CPL(E_PRE, 0, c) + CPL(E_Q, 0, c);
+    S div = CPL(E_DIV_VEL, 0, c);
+    S dEp = -lp * div;
+    S dEs = CPL(E_STE, 0, c) * CPL(E_GRAD_VEL, 0, c)
+          + CPL(E_STE, 1, c) * (CPL(E_GRAD_VEL, 1, c) + CPL(E_GRAD_VEL, 2, c))
+          + CPL(E_STE, 2, c) * CPL(E_GRAD_VEL, 3, c) + CPL(E_STE, 3, c) * CPL(E_GRAD_VEL, 4, c);
+    S eg = 0.;
     for (int m = 1; m < n; ++m) {
-        double vf = CPL(M_VOF, m, c);
+        S vf = CPL(M_VOF, m, c);
         if (vf > TOLVOF) eg += vf * mats[m].gamfrac / CPL(M_DEN, m, c);
     }
-    double dcvs = div - eg * (dEp + dEs);
+    S dcvs = div - eg * (dEp + dEs);
     for (int m = 1; m < n; ++m) {
         if (CPL(M_VOF, m, c) <= TOLVOF || CPL(M_DEN, m, c) <= TOLRHO) {
             CPL(M_ENG, m, c) = 0.;
@@ -734,16 +888,17 @@ __device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, cons
 /* ------------------------------------------------------------- tracers -- */
 /* fl_update_node, src/tracer.c:52-175; a tracer that leaves the physical
  * cells is deleted (its neighbour rank is absent on a single domain). */
-__global__ void k_tracers(Grid g, Fields F, Params P, Tracers T, const double *vx, const double *vy, const DevTime *tm)
+__global__ void k_tracers(Grid g, Fields F, Params P, Tracers Tr, const double *vx, const double *vy, const DevTime *tm)
 {
+    PRIMAL_SCOPE;
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= T.n || !T.alive[i]) return;
+    if (i >= Tr.n || !Tr.alive[i]) return;
     const double dt = tm->dt;
-    int k = T.idx[i], j = T.idy[i];
+    int k = Tr.idx[i], j = Tr.idy[i];
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
     long c = k + (long)g.CW * j;
-    double p0 = T.pos[i], p1 = T.pos[T.n + i];
-    double u0 = T.vel[i], u1 = T.vel[T.n + i];
+    double p0 = Tr.pos[i], p1 = Tr.pos[Tr.n + i];
+    double u0 = Tr.vel[i], u1 = Tr.vel[Tr.n + i];
     double r0 = 1.0 * p0 + -1.0 * VPL(V_POS, 0, vv[0]);
     double r1 = 1.0 * p1 + -1.0 * VPL(V_POS, 1, vv[0]);
     r0 = r0 / (vx[k + 1] - vx[k]) * 2.0 - 1.0;
@@ -763,19 +918,19 @@ __global__ void k_tracers(Grid g, Fields F, Params P, Tracers T, const double *v
         p0 += 1.0 * d0; p1 += 1.0 * d1;
         u0 = 1.0 * t0; u1 = 1.0 * t1;
     }
-    int mid = T.matid[i];
-    if (mid > 0 && p1 > P.threshold_z) T.matid[i] = -mid;
+    int mid = Tr.matid[i];
+    if (mid > 0 && p1 > P.threshold_z) Tr.matid[i] = -mid;
     double ep = CPL(E_PRE, 0, c), et = CPL(E_TEM, 0, c);
-    double mp = (double)T.mpre[i], mt = (double)T.mtem[i];
-    T.mpre[i] = (float)(mp > ep ? mp : ep);
-    T.mtem[i] = (float)(mt > et ? mt : et);
+    double mp = (double)Tr.mpre[i], mt = (double)Tr.mtem[i];
+    Tr.mpre[i] = (float)(mp > ep ? mp : ep);
+    Tr.mtem[i] = (float)(mt > et ? mt : et);
     int xo = (p0 < vx[k]) ? -1 : ((p0 < vx[k + 1] && p0 >= vx[k]) ? 0 : 1);
     int yo = (p1 < vy[j]) ? -1 : ((p1 < vy[j + 1] && p1 >= vy[j]) ? 0 : 1);
     k += xo; j += yo;
-    T.pos[i] = p0; T.pos[T.n + i] = p1;
-    T.vel[i] = u0; T.vel[T.n + i] = u1;
-    T.idx[i] = k; T.idy[i] = j;
-    if (k < 2 || k > g.CW - 3 || j < 2 || j > g.CH - 3) T.alive[i] = 0;
+    Tr.pos[i] = p0; Tr.pos[Tr.n + i] = p1;
+    Tr.vel[i] = u0; Tr.vel[Tr.n + i] = u1;
+    Tr.idx[i] = k; Tr.idy[i] = j;
+    if (k < 2 || k > g.CW - 3 || j < 2 || j > g.CH - 3) Tr.alive[i] = 0;
 }
 void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const Tracers &T,
                     const double *vx, const double *vy, const DevTime *tm)
@@ -787,31 +942,32 @@ void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params 
 /* ------------------------------------------------ calculate_flux_mixed -- */
 /* src/advect.c:861-1136 with _calculate_flux_set_bphi (:1138-1224).  The
  * donor writes the packet of each of its outflow faces (swept volume < 0). */
-template <int NM>
-__device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, double *bx, double *by, int k, int j)
+template <class S, int NM, class TT>
+__device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const TT &T, S *bx, S *by, int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g);
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
-    double px[4], py[4], dx[4], dy[4];
+    double px[4], py[4];
+    S dx[4], dy[4];
     for (int i = 0; i < 4; ++i) {
         px[i] = VPL(V_POS, 0, vv[i]); py[i] = VPL(V_POS, 1, vv[i]);
         dx[i] = VPL(V_DIS, 0, vv[i]); dy[i] = VPL(V_DIS, 1, vv[i]);
     }
-    double lv = flux_volcyl(px[0], py[0], px[3], py[3], dx[0], dy[0], dx[3], dy[3]);
-    double rv = flux_volcyl(px[2], py[2], px[1], py[1], dx[2], dy[2], dx[1], dy[1]);
-    double bv = flux_volcyl(px[1], py[1], px[0], py[0], dx[1], dy[1], dx[0], dy[0]);
-    double uv = flux_volcyl(px[3], py[3], px[2], py[2], dx[3], dy[3], dx[2], dy[2]);
-    double lbru[4] = {lv, bv, rv, uv};
-    double ev[Mats<NM>::cap];
+    S lv = flux_volcyl(px[0], py[0], px[3], py[3], dx[0], dy[0], dx[3], dy[3]);
+    S rv = flux_volcyl(px[2], py[2], px[1], py[1], dx[2], dy[2], dx[1], dy[1]);
+    S bv = flux_volcyl(px[1], py[1], px[0], py[0], dx[1], dy[1], dx[0], dy[0]);
+    S uv = flux_volcyl(px[3], py[3], px[2], py[2], dx[3], dy[3], dx[2], dy[2]);
+    S lbru[4] = {lv, bv, rv, uv};
+    S ev[Mats<NM>::cap];
     for (int m = 0; m < n; ++m) ev[m] = CPL(M_VOF, m, c);
     double evol = CPL(E_VOL, 0, c);
-    double fv[Mats<NM>::cap][4];
+    S fv[Mats<NM>::cap][4];
     int pm = 0;
-    double evpm = ev[0];
+    S evpm = ev[0];
     if (ev[0] < TOLVOF)
         for (int q = 1; q < n; ++q) if (ev[q] > evpm) { pm = q; evpm = ev[q]; }
-    double mlv = lv < 0 ? lv : 0, mrv = rv < 0 ? rv : 0, mbv = bv < 0 ? bv : 0, muv = uv < 0 ? uv : 0;
+    S mlv = lv < 0 ? lv : S(0), mrv = rv < 0 ? rv : S(0), mbv = bv < 0 ? bv : S(0), muv = uv < 0 ? uv : S(0);
     if (evpm > 1.0 - TOLVOF) {
         for (int q = 0; q < n; ++q) {
             if (q == pm) { fv[q][0] = mlv; fv[q][2] = -1.0 * mrv; fv[q][1] = mbv; fv[q][3] = -1.0 * muv; }
@@ -822,26 +978,26 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, double
     } else {
         double gr[8];
         for (int i = 0; i < 8; ++i) gr[i] = CPL(E_GRAD, i, c);
-        double g0 = 0, g1 = 0;
+        S g0 = 0, g1 = 0;
         for (int i = 0; i < 4; ++i) g0 += VPL(V_VOF, pm, vv[i]) * gr[i];
         for (int i = 0; i < 4; ++i) g1 += VPL(V_VOF, pm, vv[i]) * gr[4 + i];
         vnorm2(g0, g1);
-        double ca = CPL(E_POS, 2, c) * fabs(g0), cb = CPL(E_POS, 3, c) * fabs(g1);
-        if (ca < cb) { double t = ca; ca = cb; cb = t; }
-        double dstar = cut_length(evpm, ca, cb);
-        double w[4];
-        for (int i = 0; i < 4; ++i) { double z = 0; z += g0 * px[i]; z += g1 * py[i]; w[i] = z; }
-        double mw = w[0] > w[1] ? w[0] : w[1];
-        double mw2 = w[2] > w[3] ? w[2] : w[3];
+        S ca = CPL(E_POS, 2, c) * fabs(g0), cb = CPL(E_POS, 3, c) * fabs(g1);
+        if (ca < cb) { S t = ca; ca = cb; cb = t; }
+        S dstar = cut_length(evpm, ca, cb);
+        S w[4];
+        for (int i = 0; i < 4; ++i) { S z = 0; z += g0 * px[i]; z += g1 * py[i]; w[i] = z; }
+        S mw = w[0] > w[1] ? w[0] : w[1];
+        S mw2 = w[2] > w[3] ? w[2] : w[3];
         mw = mw > mw2 ? mw : mw2;
         for (int i = 0; i < 4; ++i) w[i] = w[i] - mw + dstar;
-        for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = (pm == 0) ? w[i] : -0.25;
-        double fp[4];
+        for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = (pm == 0) ? w[i] : S(-0.25);
+        S fp[4];
         fp[0] = geo_ratio(w[3], w[0]);
         fp[2] = geo_ratio(w[2], w[1]);
         fp[1] = geo_ratio(w[0], w[1]);
         fp[3] = geo_ratio(w[3], w[2]);
-        double cc = (lv < 0.0 ? lv : 0.0) + mrv + mbv + muv;
+        S cc = (lv < 0.0 ? lv : S(0.0)) + mrv + mbv + muv;
         cc = fabs(cc) / evol;
         long nb[4] = {k - 1 + (long)g.CW * j, k + (long)g.CW * (j - 1), k + 1 + (long)g.CW * j, k + (long)g.CW * (j + 1)};
         bool nbok[4] = {k - 1 >= 0, j - 1 >= 0, k + 1 < g.CW, j + 1 < g.CH};
@@ -850,19 +1006,19 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, double
         bn[2][0] = F.f[SALEC_BX_NORMAL][k + 1 + (long)g.BXW * j]; bn[2][1] = F.f[SALEC_BX_NORMAL][g.NBX + k + 1 + (long)g.BXW * j];
         bn[1][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * j]; bn[1][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * j];
         bn[3][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * (j + 1)]; bn[3][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * (j + 1)];
-        double fnp[Mats<NM>::cap][4];
-        double snp[4] = {0., 0., 0., 0.};
+        S fnp[Mats<NM>::cap][4];
+        S snp[4] = {0., 0., 0., 0.};
         for (int m = 0; m < n; ++m) fnp[m][0] = fnp[m][1] = fnp[m][2] = fnp[m][3] = 0.;
         for (int m = 1; m < n; ++m) {
             if (pm == m) continue;
-            double h0 = 0, h1 = 0;
+            S h0 = 0, h1 = 0;
             for (int i = 0; i < 4; ++i) h0 += VPL(V_VOF, m, vv[i]) * gr[i];
             for (int i = 0; i < 4; ++i) h1 += VPL(V_VOF, m, vv[i]) * gr[4 + i];
             vnorm2(h0, h1);
-            double nbv[4];
-            for (int i = 0; i < 4; ++i) nbv[i] = nbok[i] ? CPL(M_VOF, m, nb[i]) : ev[m];
-            double up[4] = {nbv[0], nbv[1], nbv[2], nbv[3]};
-            double ac[4] = {nbv[2], nbv[3], nbv[0], nbv[1]};
+            S nbv[4];
+            for (int i = 0; i < 4; ++i) nbv[i] = nbok[i] ? S(CPL(M_VOF, m, nb[i])) : ev[m];
+            S up[4] = {nbv[0], nbv[1], nbv[2], nbv[3]};
+            S ac[4] = {nbv[2], nbv[3], nbv[0], nbv[1]};
             for (int i = 0; i < 4; ++i) {
                 fnp[m][i] = alg_ratio(up[i], ev[m], ac[i], cc, bn[i][0], bn[i][1], h0, h1);
                 snp[i] += fnp[m][i];
@@ -885,11 +1041,11 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, double
         }
     }
     for (int q = 1; q < n; ++q) {
-        double z = 0;
+        S z = 0;
         z += fv[q][0] * 1.0; z += fv[q][1] * 1.0; z += fv[q][2] * -1.0; z += fv[q][3] * -1.0;
-        double cvv = -1.0 * z;
+        S cvv = -1.0 * z;
         if (cvv > ev[q] * evol) {
-            double sc = ev[q] * evol / cvv;
+            S sc = ev[q] * evol / cvv;
             for (int i = 0; i < 4; ++i) fv[q][i] *= sc;
         }
     }
@@ -898,14 +1054,14 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, double
     long fi[4] = {k + (long)g.BXW * j, k + (long)g.BYW * j, k + 1 + (long)g.BXW * j, k + (long)g.BYW * (j + 1)};
     for (int i = 0; i < 4; ++i) {
         if (lbru[i] >= 0.) continue;
-        double *ph = (i == 0 || i == 2) ? bx : by;
+        S *ph = (i == 0 || i == 2) ? bx : by;
         long st = (i == 0 || i == 2) ? g.NBX : g.NBY;
         for (int q = 0; q < n; ++q) ph[(long)q * st + fi[i]] = fv[q][i];
     }
 }
 /* ghost ring (distance < 2): rows 0,1,CH-2,CH-1 and columns 0,1,CW-2,CW-1 */
-template <int NM>
-__global__ void __launch_bounds__(128) k_flux_ring(Grid g, Fields F, double *bx, double *by)
+template <class S, int NM>
+__global__ void __launch_bounds__(128) k_flux_ring(Grid g, Fields F, TF<S> T, S *bx, S *by)
 {
     long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long nrow = 4L * g.CW, ncol = 4L * g.ny;
@@ -922,13 +1078,20 @@ __global__ void __launch_bounds__(128) k_flux_ring(Grid g, Fields F, double *bx,
     } else {
         return;
     }
-    flux_cell<NM>(g, F, bx, by, k, j);
+    flux_cell<S, NM>(g, F, T, bx, by, k, j);
 }
 void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by)
 {
     nm_dispatch(g.nmat, [&](auto I) {
         constexpr int NM = decltype(I)::value;
-        k_flux_ring<NM><<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, bx, by);
+        k_flux_ring<double, NM><<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, NoTan{}, bx, by);
+    });
+}
+void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, Dual *bx, Dual *by)
+{
+    nm_dispatch(g.nmat, [&](auto I) {
+        constexpr int NM = decltype(I)::value;
+        k_flux_ring<Dual, NM><<<nblk(4L * g.CW + 4L * g.ny, 128), 128, 0, s>>>(g, F, T, bx, by);
     });
 }
 
@@ -949,59 +1112,60 @@ void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, double *bx,
  *  - ghost-ring donors pack after that fill in the reference too.
  * A face carries flux iff its swept volume is non-zero; the donor is the
  * side on which it is negative (exactly antisymmetric, src/advect.c:1347). */
-#define CPN(fld, c, i) (Fn.f[SALEC_##fld][(long)(c) * g.NC + (i)])
-template <int NM>
-__device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
-                                            const double *bx, const double *by, const double *top, int k, int j)
+#define CPN(fld, c, i) fa<S, SALEC_##fld>(Fn.f, Tn, (long)(c) * g.NC + (i))
+template <class S, int NM, class TT>
+__device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const TT &T, const Fields &Fn, const TT &Tn,
+                                            const Params &P, const S *bx, const S *by, const S *top, int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g), PL = 3 * n + 10;
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
-    double px[4], py[4], dx[4], dy[4];
+    double px[4], py[4];
+    S dx[4], dy[4];
     for (int i = 0; i < 4; ++i) {
         px[i] = VPL(V_POS, 0, vv[i]); py[i] = VPL(V_POS, 1, vv[i]);
         dx[i] = VPL(V_DIS, 0, vv[i]); dy[i] = VPL(V_DIS, 1, vv[i]);
     }
-    double sv[4];
+    S sv[4];
     sv[0] = flux_volcyl(px[0], py[0], px[3], py[3], dx[0], dy[0], dx[3], dy[3]);
     sv[1] = flux_volcyl(px[1], py[1], px[0], py[0], dx[1], dy[1], dx[0], dy[0]);
     sv[2] = flux_volcyl(px[2], py[2], px[1], py[1], dx[2], dy[2], dx[1], dy[1]);
     sv[3] = flux_volcyl(px[3], py[3], px[2], py[2], dx[3], dy[3], dx[2], dy[2]);
     long fi[4] = {k + (long)g.BXW * j, k + (long)g.BYW * j, k + 1 + (long)g.BXW * j, k + (long)g.BYW * (j + 1)};
     const double dir[4] = {1.0, 1.0, -1.0, -1.0};
-    double dphi[3 * Mats<NM>::cap + 10];
+    S dphi[3 * Mats<NM>::cap + 10];
     for (int q = 0; q < PL; ++q) dphi[q] = 0.0;
     const long nb[4] = {c - 1, c - g.CW, c + 1, c + g.CW};
     const int nk[4] = {k - 1, k, k + 1, k}, nj[4] = {j, j - 1, j, j + 1};
     for (int i = 0; i < 4; ++i) {
         if (!(sv[i] < 0. || sv[i] > 0.)) continue;
-        const double *ph = (i == 0 || i == 2) ? bx : by;
+        const S *ph = (i == 0 || i == 2) ? bx : by;
         long st = (i == 0 || i == 2) ? g.NBX : g.NBY;
         long o = fi[i];
         const bool self = sv[i] < 0.;
         const long d = self ? c : nb[i];
         const int kd = self ? k : nk[i], jd = self ? j : nj[i];
         const bool snap = jd == g.CH - 3 && kd >= 2 && kd < g.CW - 2;
-        double tail[10];
+        S tail[10];
         tail[0] = CPL(E_VEL, 0, d); tail[1] = CPL(E_VEL, 1, d);
         for (int q = 0; q < 4; ++q) tail[2 + q] = CPL(E_STE, q, d);
         tail[6] = CPL(E_DAM, 0, d); tail[7] = CPL(E_TPS, 0, d);
         tail[8] = CPL(E_VIB, 0, d); tail[9] = CPL(E_EJT, 0, d);
         for (int m = 0; m < n; ++m) {
-            double mv = dir[i] * ph[(long)m * st + o];
-            double dm = snap ? top[(long)m * g.CW + kd] : CPL(M_DEN, m, d);
-            double de = snap ? top[(long)(n + m) * g.CW + kd] : CPL(M_ENG, m, d);
+            S mv = dir[i] * ph[(long)m * st + o];
+            S dm = snap ? top[(long)m * g.CW + kd] : S(CPL(M_DEN, m, d));
+            S de = snap ? top[(long)(n + m) * g.CW + kd] : S(CPL(M_ENG, m, d));
             dphi[m] += mv;
             dphi[m + n] += mv * dm;
             dphi[m + 2 * n] += mv * de;
-            double pa = mv * dm;
+            S pa = mv * dm;
             for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * tail[q];
         }
     }
     double evol = CPL(E_VOL, 0, c);
-    double emass = evol * CPL(E_DEN, 0, c);
-    double sphi[3 * Mats<NM>::cap + 10];
-    double ev[Mats<NM>::cap], md[Mats<NM>::cap], me[Mats<NM>::cap];
+    S emass = evol * CPL(E_DEN, 0, c);
+    S sphi[3 * Mats<NM>::cap + 10];
+    S ev[Mats<NM>::cap], md[Mats<NM>::cap], me[Mats<NM>::cap];
     for (int m = 0; m < n; ++m) {
         ev[m] = CPL(M_VOF, m, c); md[m] = CPL(M_DEN, m, c); me[m] = CPL(M_ENG, m, c);
         sphi[m] = 0.0 + ev[m] * evol;
@@ -1015,7 +1179,7 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     sphi[3 * n + 8] = emass * CPL(E_VIB, 0, c);
     sphi[3 * n + 9] = emass * CPL(E_EJT, 0, c);
     for (int q = 0; q < PL; ++q) sphi[q] += 1.0 * dphi[q];
-    double Z = 0.0;
+    S Z = 0.0;
     emass = 0.;
     for (int m = 0; m < n; ++m) {
         if (sphi[m] < 0.) { sphi[m] = 0.; sphi[m + n] = 0.; }
@@ -1024,14 +1188,14 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
         emass += sphi[m + n];
     }
     if (Z > evol * (1.0 - TOLVOF) || (Z > 0. && sphi[0] < evol * TOLVOF)) {
-        double s1 = 1.0 / Z;
+        S s1 = 1.0 / Z;
         for (int m = 0; m < n; ++m) ev[m] = s1 * sphi[m];
         ev[0] = 0.0;
     } else if (Z < evol * TOLVOF) {
         for (int m = 0; m < n; ++m) ev[m] = 0.0;
         ev[0] = 1.0;
     } else if (P.partpres) {
-        double s1 = 1.0 / (Z + sphi[0]);
+        S s1 = 1.0 / (Z + sphi[0]);
         for (int m = 0; m < n; ++m) ev[m] = s1 * sphi[m];
     } else {
         sphi[0] = evol - Z;
@@ -1046,9 +1210,9 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
             md[m] = sphi[m + n] / (ev[m] * evol);
         }
     }
-    double dv0, dv1, ste[4], dam, tps, vib, ejt;
+    S dv0, dv1, ste[4], dam, tps, vib, ejt;
     if (emass > TOLRHO * evol) {
-        double s1 = 1.0 / emass;
+        S s1 = 1.0 / emass;
         dv0 = s1 * sphi[3 * n]; dv1 = s1 * sphi[3 * n + 1];
         for (int q = 0; q < 4; ++q) ste[q] = s1 * sphi[3 * n + 2 + q];
         dam = sphi[3 * n + 6] / emass; tps = sphi[3 * n + 7] / emass;
@@ -1057,7 +1221,7 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
         dv0 = dv1 = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
         dam = tps = vib = ejt = 0.;
     }
-    double eden = emass / evol;
+    S eden = emass / evol;
     tps = dWind(tps, 0.0, 10.0);
     dam = dWind(dam, 0.0, 1.0);
     ejt = dWind(ejt, 0.0, 1.0);
@@ -1070,14 +1234,14 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     for (int m = 0; m < n; ++m) { CPL(M_VOF, m, c) = ev[m]; CPN(M_DEN, m, c) = md[m]; CPN(M_ENG, m, c) = me[m]; }
     CPL(E_DVEL, 0, c) = dv0; CPL(E_DVEL, 1, c) = dv1;
     for (int q = 0; q < 4; ++q) CPN(E_STE, q, c) = ste[q];
-    CPN(E_DAM, 0, c) = dam; CPN(E_TPS, 0, c) = tps; CPN(E_VIB, 0, c) = vib; CPN(E_EJT, 0, c) = ejt;
+    CPN(E_DAM, 0, c) = dam; CPN(E_TPS, 0, c) = tps; CPN(E_VIB, 0, c) = vib; CPN(E_EJT, 0, c) = val(ejt);
     CPL(E_DEN, 0, c) = eden;
 }
 /* ---------------------------------------------------------- damp_on_cell -- */
 /* damp_on_cell_pml2 (src/velocity.c:487-588), _pml (:378-483), _sbc
  * (:688-767); all cells including the ghost ring; inactive once t >= damp_time */
-template <int NM>
-__device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const Params &P, double dt, long c)
+template <class S, int NM, class TT>
+__device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const TT &T, const Params &P, S dt, long c)
 {
     const int n = Mats<NM>::n(g);
     if (CPL(M_VOF, 0, c) > TOLVOF) return;
@@ -1087,51 +1251,51 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
         if (!(sy > 0.0 || sx > 0.0)) return;
         CPL(E_DVEL, 0, c) += -1.0 * (sx + sy) * CPL(E_VEL, 0, c) * dt;
         CPL(E_DVEL, 1, c) += -1.0 * (sx + sy) * CPL(E_VEL, 1, c) * dt;
-        double g0 = CPL(E_GRAD_VEL, 0, c), g1 = CPL(E_GRAD_VEL, 1, c), g2 = CPL(E_GRAD_VEL, 2, c);
-        double g3 = CPL(E_GRAD_VEL, 3, c), g4 = CPL(E_GRAD_VEL, 4, c);
-        double a0 = CPL(E_ALPHA, 0, c), a1 = CPL(E_ALPHA, 1, c);
+        S g0 = CPL(E_GRAD_VEL, 0, c), g1 = CPL(E_GRAD_VEL, 1, c), g2 = CPL(E_GRAD_VEL, 2, c);
+        S g3 = CPL(E_GRAD_VEL, 3, c), g4 = CPL(E_GRAD_VEL, 4, c);
+        S a0 = CPL(E_ALPHA, 0, c), a1 = CPL(E_ALPHA, 1, c);
         a0 += (sy * g0 + sy * g4 - ady * a0) * dt + (sx * g3 + sx * g4 - adx * a0) * dt;
-        double lp = CPL(E_PRE, 0, c) + CPL(E_Q, 0, c) - CPL(E_DAMPREF, 2, c);
-        double sd = (CPL(E_STE, 0, c) - lp) * sy * g0 + CPL(E_STE, 1, c) * (sy * g1 + sx * g2)
-                  + (CPL(E_STE, 2, c) - lp) * sx * g3 + (sy + sx) * CPL(E_STE, 3, c) * g4;
+        S lp = CPL(E_PRE, 0, c) + CPL(E_Q, 0, c) - CPL(E_DAMPREF, 2, c);
+        S sd = (CPL(E_STE, 0, c) - lp) * sy * g0 + CPL(E_STE, 1, c) * (sy * g1 + sx * g2)
+             + (CPL(E_STE, 2, c) - lp) * sx * g3 + (sy + sx) * CPL(E_STE, 3, c) * g4;
         a1 += -1.0 * sd * dt - 1.0 * (adx + ady) * a1 * dt;
         CPL(E_ALPHA, 0, c) = a0; CPL(E_ALPHA, 1, c) = a1;
         for (int m = 1; m < n; ++m) {
             if (CPL(M_VOF, m, c) < 1.0 - TOLVOF) continue;
             double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
-            double dd = (-(sx + sy) * (CPL(M_DEN, m, c) - r0) - r0 * a0) * dt;
-            double de = (-(sx + sy) * (CPL(M_ENG, m, c) - r1) - a1) * dt;
+            S dd = (-(sx + sy) * (CPL(M_DEN, m, c) - r0) - r0 * a0) * dt;
+            S de = (-(sx + sy) * (CPL(M_ENG, m, c) - r1) - a1) * dt;
             CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
             CPL(M_ENG, m, c) += de / CPL(M_PTY, m, c);
             break;
         }
     } else if (P.damp_type == SALEC_DAMP_PML) {
         if (sy > 0.0) {
-            double a1 = CPL(E_ALPHA, 1, c);
+            S a1 = CPL(E_ALPHA, 1, c);
             a1 += (sy * CPL(E_GRAD_VEL, 0, c) + sy * CPL(E_GRAD_VEL, 4, c) - ady * a1) * dt;
             CPL(E_ALPHA, 1, c) = a1;
             CPL(E_DVEL, 1, c) += -1.0 * sy * CPL(E_VEL, 1, c) * dt;
             for (int m = 1; m < n; ++m) {
                 if (CPL(M_VOF, m, c) < 1.0 - TOLVOF) continue;
                 double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
-                double dd = (-sy * (CPL(M_DEN, m, c) - r0) - r0 * a1) * dt;
-                double de = (-sy * (CPL(M_ENG, m, c) - r1) - CPL(E_PRE, 0, c) * a1) * dt;
+                S dd = (-sy * (CPL(M_DEN, m, c) - r0) - r0 * a1) * dt;
+                S de = (-sy * (CPL(M_ENG, m, c) - r1) - CPL(E_PRE, 0, c) * a1) * dt;
                 CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
                 CPL(M_ENG, m, c) += de / CPL(M_PTY, m, c);
                 break;
             }
         }
         if (sx > 0.0) {
-            double a0 = CPL(E_ALPHA, 0, c);
+            S a0 = CPL(E_ALPHA, 0, c);
             a0 += (sx * CPL(E_GRAD_VEL, 3, c) + sx * CPL(E_GRAD_VEL, 4, c) - adx * a0) * dt;
             CPL(E_ALPHA, 0, c) = a0;
             CPL(E_DVEL, 0, c) += -1.0 * sx * CPL(E_VEL, 0, c) * dt;
-            double as = a0 + CPL(E_ALPHA, 1, c);
+            S as = a0 + CPL(E_ALPHA, 1, c);
             for (int m = 1; m < n; ++m) {
                 if (CPL(M_VOF, m, c) > 1.0 - TOLVOF) {
                     double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
-                    double dd = (-sx * (CPL(M_DEN, m, c) - r0) - r0 * as) * dt;
-                    double de = (-sx * (CPL(M_ENG, m, c) - r1) - CPL(E_PRE, 0, c) * as) * dt;
+                    S dd = (-sx * (CPL(M_DEN, m, c) - r0) - r0 * as) * dt;
+                    S de = (-sx * (CPL(M_ENG, m, c) - r1) - CPL(E_PRE, 0, c) * as) * dt;
                     CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
                     CPL(M_ENG, m, c) += de / CPL(M_PTY, m, c);
                     break;
@@ -1143,7 +1307,7 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
             for (int m = 1; m < n; ++m) {
                 if (CPL(M_VOF, m, c) < TOLVOF) continue;
                 double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
-                double dd = CPL(M_DEN, m, c) - r0, de = CPL(M_ENG, m, c) - r1;
+                S dd = CPL(M_DEN, m, c) - r0, de = CPL(M_ENG, m, c) - r1;
                 CPL(M_DEN, m, c) += dd * sy + r0;
                 CPL(M_ENG, m, c) += de * sy + r1;
             }
@@ -1151,7 +1315,7 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
             for (int m = 1; m < n; ++m) {
                 if (CPL(M_VOF, m, c) < TOLVOF) continue;
                 double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
-                double dd = CPL(M_DEN, m, c) - r0, de = CPL(M_ENG, m, c) - r1;
+                S dd = CPL(M_DEN, m, c) - r0, de = CPL(M_ENG, m, c) - r1;
                 CPL(M_DEN, m, c) = dd * sx + r0;
                 CPL(M_ENG, m, c) = de * sx * sx + r1;
             }
@@ -1159,7 +1323,8 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
 }
 /* ---------------------------------------------------------- ejecta_flow -- */
 /* src/ejecta.c:65-134 */
-__device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, const Params &P, double dt, int k, int j)
+template <class S, class TT>
+__device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, const TT &T, const Params &P, S dt, int k, int j)
 {
     long c = k + (long)g.CW * j;
     long v0 = k - 1 + (long)g.VW * (j - 1), v1 = k + (long)g.VW * (j - 1);
@@ -1168,7 +1333,7 @@ __device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, cons
     gy += 0.5 * VPL(V_BF, 1, v1);
     if (CPL(E_POS, 1, c) <= F.f[SALEC_Y_PROFILE][k] && CPL(E_EJT, 0, c) > 0.1) {
         if (P.flow_resistance > 0.) {
-            double pa = -P.flow_resistance * dt;
+            S pa = -P.flow_resistance * dt;
             CPL(E_DVEL, 0, c) += pa * CPL(E_VEL, 0, c);
             CPL(E_DVEL, 1, c) += pa * CPL(E_VEL, 1, c);
         }
@@ -1181,20 +1346,21 @@ __device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, cons
 }
 /* --------------------------------------------------------- update_state -- */
 /* src/sale2d.c:1635-1742 */
-template <int NM>
-__device__ __forceinline__ void state_cell(const Grid &g, const Fields &F, const Params &P, const DevMat *mats, int k, int j)
+template <class S, int NM, class TT>
+__device__ __forceinline__ void state_cell(const Grid &g, const Fields &F, const TT &T, const Params &P, const DevMat *mats,
+                                           int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g);
-    double dam = CPL(E_DAM, 0, c);
-    double pre = 0.0, csd = 0.0, tem = 0.0, den = 0.0, mu = 0.0;
+    S dam = CPL(E_DAM, 0, c);
+    S pre = 0.0, csd = 0.0, tem = 0.0, den = 0.0, mu = 0.0;
     for (int m = 1; m < n; ++m) {
-        double vf = CPL(M_VOF, m, c), md = CPL(M_DEN, m, c);
-        double mp = 0., mc = 1., mt = 0.;
+        S vf = CPL(M_VOF, m, c), md = CPL(M_DEN, m, c);
+        S mp = 0., mc = 1., mt = 0.;
         if (!(vf < TOLVOF)) {
             const DevMat &r = mats[m];
-            double pmin = (1.0 - dam) * r.minPint + dam * r.minPdam;
-            eos_state_capped(r, md, CPL(M_ENG, m, c), pmin, CPL(M_PTY, m, c), &mt, &mp, &mc);
+            S pmin = (1.0 - dam) * r.minPint + dam * r.minPdam;
+            eos_state_capped<S>(r, md, CPL(M_ENG, m, c), pmin, CPL(M_PTY, m, c), &mt, &mp, &mc);
         }
         pre += vf * mp;
         csd += vf / dMax(1.0, mc);
@@ -1204,7 +1370,7 @@ __device__ __forceinline__ void state_cell(const Grid &g, const Fields &F, const
     }
     if (mu > 0.) mu = 1.0 / mu;
     if (csd > 0.) csd = 1.0 / csd;
-    double v0 = CPL(M_VOF, 0, c);
+    S v0 = CPL(M_VOF, 0, c);
     if ((P.partpres == 0 && v0 > TOLVOF) || (P.partpres == 1 && v0 > 1.0 - TOLVOF)) {
         pre = 0.; csd = 1.0; mu = 0.;
     } else if (P.partpres == 1) {
@@ -1217,7 +1383,7 @@ __global__ void k_update_state(Grid g, Fields F, Params P, const DevMat *mats)
 {
     int k, j;
     if (!physical_cell(g, xcd_thread(), k, j)) return;
-    state_cell<NM>(g, F, P, mats, k, j);
+    state_cell<double, NM>(g, F, NoTan{}, P, mats, k, j);
 }
 void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats)
 {
@@ -1240,14 +1406,14 @@ void launch_advance_time(hipStream_t s, DevTime *tm) { k_advance_time<<<1, 64, 0
  * a cell's flux reads only its own energy, so the per-cell chain gives the
  * reference's values.  Cells of the top physical row also snapshot their
  * m_den and m_eng (see advect_cell). */
-template <int NM>
-__global__ void __launch_bounds__(128) k_energy_flux(Grid g, Fields F, const DevMat *mats, const DevTime *tm,
-                                                     double *bx, double *by, double *top)
+template <class S, int NM>
+__global__ void __launch_bounds__(128) k_energy_flux(Grid g, Fields F, TF<S> T, const DevMat *mats, const DevTime *tm,
+                                                     S *bx, S *by, S *top)
 {
     int k, j;
     if (!physical_cell(g, xcd_thread(), k, j)) return;
-    energy_cell<NM>(g, F, mats, tm->dt, k, j);
-    flux_cell<NM>(g, F, bx, by, k, j);
+    energy_cell<S, NM>(g, F, T, mats, tm_dt<S>(tm), k, j);
+    flux_cell<S, NM>(g, F, T, bx, by, k, j);
     if (j == g.CH - 3) {
         const int n = Mats<NM>::n(g);
         long c = k + (long)g.CW * j;
@@ -1261,7 +1427,15 @@ void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const Dev
                         double *bx, double *by, double *top)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_energy_flux<decltype(I)::value><<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, mats, tm, bx, by, top);
+        k_energy_flux<double, decltype(I)::value><<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, NoTan{}, mats, tm,
+                                                                                                 bx, by, top);
+    });
+}
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const DevMat *mats,
+                        const DevTime *tm, Dual *bx, Dual *by, Dual *top)
+{
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_energy_flux<Dual, decltype(I)::value><<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, T, mats, tm, bx, by, top);
     });
 }
 
@@ -1273,9 +1447,9 @@ void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const Dev
  * e_vib, e_ejt) are written to the second buffer set Fn, ring cells copying
  * theirs across, and the caller swaps the two sets afterwards.  Each phase
  * otherwise touches only its own cell. */
-template <int NM>
-__global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, Fields Fn, Params P, const DevMat *mats,
-                                                     const double *bx, const double *by, const double *top,
+template <class S, int NM>
+__global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, TF<S> T, Fields Fn, TF<S> Tn, Params P,
+                                                     const DevMat *mats, const S *bx, const S *by, const S *top,
                                                      const DevTime *tm)
 {
     long c = xcd_thread();
@@ -1283,26 +1457,36 @@ __global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, Fields Fn
     int k = (int)(c % g.CW), j = (int)(c / g.CW);
     bool phys = k >= 2 && k < g.CW - 2 && j >= 2 && j < g.CH - 2;
     const int n = Mats<NM>::n(g);
-    const double dt = tm->dt, t = tm->t;
+    const S dt = tm_dt<S>(tm);
+    const double t = tm->t;
     if (phys) {
-        advect_cell<NM>(g, F, Fn, P, bx, by, top, k, j);
+        advect_cell<S, NM>(g, F, T, Fn, Tn, P, bx, by, top, k, j);
     } else {
         for (int m = 0; m < n; ++m) { CPN(M_DEN, m, c) = CPL(M_DEN, m, c); CPN(M_ENG, m, c) = CPL(M_ENG, m, c); }
         for (int q = 0; q < 4; ++q) CPN(E_STE, q, c) = CPL(E_STE, q, c);
         CPN(E_DAM, 0, c) = CPL(E_DAM, 0, c); CPN(E_TPS, 0, c) = CPL(E_TPS, 0, c);
         CPN(E_VIB, 0, c) = CPL(E_VIB, 0, c); CPN(E_EJT, 0, c) = CPL(E_EJT, 0, c);
     }
-    if (!(P.damp_time <= t)) damp_cell<NM>(g, Fn, P, dt, c);
+    if (!(P.damp_time <= t)) damp_cell<S, NM>(g, Fn, Tn, P, dt, c);
     if (phys) {
-        if (P.flow_resistance > 0. || P.flow_friction > 0.) ejecta_cell(g, Fn, P, dt, k, j);
-        state_cell<NM>(g, Fn, P, mats, k, j);
+        if (P.flow_resistance > 0. || P.flow_friction > 0.) ejecta_cell<S>(g, Fn, Tn, P, dt, k, j);
+        state_cell<S, NM>(g, Fn, Tn, P, mats, k, j);
     }
 }
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
                         const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_remap_close<decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
+        k_remap_close<double, decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, NoTan{}, Fn, NoTan{}, P, mats,
+                                                                                    bx, by, top, tm);
+    });
+}
+void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Fields &Fn,
+                        const TFields &Tn, const Params &P, const DevMat *mats, const Dual *bx, const Dual *by,
+                        const Dual *top, const DevTime *tm)
+{
+    nm_dispatch(g.nmat, [&](auto I) {
+        k_remap_close<Dual, decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
     });
 }
 
@@ -1312,6 +1496,7 @@ void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fie
 #define DR_BS 256
 __global__ void __launch_bounds__(DR_BS) k_dampref(Grid g, Fields F, double *part)
 {
+    PRIMAL_SCOPE;
     __shared__ double red[DR_BS / 64];
     long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double ldt = 0.;
@@ -1348,6 +1533,7 @@ void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part,
 /* init_v_field, src/sale2d.c:1897-1956 */
 __global__ void k_init_v_field(Grid g, Fields F)
 {
+    PRIMAL_SCOPE;
     int k, j;
     if (!active_vertex(g, xcd_thread(), k, j)) return;
     long v = k + (long)g.VW * j;

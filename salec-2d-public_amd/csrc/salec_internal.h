@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/salec.h"
+#include "dual.h"
 
 /* rows per partial of the crater-profile column minimum */
 #define PROF_CHUNK 8
@@ -27,6 +28,7 @@ struct AxisHint {
 struct DevMat {
     int eos_type, yfunc, dfunc, tfunc;
     int till_nden, tab_nden, tab_ntem, PolyLiq, PolySol, pad_;
+    int sens_y0, sens_fr;           /* 1 + tangent direction of Yint0 / Yfricint, 0 = none */
     double TLRho0, TLCv, TLA, TLB, TLE0, TLa, TLb, TLAlpha, TLBeta, TLEiv, TLEcv, TLTref;
     double till_stepden;
     const double *till_yeng, *tab_yden, *tab_xtem, *tab_data;
@@ -55,13 +57,16 @@ struct Params {
 
 struct Fields { double *f[SALEC_NFIELDS]; };
 
-struct DevTime { double dt; double t; long long step; double pad; };
+/* dtd: derivatives of dt (sensitivity runs) */
+struct DevTime { double dt; double t; long long step; double pad; double dtd[NTAN]; double pad2; };
 
 /* ghost / restriction program: dst <- mode(src), applied to every component
  * of a field slot; modes: 0 x, 1 -x, 2 |x|, 3 -|x|, 4 +0, 5 -0 */
 enum { GM_COPY = 0, GM_NEG, GM_ABS, GM_NABS, GM_ZERO, GM_NZERO };
 struct GhostOp { int slot; int dst; int src; uint8_t m0, m1, mr, pad; };
 struct GhostSlots { double *base[4]; long plane[4]; int ncomp[4]; };
+/* a program's slots on the tangent planes (pbase: the values, for |x|) */
+struct TanSlots { const double *pbase[4]; double *tbase[4]; long plane[4]; long tst[4]; int ncomp[4]; };
 
 struct Tracers {
     long n;
@@ -96,6 +101,26 @@ void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fie
 void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part, int nblocks);
 int  dampref_blocks(const Grid &g);
 void launch_init_v_field(hipStream_t s, const Grid &g, const Fields &F);
+/* sensitivity runs: the step kernels on Dual values and tangent planes */
+void launch_ghost_tan(hipStream_t s, const TanSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase);
+void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P);
+void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T);
+void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T);
+void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, DevTime *tm,
+                       Dual *part, int nblocks);
+void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, const DevMat *mats,
+                   const DevTime *tm);
+void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, const DevTime *tm);
+void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const DevTime *tm);
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const DevMat *mats,
+                        const DevTime *tm, Dual *bx, Dual *by, Dual *top);
+void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, Dual *bx, Dual *by);
+void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Fields &Fn,
+                        const TFields &Tn, const Params &P, const DevMat *mats, const Dual *bx, const Dual *by,
+                        const Dual *top, const DevTime *tm);
+void launch_sens_accum(hipStream_t s, const Grid &g, const Fields &F, double *acc, int row);
+void launch_sens_accum(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, Dual *acc, int row);
+void launch_scale_into(hipStream_t s, const double *p, double *t, long n, double f);
 /* non-finite scan: writes first bad flat index per field into out */
 void launch_nonfinite(hipStream_t s, const double *p, long n, unsigned long long *out);
 

@@ -133,6 +133,10 @@ def lib():
     L.salec_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.salec_batch_step.argtypes = [vp, C.c_int]
     L.salec_batch_destroy.argtypes = [vp]
+    L.salec_sens_begin.argtypes = [vp, C.c_uint, C.c_uint, C.c_int, C.c_double]
+    L.salec_sens_step.argtypes = [vp, C.c_int]
+    L.salec_sens_result.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.salec_sens_end.argtypes = [vp]
     L.salec_checkpoint.argtypes = [vp, C.c_char_p]
     L.salec_restore.argtypes = [vp, C.c_char_p]
     L.salec_write_tillotson.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int]
@@ -310,6 +314,21 @@ class Device:
     def sync(self):
         check(lib().salec_sync(self.ctx), "salec_sync")
 
+    def sens_begin(self, params, mat_mask, surface_row, impact_speed=0.0):
+        check(lib().salec_sens_begin(self.ctx, params, mat_mask, surface_row, impact_speed), "salec_sens_begin")
+
+    def sens_step(self, nsteps):
+        check(lib().salec_sens_step(self.ctx, int(nsteps)), "salec_sens_step")
+
+    def sens_result(self):
+        """(loss, [dL/dcohesion, dL/dfriction, dL/dspeed])"""
+        L, g = C.c_double(), (C.c_double * 3)()
+        check(lib().salec_sens_result(self.ctx, C.byref(L), g), "salec_sens_result")
+        return L.value, list(g)
+
+    def sens_end(self):
+        check(lib().salec_sens_end(self.ctx), "salec_sens_end")
+
     def upload_tracers(self, tr):
         check(lib().salec_upload_tracers(self.ctx, C.byref(tr.struct())), "salec_upload_tracers")
 
@@ -373,6 +392,63 @@ def crater_metrics(y_profile, col_r, nx):
     d, D = C.c_double(), C.c_double()
     lib().salec_crater_metrics(_dp(y), _dp(r), nx, C.byref(d), C.byref(D))
     return d.value, D.value
+
+
+SALEC_SENS_COHESION, SALEC_SENS_FRICTION, SALEC_SENS_VELOCITY = 1, 2, 4
+
+
+class CraterSensitivity:
+    """The crater-depth proxy L of a prepared run as a function of the
+    intact cohesion, the intact friction coefficient and the impact speed,
+    and its gradient (salec_sens_*).  L is the sum over the vertices of the
+    target-surface row of (sum over `steps` steps of v_dis_z)^2.  Cohesion
+    and friction are set on every material in `mats` (default: all but
+    vacuum); the speed scales the initial velocity field."""
+
+    def __init__(self, host, steps, mats=None):
+        self.h = host
+        self.steps = int(steps)
+        self.st0 = host.state()
+        cfg = host.cfg
+        self.mats = list(mats) if mats is not None else list(range(1, cfg.nmat))
+        self.mask = sum(1 << m for m in self.mats)
+        vy = self.st0["V_POS"][1][:, 2]
+        self.row = int(np.argmin(np.abs(vy)))
+        ev = self.st0["E_VEL"]
+        self.speed0 = float(np.hypot(ev[0], ev[1]).max())
+        self.cohesion0 = float(cfg.mat[self.mats[0]].Yint0)
+        self.friction0 = float(cfg.mat[self.mats[0]].Yfricint)
+
+    def run(self, cohesion=None, friction=None, speed=None, params=7):
+        """(loss, grad[3]); params = 0 evaluates the loss with the plain step."""
+        cohesion = self.cohesion0 if cohesion is None else cohesion
+        friction = self.friction0 if friction is None else friction
+        speed = self.speed0 if speed is None else speed
+        cfg = self.h.cfg
+        saved = [(cfg.mat[m].Yint0, cfg.mat[m].Yfricint) for m in self.mats]
+        try:
+            for m in self.mats:
+                cfg.mat[m].Yint0 = cohesion
+                cfg.mat[m].Yfricint = friction
+            dev = Device(cfg)
+        finally:
+            for m, (y, f) in zip(self.mats, saved):
+                cfg.mat[m].Yint0 = y
+                cfg.mat[m].Yfricint = f
+        try:
+            st = dict(self.st0)
+            st["E_VEL"] = self.st0["E_VEL"] * (speed / self.speed0)
+            dev.upload_state(st)
+            cfg.damp_time = dev.init(self.h.damp_t_in)
+            dev.set_time(self.h.time0, self.h.dt0)
+            dev.sens_begin(params, self.mask, self.row, speed)
+            dev.sens_step(self.steps)
+            out = dev.sens_result()
+            self.last_state = dev.state(["M_VOF", "M_DEN", "M_ENG", "V_VEL", "E_STE", "E_DAM"])
+            dev.sens_end()
+            return out
+        finally:
+            dev.close()
 
 
 class Batch:

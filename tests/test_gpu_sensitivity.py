@@ -1,0 +1,143 @@
+"""Parameter sensitivities of the step (salec_sens_*), through the C ABI.
+
+The sensitivity run executes the step kernels on dual numbers: values are
+computed by the same operations as the plain step and derivatives are those
+of the computed values with every branch taken as the values take it.  The
+checks:
+
+* the value part of a sensitivity run equals the plain run bit for bit
+  (loss and state), so the gradient belongs to exactly the trajectory the
+  solver computes;
+* each derivative agrees with central finite differences of the plain run.
+  The discrete loss is only piecewise smooth: volume-fraction cutoffs, the
+  yield surface, table cells and the dt minimum switch branches at isolated
+  parameter values, and a finite difference whose interval crosses a switch
+  measures the jump, not the derivative.  The check therefore scans the
+  relative FD step over 1e-4 ... 1e-7 and requires agreement to 1e-6 at the
+  best step (where the interval crosses no switch and rounding is still
+  small); the errors at every step are printed.  The window is 50 steps from
+  contact on the c1 physics at 64 x 96 cells, long enough that all three
+  parameters change the loss well above rounding;
+* a torch.autograd.Function wrapper (forward: run the steps, backward: the
+  stored gradient) gives the same gradient through torch.autograd.grad and
+  passes torch's own finite-difference gradcheck on the smooth piece.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import pysalec as ps
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = (("cohesion", 0), ("friction", 1), ("speed", 2))
+
+
+def say(msg):
+    print("[gpu-sens] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def crater():
+    say("c1 physics 64x96, 50 steps from contact: setup")
+    d = tempfile.mkdtemp(prefix="sens_")
+    h = ps.HostRun(*ps.write_config(os.path.join(d, "c1"), "c1", nx=64, ny=96))
+    return ps.CraterSensitivity(h, 50)
+
+
+def test_values_equal_plain_run(crater):
+    L0, g0 = crater.run(params=0)
+    s0 = crater.last_state
+    L1, g1 = crater.run(params=7)
+    s1 = crater.last_state
+    say("loss plain %.17g, with tangents %.17g; grad %s" % (L0, L1, g1))
+    assert L0 == L1 and L0 > 0.
+    assert g0 == [0.0, 0.0, 0.0]
+    for k in s0:
+        assert np.array_equal(s0[k], s1[k], equal_nan=True), k
+    assert all(np.isfinite(g1)) and all(g != 0.0 for g in g1)
+
+
+def _fd(crater, name, h):
+    p0 = getattr(crater, name + "0")
+    Lp, _ = crater.run(params=0, **{name: p0 + h})
+    Lm, _ = crater.run(params=0, **{name: p0 - h})
+    return (Lp - Lm) / (2.0 * h)
+
+
+@pytest.mark.parametrize("name,idx", PARAMS, ids=[p[0] for p in PARAMS])
+def test_gradient_matches_central_differences(crater, name, idx):
+    _, g = crater.run(params=7)
+    p0 = getattr(crater, name + "0")
+    errs = []
+    for rel in (1e-4, 1e-5, 1e-6, 1e-7):
+        fd = _fd(crater, name, rel * p0)
+        errs.append((rel, fd, abs(fd - g[idx]) / abs(g[idx])))
+    say("%-8s AD %.12e | %s" % (name, g[idx], "  ".join("h=%.0e FD %.10e err %.1e" % e for e in errs)))
+    assert min(e[2] for e in errs) <= 1e-6
+
+
+def test_single_parameter_seeds_are_independent(crater):
+    """Seeding one direction gives the same derivative as seeding all three."""
+    _, gall = crater.run(params=7)
+    for name, idx in PARAMS:
+        _, g = crater.run(params=1 << idx)
+        assert g[idx] == gall[idx], name
+        assert all(g[q] == 0.0 for q in range(3) if q != idx), name
+
+
+def test_torch_autograd_harness(crater):
+    torch = pytest.importorskip("torch")
+
+    class CraterLoss(torch.autograd.Function):
+        """loss(cohesion, friction, speed) of the 50-step window; backward
+        uses the gradient carried by the dual-number run."""
+
+        @staticmethod
+        def forward(ctx, p):
+            L, g = crater.run(cohesion=float(p[0]), friction=float(p[1]), speed=float(p[2]), params=7)
+            ctx.save_for_backward(torch.tensor(g, dtype=torch.float64))
+            return torch.tensor(L, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gout):
+            (g,) = ctx.saved_tensors
+            return gout * g
+
+    p = torch.tensor([crater.cohesion0, crater.friction0, crater.speed0], dtype=torch.float64, requires_grad=True)
+    L = CraterLoss.apply(p)
+    (gt,) = torch.autograd.grad(L, p)
+    _, g = crater.run(params=7)
+    assert gt.tolist() == g
+    # torch's own check on the speed direction with a step inside the smooth piece
+    q = torch.tensor([crater.cohesion0, crater.friction0, crater.speed0], dtype=torch.float64, requires_grad=True)
+    f = lambda x: CraterLoss.apply(torch.stack([q[0].detach(), q[1].detach(), x[0]]))
+    x0 = torch.tensor([crater.speed0], dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(f, (x0,), eps=1e-6 * crater.speed0, atol=0.0, rtol=1e-6)
+
+
+def test_sensitivity_calls_out_of_order(crater):
+    h = crater.h
+    dev = ps.Device(h.cfg)
+    try:
+        dev.upload_state(crater.st0)
+        dev.init(h.damp_t_in)
+        dev.set_time(h.time0, h.dt0)
+        with pytest.raises(RuntimeError):
+            dev.sens_step(1)                      # no run open
+        with pytest.raises(RuntimeError):
+            dev.sens_begin(8, 6, crater.row, 1.0)  # unknown parameter bit
+        with pytest.raises(RuntimeError):
+            dev.sens_begin(4, 6, crater.row, 0.0)  # speed direction needs the speed
+        dev.sens_begin(7, 6, crater.row, crater.speed0)
+        with pytest.raises(RuntimeError):
+            dev.step()                            # plain steps refused while tangents are open
+        dev.sens_step(2)
+        dev.sens_end()
+        dev.step()
+    finally:
+        dev.close()
