@@ -1447,10 +1447,18 @@ void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFi
  * e_vib, e_ejt) are written to the second buffer set Fn, ring cells copying
  * theirs across, and the caller swaps the two sets afterwards.  Each phase
  * otherwise touches only its own cell. */
+#ifndef REMAP_WAVES
+#define REMAP_WAVES 0
+#endif
+#if REMAP_WAVES > 0
+#define REMAP_ATTR __attribute__((amdgpu_waves_per_eu(REMAP_WAVES)))
+#else
+#define REMAP_ATTR
+#endif
 template <class S, int NM>
-__global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, TF<S> T, Fields Fn, TF<S> Tn, Params P,
-                                                     const DevMat *mats, const S *bx, const S *by, const S *top,
-                                                     const DevTime *tm)
+__device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F, const TF<S> &T, const Fields &Fn,
+                                                 const TF<S> &Tn, const Params &P, const DevMat *mats, const S *bx,
+                                                 const S *by, const S *top, const DevTime *tm)
 {
     long c = xcd_thread();
     if (c >= g.NC) return;
@@ -1473,12 +1481,27 @@ __global__ void __launch_bounds__(128) k_remap_close(Grid g, Fields F, TF<S> T, 
         state_cell<S, NM>(g, Fn, Tn, P, mats, k, j);
     }
 }
+/* the double kernel may be given an occupancy target; the dual-number one
+ * keeps the compiler's own register budget */
+template <int NM>
+__global__ void __launch_bounds__(128) REMAP_ATTR k_remap_close(Grid g, Fields F, Fields Fn, Params P, const DevMat *mats,
+                                                                const double *bx, const double *by, const double *top,
+                                                                const DevTime *tm)
+{
+    remap_close_cell<double, NM>(g, F, NoTan{}, Fn, NoTan{}, P, mats, bx, by, top, tm);
+}
+template <int NM>
+__global__ void __launch_bounds__(128) k_remap_close_t(Grid g, Fields F, TFields T, Fields Fn, TFields Tn, Params P,
+                                                       const DevMat *mats, const Dual *bx, const Dual *by, const Dual *top,
+                                                       const DevTime *tm)
+{
+    remap_close_cell<Dual, NM>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
+}
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
                         const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_remap_close<double, decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, NoTan{}, Fn, NoTan{}, P, mats,
-                                                                                    bx, by, top, tm);
+        k_remap_close<decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
     });
 }
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Fields &Fn,
@@ -1486,7 +1509,7 @@ void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFi
                         const Dual *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_remap_close<Dual, decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
+        k_remap_close_t<decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
     });
 }
 
