@@ -3,22 +3,34 @@
  * the reference's main() (src/salec2d.c:11-295) on one GPU:
  *
  *   salec2d <file.inp> <eos_dir> [--steps N] [--log K] [--state out.bin]
+ *           [--out DIR] [--no-output]
  *
  * Setup runs on the host (salec_host_*), the initialisation tail and the
  * time loop on the device (salec_init, salec_step).  The loop stops under the
  * rules of update_cycle_control (src/sale2d.c:71-136): after maxstep+1 steps
- * or once t > time1 + 2 maxdt (or after --steps N steps).  Output writers are
- * not part of this driver: the crater profile (y_profile) is downloaded every
- * K steps to report the transient crater depth and diameter, and --state
- * writes the final fields as raw FP64 planes in field-id order.
+ * or once t > time1 + 2 maxdt (or after --steps N steps).
+ *
+ * Output follows the reference's schedule and files: the first step and then
+ * every output.time of simulated time (and the first output.init steps, and
+ * every output.out_cycle steps), at the mid-step output point
+ * (src/salec2d.c:241-245), it writes DIR/vts/<prefix>.proc0000.NNNN.vts and
+ * DIR/vtp/<prefix>.tracer.proc0000.NNNN.vtp with their .pvts, .pvtp and .vtm
+ * indexes in DIR, and once after tracer placement DIR/txt/mass.tracer.
+ * proc0000.txt.  Directories are created when missing, never emptied.
+ * The crater profile (y_profile) is downloaded every K steps to report the
+ * transient crater depth and diameter, and --state writes the final fields as
+ * raw FP64 planes in field-id order.
  */
+#include <errno.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 #include <time.h>
 
 #include "../../include/salec.h"
+#include "../../include/salec_output.h"
 #include "setup.h"
 
 static double now_s(void)
@@ -34,6 +46,56 @@ static int die(const char *what, int rc)
     return 1;
 }
 
+static int ensure_dir(const char *d)
+{
+    if (mkdir(d, 0755) == 0 || errno == EEXIST) return SALEC_OK;
+    fprintf(stderr, "salec2d: cannot create %s\n", d);
+    return SALEC_EINVAL;
+}
+
+/* fields a state file reads (point data, cell data, coordinates) */
+static const int kOutFields[] = {
+    SALEC_V_VEL, SALEC_V_ACC, SALEC_V_BF, SALEC_V_VOF, SALEC_V_POS, SALEC_E_DEN, SALEC_E_PRE, SALEC_E_WPT,
+    SALEC_M_VOF, SALEC_E_CSD, SALEC_E_DAM, SALEC_E_TPS, SALEC_E_Q, SALEC_E_TEM, SALEC_M_ENG, SALEC_M_DEN,
+    SALEC_E_STE2S, SALEC_E_STA2S, SALEC_E_STE, SALEC_E_STRENGTH, SALEC_E_VIB, SALEC_E_EJT, SALEC_E_CVS, SALEC_M_PTY};
+
+/* one output (sale2d_write_vtk + sale2d_write_vtp and their indexes) of the
+ * current device state into dir; host fields of h serve as buffers */
+static int write_output(salec_ctx *ctx, salec_host *h, salec_tracers *tr, const char *dir, const char *prefix,
+                        const char *format, int cycle, int nx, int ny, int nmat)
+{
+    int rc;
+    const double *tab[SALEC_NFIELDS] = {0};
+    for (unsigned i = 0; i < sizeof kOutFields / sizeof kOutFields[0]; ++i) {
+        int f = kOutFields[i];
+        if ((rc = salec_download(ctx, f, salec_host_field(h, f))) != SALEC_OK) return rc;
+        tab[f] = salec_host_field(h, f);
+    }
+    size_t need = strlen(dir) + 2 * strlen(prefix) + 64;
+    char *p = malloc(need);
+    if (!p) return SALEC_ENOMEM;
+    snprintf(p, need, "%s/vts/%s.proc%04d.%04d.vts", dir, prefix, 0, cycle);
+    rc = salec_write_vts(p, nx, ny, nmat, tab, format);
+    if (rc == SALEC_OK) {
+        snprintf(p, need, "%s/%s.%04d.pvts", dir, prefix, cycle);
+        rc = salec_write_pvts(p, prefix, cycle, nx, ny, nmat);
+    }
+    if (rc == SALEC_OK) rc = salec_write_vtm(dir, prefix, cycle, 1);
+    if (rc == SALEC_OK && tr->n > 0) {
+        long cap = tr->n;
+        if ((rc = salec_download_tracers(ctx, tr)) != SALEC_OK) { free(p); return rc; }
+        snprintf(p, need, "%s/vtp/%s.tracer.proc%04d.%04d.vtp", dir, prefix, 0, cycle);
+        rc = salec_write_vtp(p, NULL, nx, ny, tr, tab[SALEC_E_DEN], tab[SALEC_E_TEM], tab[SALEC_E_PRE]);
+        if (rc == SALEC_OK) {
+            snprintf(p, need, "%s/%s.tracer.%04d.pvtp", dir, prefix, cycle);
+            rc = salec_write_pvtp(p, prefix, cycle, 1);
+        }
+        tr->n = cap;
+    }
+    free(p);
+    return rc;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 3) {
@@ -41,11 +103,14 @@ int main(int argc, char **argv)
         return 2;
     }
     long max_steps = -1, log_every = 100;
-    const char *state_out = NULL;
+    const char *state_out = NULL, *out_dir = ".";
+    int output = 1;
     for (int a = 3; a < argc; ++a) {
         if (!strcmp(argv[a], "--steps") && a + 1 < argc) max_steps = atol(argv[++a]);
         else if (!strcmp(argv[a], "--log") && a + 1 < argc) log_every = atol(argv[++a]);
         else if (!strcmp(argv[a], "--state") && a + 1 < argc) state_out = argv[++a];
+        else if (!strcmp(argv[a], "--out") && a + 1 < argc) out_dir = argv[++a];
+        else if (!strcmp(argv[a], "--no-output")) output = 0;
         else { fprintf(stderr, "salec2d: unknown argument %s\n", argv[a]); return 2; }
     }
     if (log_every < 1) log_every = 1;
@@ -62,6 +127,18 @@ int main(int argc, char **argv)
     const double time0 = cyc[0], time1 = cyc[1], maxdt = cyc[2], dt0 = cyc[3];
     const long maxstep = (long)cyc[4];
     const int nx = cfg.nx, ny = cfg.ny, nmat = cfg.nmat;
+    const char *prefix, *format;
+    double out_time;
+    int init_step, out_cycle;
+    salec_host_output(h, &prefix, &format, &out_time, &init_step, &out_cycle);
+    if (output) {
+        char d[4096];
+        const char *sub[] = {"", "/vts", "/vtp", "/txt"};
+        for (int i = 0; i < 4; ++i) {
+            snprintf(d, sizeof d, "%s%s", out_dir, sub[i]);
+            if (ensure_dir(d) != SALEC_OK) return 1;
+        }
+    }
     cfg.max_tracers = salec_host_count_tracers(h);
 
     salec_ctx *ctx = NULL;
@@ -87,6 +164,14 @@ int main(int argc, char **argv)
         for (unsigned k = 0; k < sizeof ip / sizeof ip[0]; ++k) *ip[k] = calloc(ntr, sizeof(int));
         if ((rc = salec_host_init_tracers(h, &tr)) != SALEC_OK) return die("tracer init", rc);
         if ((rc = salec_upload_tracers(ctx, &tr)) != SALEC_OK) return die("salec_upload_tracers", rc);
+        if (output) {                    /* export_init_tracers, src/sale2d.c:2093-2126 */
+            double *eden = salec_host_field(h, SALEC_E_DEN);
+            if ((rc = salec_download(ctx, SALEC_E_DEN, eden)) != SALEC_OK) return die("salec_download", rc);
+            char p[4096];
+            snprintf(p, sizeof p, "%s/txt/mass.tracer.proc%04d.txt", out_dir, 0);
+            if ((rc = salec_write_mass_tracers(p, nx, ny, &tr, eden, salec_host_field(h, SALEC_E_VOL))) != SALEC_OK)
+                return die("tracer export", rc);
+        }
     }
     if ((rc = salec_set_time(ctx, time0, dt0)) != SALEC_OK) return die("salec_set_time", rc);
 
@@ -103,11 +188,27 @@ int main(int argc, char **argv)
     salec_stats st = {0};
     double best_depth = -INFINITY, best_diam = 0.0, best_t = 0.0;
     long k = 0;
+    int out_vts = 1, k_out = 1, i_out = 0, n_out = 0;   /* src/salec2d.c:98-99 */
+    double t_out = 0.0;
     for (;;) {
         /* the stop rule needs t after every step, so each step returns its stats */
-        rc = salec_step(ctx, 0u, &st);
-        if (rc != SALEC_OK) return die("salec_step", rc);
+        if (output && out_vts) {
+            if ((rc = salec_step(ctx, SALEC_STEP_TO_OUTPUT, &st)) != SALEC_OK) return die("salec_step", rc);
+            double t0w = now_s();
+            rc = write_output(ctx, h, &tr, out_dir, prefix, format, k_out - 1, nx, ny, nmat);
+            if (rc != SALEC_OK) return die("output", rc);
+            t_out += now_s() - t0w;
+            ++n_out;
+            if ((rc = salec_step(ctx, SALEC_STEP_FROM_OUTPUT, &st)) != SALEC_OK) return die("salec_step", rc);
+        } else {
+            if ((rc = salec_step(ctx, 0u, &st)) != SALEC_OK) return die("salec_step", rc);
+        }
         ++k;
+        /* output state of the next step, update_cycle_control (src/sale2d.c:96-117) */
+        if (st.t >= out_time * (k_out - i_out)) { ++k_out; out_vts = 1; }
+        else if (k <= init_step) { ++k_out; ++i_out; out_vts = 1; }
+        else if (out_cycle > 0 && k % out_cycle == 0) { ++k_out; ++i_out; out_vts = 1; }
+        else out_vts = 0;
         int stop = (k > maxstep) || (st.t > time1 + 2.0 * maxdt) || (max_steps >= 0 && k >= max_steps);
         if (k % log_every == 0 || stop) {
             if ((rc = salec_download(ctx, SALEC_Y_PROFILE, yprof)) != SALEC_OK) return die("salec_download", rc);
@@ -126,6 +227,7 @@ int main(int argc, char **argv)
     if (rc != SALEC_OK) fprintf(stderr, "salec2d: non-finite %s at %ld\n", salec_field_name(bf), bi);
     printf("salec2d: %ld steps in %.3f s (%.4e cell-steps/s); max transient depth %.2f m, diameter %.2f m at t %.4e\n",
            k, t_end - t_loop, (double)nx * ny * k / (t_end - t_loop), best_depth, best_diam, best_t);
+    if (output) printf("salec2d: %d outputs (%s) in %s, %.3f s of writing\n", n_out, prefix, out_dir, t_out);
 
     if (state_out) {
         FILE *fp = fopen(state_out, "wb");

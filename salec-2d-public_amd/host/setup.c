@@ -44,6 +44,10 @@ struct salec_host {
     /* cycle */
     double time0, time1, maxdt, dt0;
     int maxstep;
+    /* output (init_cycle_control, src/sale2d.c:6-24; output.format, src/sale2d_back.c:212) */
+    char out_prefix[100], out_format[16];
+    double out_time;
+    int out_init, out_cycle;
     /* projectile */
     int p_mat;
     double p_radius, p_center[2], p_vel[2], p_dam, p_pre, p_tem;
@@ -561,6 +565,11 @@ int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out,
     h->time0 = inp_d(f, "cycle.time0", 0.0); h->time1 = inp_d(f, "cycle.time1", 100.0);
     h->maxdt = inp_d(f, "cycle.maxdt", 1.0); h->dt0 = inp_d(f, "cycle.dt0", 0.0);
     h->maxstep = inp_i(f, "cycle.maxstep", 200);
+    inp_s(f, "output.prefix", h->out_prefix, sizeof h->out_prefix, "ParaTest");
+    inp_s(f, "output.format", h->out_format, sizeof h->out_format, "binary");
+    h->out_time = inp_d(f, "output.time", 1.0);
+    h->out_init = inp_i(f, "output.init", 0);
+    h->out_cycle = inp_i(f, "output.out_cycle", -1);
     c->max_dt = h->maxdt;
     /* materials */
     c->nmat = inp_i(f, "material.nm", 2);
@@ -730,6 +739,16 @@ void salec_host_free(salec_host *h)
 const salec_config *salec_host_config(const salec_host *h) { return &h->cfg; }
 double *salec_host_field(salec_host *h, int field) { return (field >= 0 && field < SALEC_NFIELDS) ? h->f[field] : NULL; }
 void salec_host_set_damp_time(salec_host *h, double t) { h->cfg.damp_time = t; }
+void salec_host_output(const salec_host *h, const char **prefix, const char **format, double *out_time,
+                       int *init_step, int *out_cycle)
+{
+    if (prefix) *prefix = h->out_prefix;
+    if (format) *format = h->out_format;
+    if (out_time) *out_time = h->out_time;
+    if (init_step) *init_step = h->out_init;
+    if (out_cycle) *out_cycle = h->out_cycle;
+}
+
 void salec_host_cycle(const salec_host *h, double out[6])
 {
     out[0] = h->time0; out[1] = h->time1; out[2] = h->maxdt; out[3] = h->dt0; out[4] = h->maxstep; out[5] = h->damp_t_in;

@@ -34,6 +34,9 @@ const salec_config *salec_host_config(const salec_host *h);
 double *salec_host_field(salec_host *h, int field);
 /* cycle.{time0,time1,maxdt,dt0,maxstep}, numerical damp time (input) */
 void salec_host_cycle(const salec_host *h, double out[6]);
+/* output.{prefix,format,time,init,out_cycle} (strings owned by h) */
+void salec_host_output(const salec_host *h, const char **prefix, const char **format, double *out_time,
+                       int *init_step, int *out_cycle);
 /* Number of tracers fl_tracer_init would create (strip from the .inp). */
 long salec_host_count_tracers(const salec_host *h);
 /* Fill tracer arrays (capacity tr->n on entry, count on return); uses the

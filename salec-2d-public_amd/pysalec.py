@@ -137,6 +137,15 @@ def lib():
     L.salec_sens_step.argtypes = [vp, C.c_int]
     L.salec_sens_result.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.salec_sens_end.argtypes = [vp]
+    dpp = C.POINTER(C.POINTER(C.c_double))
+    L.salec_write_vts.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, dpp, C.c_char_p]
+    L.salec_write_pvts.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.salec_write_vtm.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int]
+    L.salec_write_vtp.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(Tracers),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.salec_write_pvtp.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int]
+    L.salec_write_mass_tracers.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(Tracers),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.salec_checkpoint.argtypes = [vp, C.c_char_p]
     L.salec_restore.argtypes = [vp, C.c_char_p]
     L.salec_write_tillotson.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int]
@@ -395,6 +404,47 @@ def crater_metrics(y_profile, col_r, nx):
 
 
 SALEC_SENS_COHESION, SALEC_SENS_FRICTION, SALEC_SENS_VELOCITY = 1, 2, 4
+
+
+def _field_table(state):
+    """pointer table indexed by field id for the writers (absent fields NULL)"""
+    keep, tab = [], (C.POINTER(C.c_double) * len(FIELDS))()
+    for name, arr in state.items():
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        keep.append(a)
+        tab[FIELD_ID[name]] = _dp(a)
+    return tab, keep
+
+
+def write_vts(path, state, nx, ny, nmat, coord_format="binary"):
+    """one VTK state file (salec_write_vts) from host planes by field name"""
+    tab, keep = _field_table(state)
+    check(lib().salec_write_vts(path.encode(), nx, ny, nmat, tab, coord_format.encode()), "salec_write_vts")
+
+
+def write_pvts(path, prefix, cycle, nx, ny, nmat):
+    check(lib().salec_write_pvts(path.encode(), prefix.encode(), cycle, nx, ny, nmat), "salec_write_pvts")
+
+
+def write_vtm(directory, prefix, cycle, nproc=1):
+    check(lib().salec_write_vtm(directory.encode(), prefix.encode(), cycle, nproc), "salec_write_vtm")
+
+
+def write_vtp(path, tr, nx, ny, e_den, e_tem, e_pre, txt_path=None):
+    """tracer file (salec_write_vtp) from a TracerSet and the cell fields"""
+    a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (e_den, e_tem, e_pre)]
+    check(lib().salec_write_vtp(path.encode(), txt_path.encode() if txt_path else None, nx, ny,
+                                C.byref(tr.struct()), _dp(a[0]), _dp(a[1]), _dp(a[2])), "salec_write_vtp")
+
+
+def write_pvtp(path, prefix, cycle, nproc=1):
+    check(lib().salec_write_pvtp(path.encode(), prefix.encode(), cycle, nproc), "salec_write_pvtp")
+
+
+def write_mass_tracers(path, tr, nx, ny, e_den, e_vol):
+    a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (e_den, e_vol)]
+    check(lib().salec_write_mass_tracers(path.encode(), nx, ny, C.byref(tr.struct()), _dp(a[0]), _dp(a[1])),
+          "salec_write_mass_tracers")
 
 
 class CraterSensitivity:
