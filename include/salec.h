@@ -38,7 +38,7 @@ extern "C" {
 #endif
 
 #define SALEC_MAXMAT 10   /* MAXMAT, src/sale2d_constant.h:305 */
-#define SALEC_ABI_VERSION 1
+#define SALEC_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define SALEC_OK            0
@@ -125,7 +125,8 @@ enum { SALEC_GRID_CELL = 0, SALEC_GRID_VERTEX = 1, SALEC_GRID_XFACE = 2,
     X(V_LAPLACE,  SALEC_GRID_VERTEX,  9)                                        \
     X(BX_NORMAL,  SALEC_GRID_XFACE,   2)                                        \
     X(BY_NORMAL,  SALEC_GRID_YFACE,   2)                                        \
-    X(Y_PROFILE,  SALEC_GRID_COLUMN,  1)
+    X(Y_PROFILE,  SALEC_GRID_COLUMN,  1)                                        \
+    X(E_BETA,     SALEC_GRID_CELL,    4)  /* pml3 flux auxiliaries (e_beta)  */
 
 #define SALEC_FIELD_ENUM_(name, grid, nc) SALEC_##name,
 enum { SALEC_FIELD_LIST(SALEC_FIELD_ENUM_) SALEC_NFIELDS };

@@ -1280,13 +1280,20 @@ static void clean_flux(oracle_state *s, const grid_t g)
     memset(s->by_phi, 0, sizeof(double) * PL * g.NBY);
 }
 
-/* advect_flux_default, src/advect.c:26-302 (porosity model off) */
-static void advect_flux_default(oracle_state *s, const salec_config *cfg, const grid_t g)
+/* advect_flux_default, src/advect.c:26-302 (porosity model off), and with
+ * pml3 = 1 advect_flux_pml3, src/advect.c:304-593: e_beta accumulation from
+ * the face packets (:366-383), its relaxation and coupling (:453-480), and
+ * the pml3 forms of the negative-volume, material and density cutoffs
+ * (:486-489, :516, :565) */
+static void advect_flux_default(oracle_state *s, const salec_config *cfg, const grid_t g, int pml3, double dt)
 {
     int n = g.nmat, PL = 3 * n + 10;
+    double adx = cfg->alpha_damp_x, ady = cfg->alpha_damp_y;
     for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
         if (!cell_physical(&g, k, j)) continue;
         long c = CI(k, j);
+        double sgx = CF(E_SIGMA, 0, c), sgy = CF(E_SIGMA, 1, c);
+        double pml_para[4] = {-sgy, -sgx, -sgy, -sgx};
         const double *ph[4] = {s->bx_phi + BXI(k, j), s->by_phi + BYI(k, j), s->bx_phi + BXI(k + 1, j), s->by_phi + BYI(k, j + 1)};
         long st[4] = {g.NBX, g.NBY, g.NBX, g.NBY};
         double dir[4] = {1.0, 1.0, -1.0, -1.0};
@@ -1303,6 +1310,12 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
                 dm += mv * PH(m + n);
                 double pa = mv * PH(m + n);
                 for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * PH(3 * n + q);
+                if (pml3 && (sgx > 0. || sgy > 0.)) {
+                    CF(E_BETA, 0, c) += mv * PH(m + n) * pml_para[i];
+                    CF(E_BETA, 1, c) += mv * PH(m + 2 * n) * pml_para[i];
+                    CF(E_BETA, 2, c) += mv * PH(m + n) * pml_para[i] * PH(3 * n + 0);
+                    CF(E_BETA, 3, c) += mv * PH(m + n) * pml_para[i] * PH(3 * n + 1);
+                }
             }
 #undef PH
         }
@@ -1320,10 +1333,24 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
         sphi[3 * n + 8] = emass * CF(E_VIB, 0, c);
         sphi[3 * n + 9] = emass * CF(E_EJT, 0, c);
         for (int q = 0; q < PL; ++q) sphi[q] += 1.0 * dphi[q];
+        if (pml3 && sgx + sgy > 0.) {
+            for (int q = 0; q < 4; ++q) CF(E_BETA, q, c) += -1.0 * (adx + ady) * CF(E_BETA, q, c) * dt;
+            if (sphi[VACUUM] < TOLVOF) {
+                sphi[3 * n + 0] += CF(E_BETA, 2, c) * dt;
+                sphi[3 * n + 1] += CF(E_BETA, 3, c) * dt;
+            }
+            for (int m = 1; m < n; ++m) {
+                if (sphi[m] > TOLVOF) {
+                    sphi[m + n] += CF(E_BETA, 0, c) * dt;
+                    sphi[m + 2 * n] += CF(E_BETA, 1, c) * dt;
+                    break;
+                }
+            }
+        }
         double Z = 0.0;
         emass = 0.;
         for (int m = 0; m < n; ++m) {
-            if (sphi[m] < 0.) { sphi[m] = 0.; sphi[m + n] = 0.; }
+            if (sphi[m] < 0. || (pml3 && sphi[m + n] < 0.)) { sphi[m] = 0.; sphi[m + n] = 0.; }
             if (VACUUM == m) continue;
             Z += sphi[m];
             emass += sphi[m + n];
@@ -1350,7 +1377,7 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
         double md[SALEC_MAXMAT], me[SALEC_MAXMAT];
         for (int m = 0; m < n; ++m) { md[m] = CF(M_DEN, m, c); me[m] = CF(M_ENG, m, c); }
         for (int m = 1; m < n; ++m) {
-            if (ev[m] < TOLVOF || sphi[m + n] < TOLRHO * TOLVOF * evol) {
+            if (ev[m] < TOLVOF || (!pml3 && sphi[m + n] < TOLRHO * TOLVOF * evol)) {
                 me[m] = 0.0; md[m] = 0.0; ev[m] = 0.0;
             } else {
                 me[m] = sphi[m + 2 * n] / (ev[m] * evol);
@@ -1372,7 +1399,7 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
         tps = Wind(tps, 0.0, 10.0);
         dam = Wind(dam, 0.0, 1.0);
         ejt = Wind(ejt, 0.0, 1.0);
-        if (eden < cfg->density_cutoff || eden < 10.0 * cfg->density_cutoff * (1.0 - ev[VACUUM])) {
+        if (eden < cfg->density_cutoff || (!pml3 && eden < 10.0 * cfg->density_cutoff * (1.0 - ev[VACUUM]))) {
             for (int m = 0; m < n; ++m) { ev[m] = 0.0; md[m] = 0.0; me[m] = 0.0; }
             ev[VACUUM] = 1.0;
             dv[0] = dv[1] = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
@@ -1386,8 +1413,11 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
     }
 }
 
-/* damp_on_cell_pml2, src/velocity.c:487-588 */
-static void damp_on_cell_pml2(oracle_state *s, const salec_config *cfg, const grid_t g, double dt, double t)
+/* damp_on_cell_pml2, src/velocity.c:487-588; with pml3 = 1
+ * damp_on_cell_pml3, src/velocity.c:591-686, which relaxes the first full
+ * material's density without the auxiliary term and without the distension
+ * division (:675-680) */
+static void damp_on_cell_pml2(oracle_state *s, const salec_config *cfg, const grid_t g, double dt, double t, int pml3)
 {
     if (cfg->damp_time <= t) return;
     int n = g.nmat;
@@ -1415,6 +1445,13 @@ static void damp_on_cell_pml2(oracle_state *s, const salec_config *cfg, const gr
             }
             for (int m = 1; m < n; ++m) {
                 if (CF(M_VOF, m, c) < 1.0 - TOLVOF) continue;
+                if (pml3) {
+                    double dd3 = (-(sx + sy) * (CF(M_DEN, m, c) - CF(E_DAMPREF, 0, c))) * dt;
+                    double de3 = (-(sx + sy) * (CF(M_ENG, m, c) - CF(E_DAMPREF, 1, c)) - CF(E_ALPHA, 1, c)) * dt;
+                    CF(M_DEN, m, c) += dd3;
+                    CF(M_ENG, m, c) += de3;
+                    break;
+                }
                 double dd = (-(sx + sy) * (CF(M_DEN, m, c) - CF(E_DAMPREF, 0, c)) - CF(E_DAMPREF, 0, c) * CF(E_ALPHA, 0, c)) * dt;
                 double de = (-(sx + sy) * (CF(M_ENG, m, c) - CF(E_DAMPREF, 1, c)) - CF(E_ALPHA, 1, c)) * dt;
                 CF(M_DEN, m, c) += dd / CF(M_PTY, m, c);
@@ -1754,10 +1791,11 @@ void oracle_step_from_output(oracle_state *s, const salec_config *cfg)
 {
     grid_t g = mkgrid(s);
     double dt = s->dt;
-    advect_flux_default(s, cfg, g);
+    advect_flux_default(s, cfg, g, cfg->damp_type == SALEC_DAMP_PML3, dt);
     switch (cfg->damp_type) {
     case SALEC_DAMP_PML: damp_on_cell_pml(s, cfg, g, dt, s->t); break;
-    case SALEC_DAMP_PML2: damp_on_cell_pml2(s, cfg, g, dt, s->t); break;
+    case SALEC_DAMP_PML2: damp_on_cell_pml2(s, cfg, g, dt, s->t, 0); break;
+    case SALEC_DAMP_PML3: damp_on_cell_pml2(s, cfg, g, dt, s->t, 1); break;
     case SALEC_DAMP_SBC: damp_on_cell_sbc(s, cfg, g, s->t); break;
     default: break;
     }

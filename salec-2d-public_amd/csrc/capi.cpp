@@ -462,7 +462,7 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     if (!cfg || !out) return SALEC_EINVAL;
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || cfg->nmat < 2 || cfg->nmat > SALEC_MAXMAT || !cfg->mat) return SALEC_EINVAL;
-    if (cfg->damp_type == SALEC_DAMP_PML3) return SALEC_EINVAL;   /* advect_flux_pml3 not provided */
+    if (cfg->damp_type < SALEC_DAMP_PML || cfg->damp_type > SALEC_DAMP_NONE) return SALEC_EINVAL;
     for (int m = 1; m < cfg->nmat; ++m) {
         const salec_material &mm = cfg->mat[m];
         if (mm.pfunc != SALEC_FN_NONE) return SALEC_EINVAL;      /* porosity model not provided */

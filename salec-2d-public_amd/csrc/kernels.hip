@@ -1115,10 +1115,22 @@ void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, const TFiel
 #define CPN(fld, c, i) fa<S, SALEC_##fld>(Fn.f, Tn, (long)(c) * g.NC + (i))
 template <class S, int NM, class TT>
 __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const TT &T, const Fields &Fn, const TT &Tn,
-                                            const Params &P, const S *bx, const S *by, const S *top, int k, int j)
+                                            const Params &P, const S *bx, const S *by, const S *top, S dt, int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g), PL = 3 * n + 10;
+    /* pml3 (advect_flux_pml3, src/advect.c:304-593): the flux terms of the
+     * damping layer accumulate into e_beta, which relaxes and feeds the
+     * mass, energy and momentum of the cell */
+    const bool pml3 = P.damp_type == SALEC_DAMP_PML3;
+    double sgx = 0., sgy = 0.;
+    S beta[4];
+    if (pml3) {
+        sgx = CPL(E_SIGMA, 0, c); sgy = CPL(E_SIGMA, 1, c);
+        for (int q = 0; q < 4; ++q) beta[q] = CPL(E_BETA, q, c);
+    }
+    const bool pmlc = pml3 && (sgx > 0. || sgy > 0.);
+    const double pp[4] = {-sgy, -sgx, -sgy, -sgx};
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
     double px[4], py[4];
     S dx[4], dy[4];
@@ -1160,6 +1172,12 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
             dphi[m + 2 * n] += mv * de;
             S pa = mv * dm;
             for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * tail[q];
+            if (pmlc) {
+                beta[0] += mv * dm * pp[i];
+                beta[1] += mv * de * pp[i];
+                beta[2] += mv * dm * pp[i] * tail[0];
+                beta[3] += mv * dm * pp[i] * tail[1];
+            }
         }
     }
     double evol = CPL(E_VOL, 0, c);
@@ -1179,10 +1197,26 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     sphi[3 * n + 8] = emass * CPL(E_VIB, 0, c);
     sphi[3 * n + 9] = emass * CPL(E_EJT, 0, c);
     for (int q = 0; q < PL; ++q) sphi[q] += 1.0 * dphi[q];
+    if (pml3) {
+        if (sgx + sgy > 0.) {
+            for (int q = 0; q < 4; ++q) beta[q] += -1.0 * (P.alpha_damp_x + P.alpha_damp_y) * beta[q] * dt;
+            if (sphi[0] < TOLVOF) {
+                sphi[3 * n] += beta[2] * dt;
+                sphi[3 * n + 1] += beta[3] * dt;
+            }
+            for (int m = 1; m < n; ++m)
+                if (sphi[m] > TOLVOF) {
+                    sphi[m + n] += beta[0] * dt;
+                    sphi[m + 2 * n] += beta[1] * dt;
+                    break;
+                }
+        }
+        for (int q = 0; q < 4; ++q) CPL(E_BETA, q, c) = beta[q];
+    }
     S Z = 0.0;
     emass = 0.;
     for (int m = 0; m < n; ++m) {
-        if (sphi[m] < 0.) { sphi[m] = 0.; sphi[m + n] = 0.; }
+        if (sphi[m] < 0. || (pml3 && sphi[m + n] < 0.)) { sphi[m] = 0.; sphi[m + n] = 0.; }
         if (m == 0) continue;
         Z += sphi[m];
         emass += sphi[m + n];
@@ -1203,7 +1237,7 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
         for (int m = 0; m < n; ++m) ev[m] = s1 * sphi[m];
     }
     for (int m = 1; m < n; ++m) {
-        if (ev[m] < TOLVOF || sphi[m + n] < TOLRHO * TOLVOF * evol) {
+        if (ev[m] < TOLVOF || (!pml3 && sphi[m + n] < TOLRHO * TOLVOF * evol)) {
             me[m] = 0.0; md[m] = 0.0; ev[m] = 0.0;
         } else {
             me[m] = sphi[m + 2 * n] / (ev[m] * evol);
@@ -1225,7 +1259,7 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     tps = dWind(tps, 0.0, 10.0);
     dam = dWind(dam, 0.0, 1.0);
     ejt = dWind(ejt, 0.0, 1.0);
-    if (eden < P.density_cutoff || eden < 10.0 * P.density_cutoff * (1.0 - ev[0])) {
+    if (eden < P.density_cutoff || (!pml3 && eden < 10.0 * P.density_cutoff * (1.0 - ev[0]))) {
         for (int m = 0; m < n; ++m) { ev[m] = 0.0; md[m] = 0.0; me[m] = 0.0; }
         ev[0] = 1.0;
         dv0 = dv1 = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
@@ -1247,7 +1281,9 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
     if (CPL(M_VOF, 0, c) > TOLVOF) return;
     double sx = CPL(E_SIGMA, 0, c), sy = CPL(E_SIGMA, 1, c);
     double adx = P.alpha_damp_x, ady = P.alpha_damp_y;
-    if (P.damp_type == SALEC_DAMP_PML2) {
+    if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3) {
+        /* pml2 and pml3 (damp_on_cell_pml3, src/velocity.c:591-686) differ
+         * only in the relaxation of the first full material */
         if (!(sy > 0.0 || sx > 0.0)) return;
         CPL(E_DVEL, 0, c) += -1.0 * (sx + sy) * CPL(E_VEL, 0, c) * dt;
         CPL(E_DVEL, 1, c) += -1.0 * (sx + sy) * CPL(E_VEL, 1, c) * dt;
@@ -1263,6 +1299,13 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
         for (int m = 1; m < n; ++m) {
             if (CPL(M_VOF, m, c) < 1.0 - TOLVOF) continue;
             double r0 = CPL(E_DAMPREF, 0, c), r1 = CPL(E_DAMPREF, 1, c);
+            if (P.damp_type == SALEC_DAMP_PML3) {
+                S dd = (-(sx + sy) * (CPL(M_DEN, m, c) - r0)) * dt;
+                S de = (-(sx + sy) * (CPL(M_ENG, m, c) - r1) - a1) * dt;
+                CPL(M_DEN, m, c) += dd;
+                CPL(M_ENG, m, c) += de;
+                break;
+            }
             S dd = (-(sx + sy) * (CPL(M_DEN, m, c) - r0) - r0 * a0) * dt;
             S de = (-(sx + sy) * (CPL(M_ENG, m, c) - r1) - a1) * dt;
             CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
@@ -1468,7 +1511,7 @@ __device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F,
     const S dt = tm_dt<S>(tm);
     const double t = tm->t;
     if (phys) {
-        advect_cell<S, NM>(g, F, T, Fn, Tn, P, bx, by, top, k, j);
+        advect_cell<S, NM>(g, F, T, Fn, Tn, P, bx, by, top, dt, k, j);
     } else {
         for (int m = 0; m < n; ++m) { CPN(M_DEN, m, c) = CPL(M_DEN, m, c); CPN(M_ENG, m, c) = CPL(M_ENG, m, c); }
         for (int q = 0; q < 4; ++q) CPN(E_STE, q, c) = CPL(E_STE, q, c);

@@ -560,7 +560,6 @@ int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out,
     else if (!strcasecmp(buf, "sbc")) { c->damp_type = SALEC_DAMP_SBC; h->lambda_x = Wind(h->lambda_x, 0.0, 0.5); h->lambda_y = Wind(h->lambda_y, 0.0, 0.5); }
     else if (!strcasecmp(buf, "ctlg") || !strcasecmp(buf, "rock")) c->damp_type = SALEC_DAMP_CTLG;
     else { c->damp_type = SALEC_DAMP_NONE; h->damp_x = h->damp_y = 0; h->lambda_x = h->lambda_y = 0.; h->damp_t_in = -1.0; h->damp_x_left = h->damp_y_top = 0; }
-    if (c->damp_type == SALEC_DAMP_PML3) { rc = SALEC_EINVAL; ERR("damping 'pml3' is not supported"); }
     /* cycle, src/sale2d.c:6-13 */
     h->time0 = inp_d(f, "cycle.time0", 0.0); h->time1 = inp_d(f, "cycle.time1", 100.0);
     h->maxdt = inp_d(f, "cycle.maxdt", 1.0); h->dt0 = inp_d(f, "cycle.dt0", 0.0);

@@ -24,6 +24,7 @@ FIELDS = [
     ("V_VEL", 1, 2), ("V_MASS", 1, 2), ("V_ALPHA", 1, 2), ("V_VOF", 1, -1), ("V_ACC", 1, 2),
     ("V_DIS", 1, 2), ("V_POS", 1, 2), ("V_BF", 1, 2), ("V_VOL", 1, 5), ("V_SUBVOL", 1, 4),
     ("V_SUF", 1, 8), ("V_LAPLACE", 1, 9), ("BX_NORMAL", 2, 2), ("BY_NORMAL", 3, 2), ("Y_PROFILE", 4, 1),
+    ("E_BETA", 0, 4),
 ]
 FIELD_ID = {name: i for i, (name, _, _) in enumerate(FIELDS)}
 NFIELDS = len(FIELDS)
@@ -31,7 +32,7 @@ NFIELDS = len(FIELDS)
 # persistent state carried from step to step (reference section 2.2)
 PERSISTENT = ["M_VOF", "M_DEN", "M_ENG", "E_STE", "E_DAM", "E_TPS", "E_VIB", "E_EJT", "E_STE2S",
               "E_DVEL", "E_ALPHA", "E_WPT", "E_DEN", "E_PRE", "E_CSD", "E_TEM", "E_MU", "E_CVS",
-              "V_VEL", "V_MASS", "V_ALPHA"]
+              "V_VEL", "V_MASS", "V_ALPHA", "E_BETA"]
 TRANSIENT = ["E_VEL", "E_GRAD_VEL", "E_DIV_VEL", "E_Q", "E_STA2S", "E_STRENGTH", "V_VOF", "V_ACC",
              "V_DIS", "Y_PROFILE"]
 

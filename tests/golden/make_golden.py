@@ -7,8 +7,8 @@ The reference holds no golden vectors of its own.  These come from:
     src/io/InputParser.c) compiled unchanged by `make -C oracle ref`:
       eos_ref.npz      calculate_state / calculate_state_correct on seeded
                        (rho, rho*e) samples of the three test EOS files,
-                       interpolate_eos_tp on seeded (T, P) samples (states
-                       inside each table's coverage), state_reference
+                       interpolate_eos_tp on seeded (T, P) samples (NaN
+                       where no table cell holds the state), state_reference
                        values and the Tillotson cold table;
       laplace_ref.npz  Laplace2d9 weights (gamma 0.5) of seeded stencils;
       bisearch_ref.npz bisearch on seeded sorted lists;
@@ -66,6 +66,16 @@ def write_eos_ref(wd):
         T = rng_tp.uniform(200.0, 2000.0, 400)
         P = 10.0 ** rng_tp.uniform(3.0, 10.0, 400)
         tp = np.array([E.eos_tp(a, b) for a, b in zip(T, P)])
+        # a state that no table cell holds makes the reference's search read
+        # past its arrays (undefined values); such entries are stored as NaN.
+        # They are the states the host search reports as uncovered.
+        hr = ps.HostRun(inp, eos)
+        mat = hr.cfg.mat[m]
+        for i, (a, b) in enumerate(zip(T, P)):
+            try:
+                ps.eos_tp(mat, a, b)
+            except RuntimeError:
+                tp[i] = np.nan
         out[k + "_tp_T"], out[k + "_tp_P"], out[k + "_tp"] = T, P, tp
         sr = E.state_reference()
         out[k + "_stateref"] = np.array([sr[n] for n in sorted(sr)])

@@ -146,6 +146,11 @@ def test_golden_run():
 
 VARIANTS = {
     "damp_pml": ("c1", dict(damping="pml")),
+    # pml3 couples the layer into the remap through e_beta (src/advect.c:304-593)
+    "damp_pml3": ("c1", dict(damping="pml3")),
+    "damp_pml3_plain_hydro": ("c0", dict(damping="pml3")),
+    # by step ~156 the wave carries momentum into the layer's flux terms
+    "damp_pml3_wave_in_layer": ("c1", dict(damping="pml3"), 160, 25),
     # the reference's sponge layer adds the reference state on top of the
     # damped one in y (src/velocity.c:746-747), so densities in the bottom
     # layer grow geometrically and dt collapses within ~20 steps: compare the

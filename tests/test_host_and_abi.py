@@ -95,7 +95,6 @@ def test_v_laplace_field_on_stretched_mesh(tmp_path):
 
 
 @pytest.mark.parametrize("over,what", [
-    ({"numerical.damping": "pml3"}, "pml3"),
     ({"material.Porosity": "[, Wunnemann]"}, "orosity"),
     ({"target.type": "sphere"}, "sphere"),
 ])
@@ -193,7 +192,7 @@ def test_create_validates_before_touching_a_device(tmp_path):
     cfg.nx = 2
     assert L.salec_create(C.byref(cfg), C.byref(ctx)) == -1
     cfg = ps.Config.from_buffer_copy(h.cfg)
-    cfg.damp_type = ps.DAMP["pml3"]
+    cfg.damp_type = 17
     assert L.salec_create(C.byref(cfg), C.byref(ctx)) == -1
     mats = (ps.Material * h.nmat)(*[ps.Material.from_buffer_copy(h.cfg.mat[m]) for m in range(h.nmat)])
     mats[1].yfunc = 99
@@ -238,7 +237,7 @@ def test_eos_tp_inversion_golden(eos_runs, case):
         try:
             got = ps.eos_tp(mat, T, Pr)
         except RuntimeError:
-            assert fn.endswith(".sd")
+            assert fn.endswith(".sd") and np.isnan(ref).all()
             continue
         assert np.array_equal(got, ref, equal_nan=True), (T, Pr, got, ref)
         checked += 1
