@@ -156,6 +156,10 @@ typedef struct salec_material {
     double Toff, Cvib, VibMax, Tdamp, Pvlim, Acvis;
     double alpha0, chi, gamfrac, den0, melt_viscosity;
     int    yfunc, dfunc, tfunc, pfunc; /* SALEC_FN_* (shear, damage, tensile, porosity) */
+    /* epsilon-alpha porosity (Wunnemann), src/sale2d_back.c:369-398: input
+     * alphax, epse0, kappa, avel; derived alphamax, epsex, epsec, alphae;
+     * alphamax2 bounds the distension (1 for a material without porosity) */
+    double alphax, epse0, kappa, avel, alphamax, epsex, epsec, alphae, alphamax2;
 } salec_material;
 
 /* ---- run configuration (sale2d_var scalars + mesh2d_info options) ------ */

@@ -72,6 +72,7 @@ double oracle_cut_length(double vf, double a, double b);
 double oracle_geometric_flux_ratio(double p1, double p2);
 double oracle_vof_mstacs(double nvofd, double gf, double courant);
 double oracle_till_pres(const salec_material *m, double eng, double rho, double *pre, double *cs);
+double oracle_porosity_wunnemann(const double *paras, const salec_material *s);
 double oracle_till_temp(const salec_material *m, double eng, double rho);
 void   oracle_till_cold_energy(const salec_material *m, int nden, double *xden, double *yeng);
 int    oracle_bisearch(double x, const double *xl, int n);

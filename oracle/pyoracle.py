@@ -54,6 +54,8 @@ def olib():
     L.oracle_calculate_state_correct.argtypes = [MP, d, d, d, dp, dp, dp, d]
     L.oracle_calculate_state_correct.restype = d
     L.oracle_strength_fn.argtypes = [C.c_int, dp, MP]
+    L.oracle_porosity_wunnemann.argtypes = [dp, MP]
+    L.oracle_porosity_wunnemann.restype = C.c_double
     L.oracle_strength_fn.restype = d
     L.oracle_get_flux_volcyl.argtypes = [dp, dp, dp, dp]
     L.oracle_get_flux_volcyl.restype = d

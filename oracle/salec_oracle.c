@@ -1052,9 +1052,40 @@ static void calculate_v_dis(oracle_state *s, const grid_t g, double dt)
 }
 
 /* update_energy, src/sale2d.c:564-712 (porosity model off) */
+/* porosity_model, src/sale2d.c:490-513: some material has a porosity model */
+static int porous(const salec_config *cfg)
+{
+    for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].pfunc != SALEC_FN_NONE) return 1;
+    return 0;
+}
+
+/* PorosityCsdRatio and WunnemannPorosity, src/strength.c:794-852:
+ * paras = {alpha, cold volume strain, pressure, d(strain)/dt, u, w} */
+double oracle_porosity_wunnemann(const double *paras, const salec_material *s)
+{
+    double pty = paras[0], cvs = paras[1], pre = paras[2], dcvs = paras[3], evx = paras[4], evy = paras[5];
+    if (pty <= 1.0 || pty > s->alphamax || pre <= 0.) return 0.;
+    double evel2 = evx * evx + evy * evy;
+    if (evel2 <= s->avel * s->avel) return 0.;
+    double cvs_c = 0.;
+    if (pty > s->alphae) cvs_c = s->epse0;
+    else if (pty > s->alphax) cvs_c = s->epse0 + log(pty / s->alphae) / s->kappa;
+    else cvs_c = s->epsec - sqrt((pty - 1.0) / (s->alphax - 1.0)) * (s->epsec - s->epsex);
+    double ratio = 1.0 + (s->chi - 1.0) * fmax(0.0, fmin(1.0, (pty - 1.0) / (s->alphamax - 1.0)));
+    double dade = pty * (1.0 - ratio * ratio);
+    if (dcvs < 0. && cvs < cvs_c) {
+        if (pty > s->alphax) dade = s->kappa * pty;
+        else dade = sqrt((pty - 1.) / (s->alphax - 1.0)) * s->kappa * s->alphax;
+    } else {
+        dade = pty * (1.0 - ratio * ratio);
+    }
+    return dade * dcvs;
+}
+
 static void update_energy(oracle_state *s, const salec_config *cfg, const grid_t g, double dt)
 {
     int n = g.nmat;
+    const int por = porous(cfg);
     for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
         if (!cell_physical(&g, k, j)) continue;
         long c = CI(k, j);
@@ -1072,9 +1103,22 @@ static void update_energy(oracle_state *s, const salec_config *cfg, const grid_t
         for (int m = 1; m < n; ++m) {
             if (CF(M_VOF, m, c) <= TOLVOF || CF(M_DEN, m, c) <= TOLRHO) {
                 CF(M_ENG, m, c) = 0.;
+                if (por) CF(M_PTY, m, c) = 1.;
             } else {
                 double f = 1.0;
                 CF(M_ENG, m, c) = Max(0., CF(M_ENG, m, c) + (dEp + dEs) * dt * f);
+                if (por) {
+                    /* EXCLUDE_PML_POROSITY (src/sale2d_constant.h:32): no
+                     * compaction inside the damping layer; ZeroFunc for a
+                     * material without a porosity model */
+                    double dp = 0.;
+                    if (CF(E_SIGMA, 0, c) + CF(E_SIGMA, 1, c) <= 0. && cfg->mat[m].pfunc == SALEC_FN_WUNNEMANN) {
+                        double pa[6] = {CF(M_PTY, m, c), CF(E_CVS, 0, c), CF(E_PRE, 0, c), dcvs, CF(E_VEL, 0, c), CF(E_VEL, 1, c)};
+                        dp = oracle_porosity_wunnemann(pa, &cfg->mat[m]);
+                    }
+                    CF(M_PTY, m, c) += dp * dt;
+                    CF(M_PTY, m, c) = Wind(CF(M_PTY, m, c), 1.0, cfg->mat[m].alphamax2);
+                }
             }
         }
         CF(E_CVS, 0, c) = Wind(CF(E_CVS, 0, c) + dcvs * dt, -10.0, 10.0);
@@ -1144,7 +1188,7 @@ static double algebraic_flux_ratio(double U, double D, double A, double cd, cons
 }
 
 /* calculate_flux_mixed + _calculate_flux_set_bphi, src/advect.c:861-1224 */
-static void calculate_flux_mixed(oracle_state *s, const grid_t g, int physical)
+static void calculate_flux_mixed(oracle_state *s, const grid_t g, int physical, int por)
 {
     int n = g.nmat, PL = 3 * n + 10;
     for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
@@ -1267,6 +1311,10 @@ static void calculate_flux_mixed(oracle_state *s, const grid_t g, int physical)
             ph[i][(long)(3 * n + 7) * st[i]] = fm * CF(E_TPS, 0, c);
             ph[i][(long)(3 * n + 8) * st[i]] = fm * CF(E_VIB, 0, c);
             ph[i][(long)(3 * n + 9) * st[i]] = fm * CF(E_EJT, 0, c);
+            if (por) {
+                ph[i][(long)(3 * n + 10) * st[i]] = fm * CF(E_CVS, 0, c);
+                for (int q = 0; q < n; ++q) ph[i][(long)(3 * n + 11 + q) * st[i]] = CF(M_PTY, q, c);
+            }
         }
         (void)PL;
     }
@@ -1275,7 +1323,7 @@ static void calculate_flux_mixed(oracle_state *s, const grid_t g, int physical)
 /* clean_flux, src/advect.c:597-626 */
 static void clean_flux(oracle_state *s, const grid_t g)
 {
-    int PL = 3 * g.nmat + 10;
+    int PL = 4 * g.nmat + 11;
     memset(s->bx_phi, 0, sizeof(double) * PL * g.NBX);
     memset(s->by_phi, 0, sizeof(double) * PL * g.NBY);
 }
@@ -1287,7 +1335,8 @@ static void clean_flux(oracle_state *s, const grid_t g)
  * (:486-489, :516, :565) */
 static void advect_flux_default(oracle_state *s, const salec_config *cfg, const grid_t g, int pml3, double dt)
 {
-    int n = g.nmat, PL = 3 * n + 10;
+    const int por = porous(cfg) && !pml3;     /* advect_flux_pml3 has no porosity terms */
+    int n = g.nmat, PL = 3 * n + 10 + (por ? 1 + n : 0);
     double adx = cfg->alpha_damp_x, ady = cfg->alpha_damp_y;
     for (int j = 0; j < g.CH; ++j) for (int k = 0; k < g.CW; ++k) {
         if (!cell_physical(&g, k, j)) continue;
@@ -1310,6 +1359,10 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
                 dm += mv * PH(m + n);
                 double pa = mv * PH(m + n);
                 for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * PH(3 * n + q);
+                if (por) {
+                    dphi[3 * n + 10] += (PH(3 * n + 10)) * (mv * PH(m + n));
+                    dphi[m + 3 * n + 11] += (PH(m + 3 * n + 11)) * (mv * PH(m + n));
+                }
                 if (pml3 && (sgx > 0. || sgy > 0.)) {
                     CF(E_BETA, 0, c) += mv * PH(m + n) * pml_para[i];
                     CF(E_BETA, 1, c) += mv * PH(m + 2 * n) * pml_para[i];
@@ -1332,6 +1385,10 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
         sphi[3 * n + 7] = emass * CF(E_TPS, 0, c);
         sphi[3 * n + 8] = emass * CF(E_VIB, 0, c);
         sphi[3 * n + 9] = emass * CF(E_EJT, 0, c);
+        if (por) {
+            sphi[3 * n + 10] = emass * CF(E_CVS, 0, c);
+            for (int m = 0; m < n; ++m) sphi[3 * n + 11 + m] = CF(M_VOF, m, c) * evol * CF(M_DEN, m, c) * CF(M_PTY, m, c);
+        }
         for (int q = 0; q < PL; ++q) sphi[q] += 1.0 * dphi[q];
         if (pml3 && sgx + sgy > 0.) {
             for (int q = 0; q < 4; ++q) CF(E_BETA, q, c) += -1.0 * (adx + ady) * CF(E_BETA, q, c) * dt;
@@ -1379,9 +1436,11 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
         for (int m = 1; m < n; ++m) {
             if (ev[m] < TOLVOF || (!pml3 && sphi[m + n] < TOLRHO * TOLVOF * evol)) {
                 me[m] = 0.0; md[m] = 0.0; ev[m] = 0.0;
+                if (por) CF(M_PTY, m, c) = 1.0;
             } else {
                 me[m] = sphi[m + 2 * n] / (ev[m] * evol);
                 md[m] = sphi[m + n] / (ev[m] * evol);
+                if (por) CF(M_PTY, m, c) = sphi[3 * n + 11 + m] / sphi[m + n];
             }
         }
         double dv[2], ste[4], dam, tps, vib, ejt;
@@ -1391,19 +1450,27 @@ static void advect_flux_default(oracle_state *s, const salec_config *cfg, const 
             for (int q = 0; q < 4; ++q) ste[q] = px * sphi[3 * n + 2 + q];
             dam = sphi[3 * n + 6] / emass; tps = sphi[3 * n + 7] / emass;
             vib = sphi[3 * n + 8] / emass; ejt = sphi[3 * n + 9] / emass;
+            if (por) CF(E_CVS, 0, c) = sphi[3 * n + 10] / emass;
         } else {
             dv[0] = dv[1] = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
             dam = tps = vib = ejt = 0.;
+            if (por) CF(E_CVS, 0, c) = 0.;
         }
         double eden = emass / evol;
         tps = Wind(tps, 0.0, 10.0);
         dam = Wind(dam, 0.0, 1.0);
         ejt = Wind(ejt, 0.0, 1.0);
+        if (por)
+            for (int m = 1; m < n; ++m) CF(M_PTY, m, c) = Wind(CF(M_PTY, m, c), 1.0, cfg->mat[m].alphamax2);
         if (eden < cfg->density_cutoff || (!pml3 && eden < 10.0 * cfg->density_cutoff * (1.0 - ev[VACUUM]))) {
             for (int m = 0; m < n; ++m) { ev[m] = 0.0; md[m] = 0.0; me[m] = 0.0; }
             ev[VACUUM] = 1.0;
             dv[0] = dv[1] = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
             dam = tps = vib = ejt = 0.;
+            if (por) {
+                CF(E_CVS, 0, c) = 0.;
+                for (int m = 1; m < n; ++m) CF(M_PTY, m, c) = 1.0;
+            }
         }
         for (int m = 0; m < n; ++m) { CF(M_VOF, m, c) = ev[m]; CF(M_DEN, m, c) = md[m]; CF(M_ENG, m, c) = me[m]; }
         CF(E_DVEL, 0, c) = dv[0]; CF(E_DVEL, 1, c) = dv[1];
@@ -1669,7 +1736,7 @@ static void tracer_update(oracle_state *s, const salec_config *cfg, const grid_t
 int oracle_attach(oracle_state *s, const salec_config *cfg)
 {
     grid_t g = mkgrid(s);
-    int PL = 3 * cfg->nmat + 10;
+    int PL = 4 * cfg->nmat + 11;          /* phi_length with the porosity entries */
     s->bx_phi = (double *)calloc((size_t)PL * g.NBX, sizeof(double));
     s->by_phi = (double *)calloc((size_t)PL * g.NBY, sizeof(double));
     s->tr_vx = (double *)malloc(sizeof(double) * g.VW);
@@ -1780,11 +1847,15 @@ void oracle_step_to_output(oracle_state *s, const salec_config *cfg)
     update_energy(s, cfg, g, dt);
     tracer_update(s, cfg, g, dt);
     clean_flux(s, g);
-    calculate_flux_mixed(s, g, 1);
+    calculate_flux_mixed(s, g, 1, porous(cfg));
     oracle_ghost_fill(s, cfg, SALEC_M_DEN, O_GHT_DENSITY);
     oracle_ghost_fill(s, cfg, SALEC_E_VIB, O_GHT_ALWAYS_COPY);
     oracle_ghost_fill(s, cfg, SALEC_M_ENG, O_GHT_DENSITY);
-    calculate_flux_mixed(s, g, 0);
+    if (porous(cfg)) {                     /* src/salec2d.c:228-229 */
+        oracle_ghost_fill(s, cfg, SALEC_M_PTY, O_GHT_ALWAYS_COPY);
+        oracle_ghost_fill(s, cfg, SALEC_E_CVS, O_GHT_ALWAYS_COPY);
+    }
+    calculate_flux_mixed(s, g, 0, porous(cfg));
     (void)cell_receive;
 }
 void oracle_step_from_output(oracle_state *s, const salec_config *cfg)

@@ -79,7 +79,8 @@ struct salec_ctx {
     int prof_chunks = 0;
     double *d_stage = nullptr;           /* two-phase ghost staging       */
     long stage_len = 0;
-    Program gfA, gfE, gfB, gfC, rstV, rstA, rstD, gfInit, gfInit2;
+    Program gfA, gfE, gfB, gfC, gfP, rstV, rstA, rstD, gfInit, gfInit2;
+    std::vector<int> swapped;            /* fields of the two buffer sets (kSwapped + porosity) */
     Tracers T{};
     long tr_cap = 0;
     double *d_vx = nullptr, *d_vy = nullptr;
@@ -393,8 +394,10 @@ static void run_program(salec_ctx *c, const Program &P)
     launch_ghost(c->stream, sl, P.d_ops, P.nops, c->d_stage, P.two_phase, P.maxc);
 }
 
-/* Fields the remap writes into the second buffer set (k_remap_close). */
+/* Fields the remap writes into the second buffer set (k_remap_close); with
+ * a porosity model the distension and cold volume strain are remapped too. */
 const int kSwapped[] = {SALEC_M_DEN, SALEC_M_ENG, SALEC_E_STE, SALEC_E_DAM, SALEC_E_TPS, SALEC_E_VIB, SALEC_E_EJT};
+const int kSwappedPorous[] = {SALEC_M_PTY, SALEC_E_CVS};
 
 /* host bisearch (same rule as the device's) for the hint table */
 static int host_bisearch(double x, const double *xl, int n)
@@ -434,7 +437,7 @@ static DevMat to_dev(const salec_material &m)
 {
     DevMat d;
     std::memset(&d, 0, sizeof d);
-    d.eos_type = m.eos_type; d.yfunc = m.yfunc; d.dfunc = m.dfunc; d.tfunc = m.tfunc;
+    d.eos_type = m.eos_type; d.yfunc = m.yfunc; d.dfunc = m.dfunc; d.tfunc = m.tfunc; d.pfunc = m.pfunc;
     d.till_nden = m.till_nden; d.tab_nden = m.tab_nden; d.tab_ntem = m.tab_ntem;
     d.PolyLiq = m.PolyLiq; d.PolySol = m.PolySol;
     d.TLRho0 = m.TLRho0; d.TLCv = m.TLCv; d.TLA = m.TLA; d.TLB = m.TLB; d.TLE0 = m.TLE0; d.TLa = m.TLa;
@@ -450,6 +453,8 @@ static DevMat to_dev(const salec_material &m)
     d.Tsolc1 = m.Tsolc1; d.Tsolc2 = m.Tsolc2; d.Tsolc3 = m.Tsolc3; d.Toff = m.Toff; d.Cvib = m.Cvib;
     d.VibMax = m.VibMax; d.Tdamp = m.Tdamp; d.Pvlim = m.Pvlim; d.Acvis = m.Acvis; d.alpha0 = m.alpha0;
     d.chi = m.chi; d.gamfrac = m.gamfrac; d.melt_viscosity = m.melt_viscosity;
+    d.alphax = m.alphax; d.epse0 = m.epse0; d.kappa = m.kappa; d.avel = m.avel; d.alphamax = m.alphamax;
+    d.epsex = m.epsex; d.epsec = m.epsec; d.alphae = m.alphae; d.alphamax2 = m.alphamax2;
     return d;
 }
 
@@ -465,7 +470,7 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     if (cfg->damp_type < SALEC_DAMP_PML || cfg->damp_type > SALEC_DAMP_NONE) return SALEC_EINVAL;
     for (int m = 1; m < cfg->nmat; ++m) {
         const salec_material &mm = cfg->mat[m];
-        if (mm.pfunc != SALEC_FN_NONE) return SALEC_EINVAL;      /* porosity model not provided */
+        if (mm.pfunc != SALEC_FN_NONE && mm.pfunc != SALEC_FN_WUNNEMANN) return SALEC_EINVAL;
         if (mm.eos_type == SALEC_EOS_TILLOTSON && (!mm.till_yeng || mm.till_nden < 2)) return SALEC_EINVAL;
         if ((mm.eos_type == SALEC_EOS_ANEOS_SD || mm.eos_type == SALEC_EOS_ANEOS) &&
             (!mm.tab_yden || !mm.tab_xtem || !mm.tab_data || mm.tab_nden < 2 || mm.tab_ntem < 2)) return SALEC_EINVAL;
@@ -494,6 +499,9 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     P.damp_type = cfg->damp_type;
     P.acoustic_fluid = 0;
     for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].Toff > 0.0) P.acoustic_fluid = 1;
+    /* porosity_model, src/sale2d.c:490-513 */
+    P.porosity = 0;
+    for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].pfunc != SALEC_FN_NONE) P.porosity = 1;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SALEC_EDEVICE; }
 #define TRY(x) do { rc = (x); if (rc != SALEC_OK) { salec_destroy(c); return rc; } } while (0)
     for (int f = 0; f < SALEC_NFIELDS; ++f) {
@@ -501,7 +509,9 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
         salec_field_info(f, g.nmat, &grid, &nc);
         TRY(dalloc(c, &c->F.f[f], (size_t)salec_grid_points(grid, g.nx, g.ny) * nc));
     }
-    for (int f : kSwapped) {
+    c->swapped.assign(std::begin(kSwapped), std::end(kSwapped));
+    if (P.porosity) c->swapped.insert(c->swapped.end(), std::begin(kSwappedPorous), std::end(kSwappedPorous));
+    for (int f : c->swapped) {
         int grid, nc;
         salec_field_info(f, g.nmat, &grid, &nc);
         TRY(dalloc(c, &c->Falt.f[f], (size_t)salec_grid_points(grid, g.nx, g.ny) * nc));
@@ -549,6 +559,13 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     TRY(build_program(c, c->gfE, {{SALEC_E_VEL, VT_VELOCITY}}));
     TRY(build_program(c, c->gfB, {{SALEC_E_PRE, VT_ALWAYS_COPY}, {SALEC_E_STE, VT_STRESS}, {SALEC_E_Q, VT_ALWAYS_COPY}}));
     TRY(build_program(c, c->gfC, {{SALEC_M_DEN, VT_DENSITY}, {SALEC_E_VIB, VT_ALWAYS_COPY}, {SALEC_M_ENG, VT_DENSITY}}));
+    /* with a porosity model: m_pty and e_cvs, src/salec2d.c:228-229 */
+    if (P.porosity) {
+        TRY(build_program(c, c->gfP, {{SALEC_M_PTY, VT_ALWAYS_COPY}, {SALEC_E_CVS, VT_ALWAYS_COPY}}));
+        /* the remap reads donor distension and strain after this fill: it
+         * must leave every physical cell as it is */
+        if (!c->gfP.phys_dst.empty()) { salec_destroy(c); return SALEC_EINVAL; }
+    }
     /* the remap reads donor m_den/m_eng after this fill and relies on it
      * changing no physical cell outside the top row (ght_clean), which the
      * flux kernel snapshots */
@@ -782,16 +799,16 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     PHASE(PH_VDIS, launch_v_dis(s, g, F, c->d_time));
     PHASE(PH_RST_DIS, run_program(c, c->rstD));
     PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
-    PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, c->d_mats, c->d_time, c->bx, c->by, c->d_top));
-    PHASE(PH_GF_C, run_program(c, c->gfC));
+    PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, P, c->d_mats, c->d_time, c->bx, c->by, c->d_top));
+    PHASE(PH_GF_C, run_program(c, c->gfC); run_program(c, c->gfP));
     PHASE(PH_FLUX_RING, launch_flux_ring(s, g, F, c->bx, c->by));
     }
     c->half_open = to_out;
     if (!to_out) {
         Fields Fn = c->F;
-        for (int f : kSwapped) Fn.f[f] = c->Falt.f[f];
+        for (int f : c->swapped) Fn.f[f] = c->Falt.f[f];
         PHASE(PH_REMAP_CLOSE, launch_remap_close(s, g, F, Fn, P, c->d_mats, c->bx, c->by, c->d_top, c->d_time));
-        for (int f : kSwapped) std::swap(c->F.f[f], c->Falt.f[f]);
+        for (int f : c->swapped) std::swap(c->F.f[f], c->Falt.f[f]);
         c->parity ^= 1;
         PHASE(PH_TIME, launch_advance_time(s, c->d_time));
     }
@@ -883,14 +900,15 @@ static void sens_step_once(salec_ctx *c)
     run_program_sens(c, c->rstA);
     launch_v_dis(s, g, c->F, T, c->d_time);
     run_program_sens(c, c->rstD);
-    launch_energy_flux(s, g, c->F, T, c->d_mats, c->d_time, S->bx, S->by, S->top);
+    launch_energy_flux(s, g, c->F, T, P, c->d_mats, c->d_time, S->bx, S->by, S->top);
     run_program_sens(c, c->gfC);
+    run_program_sens(c, c->gfP);
     launch_flux_ring(s, g, c->F, T, S->bx, S->by);
     Fields Fn = c->F;
     TFields Tn = S->T;
-    for (int f : kSwapped) { Fn.f[f] = c->Falt.f[f]; Tn.t[f] = S->Talt.t[f]; }
+    for (int f : c->swapped) { Fn.f[f] = c->Falt.f[f]; Tn.t[f] = S->Talt.t[f]; }
     launch_remap_close(s, g, c->F, T, Fn, Tn, P, c->d_mats, S->bx, S->by, S->top, c->d_time);
-    for (int f : kSwapped) { std::swap(c->F.f[f], c->Falt.f[f]); std::swap(S->T.t[f], S->Talt.t[f]); }
+    for (int f : c->swapped) { std::swap(c->F.f[f], c->Falt.f[f]); std::swap(S->T.t[f], S->Talt.t[f]); }
     c->parity ^= 1;
     launch_advance_time(s, c->d_time);
     launch_sens_accum(s, g, c->F, S->T, S->acc, S->row);
@@ -921,7 +939,7 @@ int salec_sens_begin(salec_ctx *c, unsigned params, unsigned mat_mask, int surfa
         S->T.st[f] = S->Talt.st[f] = n;
         TRY(salloc(S, &S->T.t[f], (size_t)NTAN * n));
     }
-    for (int f : kSwapped)
+    for (int f : c->swapped)
         if (tangent_field(f)) TRY(salloc(S, &S->Talt.t[f], (size_t)NTAN * S->T.st[f]));
     TRY(salloc(S, &S->bx, (size_t)g.nmat * g.NBX));
     TRY(salloc(S, &S->by, (size_t)g.nmat * g.NBY));
@@ -1152,7 +1170,7 @@ int salec_batch_step(salec_batch *b, int nsteps)
     for (int k = 0; k < nsteps; ++k)
         for (salec_ctx *c : b->ctx) {
             HIPCHK(hipGraphLaunch(c->gexec[c->parity], c->stream));
-            for (int f : kSwapped) std::swap(c->F.f[f], c->Falt.f[f]);
+            for (int f : c->swapped) std::swap(c->F.f[f], c->Falt.f[f]);
             c->parity ^= 1;
         }
     for (salec_ctx *c : b->ctx) HIPCHK(hipStreamSynchronize(c->stream));

@@ -162,7 +162,7 @@ __device__ inline void aneos_state(const DevMat &t, S den, S eng, S *tem, S *pre
 /* calculate_state dispatch (src/eos/eos_state.c:727-745, 844-850) followed by
  * calculate_state_correct with CAP_MINPRESSURE (src/eos/eos_state.c:747-791) */
 template <class S>
-__device__ inline void eos_state_capped(const DevMat &m, S den, S eng, S plim, double pty,
+__device__ inline void eos_state_capped(const DevMat &m, S den, S eng, S plim, S pty,
                                         S *tem, S *pre, S *csd)
 {
     S tt = 0., tp = 0., tc = 0.;
@@ -283,6 +283,30 @@ __device__ inline S strength_fn(int fn, S p0, S p1, S p2, S p3, S p4, const DevM
     default:
         return 0.0;
     }
+}
+
+/* WunnemannPorosity (src/strength.c:801-852) with PorosityCsdRatio
+ * (:794-799): d(alpha)/dt of the epsilon-alpha compaction model from the
+ * distension, cold volume strain, pressure, its rate and the cell velocity */
+template <class S>
+__device__ inline S porosity_rate(const DevMat &s, S pty, S cvs, S pre, S dcvs, S evx, S evy)
+{
+    if (pty <= 1.0 || pty > s.alphamax || pre <= 0.) return S(0.);
+    S evel2 = evx * evx + evy * evy;
+    if (evel2 <= s.avel * s.avel) return S(0.);
+    S cvs_c = 0.;
+    if (pty > s.alphae) cvs_c = s.epse0;
+    else if (pty > s.alphax) cvs_c = s.epse0 + log(pty / s.alphae) / s.kappa;
+    else cvs_c = s.epsec - sqrt((pty - 1.0) / (s.alphax - 1.0)) * (s.epsec - s.epsex);
+    S ratio = 1.0 + (s.chi - 1.0) * fmax(0.0, fmin(1.0, (pty - 1.0) / (s.alphamax - 1.0)));
+    S dade = pty * (1.0 - ratio * ratio);
+    if (dcvs < 0. && cvs < cvs_c) {
+        if (pty > s.alphax) dade = s.kappa * pty;
+        else dade = sqrt((pty - 1.) / (s.alphax - 1.0)) * s.kappa * s.alphax;
+    } else {
+        dade = pty * (1.0 - ratio * ratio);
+    }
+    return dade * dcvs;
 }
 
 /* get_flux_volcyl, src/advect.c:1347-1354 */

@@ -13,7 +13,7 @@
  * field element; with S = Dual it is a DRef that reads and writes the value
  * plane and the NTAN tangent planes of the field together.  Fields whose
  * values never depend on the parameters (mesh geometry, PML profiles,
- * distension, the ejecta flag, the crater profile) have no tangent planes and
+ * the ejecta flag, the crater profile) have no tangent planes and
  * are read as plain doubles in both modes.
  */
 #pragma once
@@ -25,6 +25,7 @@ namespace salec {
 
 /* the double overloads stay visible beside the Dual ones below */
 using ::sqrt; using ::fabs; using ::exp; using ::log; using ::tanh; using ::pow; using ::floor; using ::isnan;
+using ::fmin; using ::fmax;
 
 #define NTAN 3
 
@@ -129,6 +130,20 @@ __device__ __forceinline__ Dual pow(const Dual &a, double p)
     return r;
 }
 __device__ __forceinline__ double floor(const Dual &a) { return ::floor(a.v); }
+/* fmin/fmax against a constant: the value is the double function's (NaN
+ * handling included), the derivative that of the operand it returns */
+__device__ __forceinline__ Dual fmin(double a, const Dual &b)
+{
+    Dual r(::fmin(a, b.v));
+    if (r.v == b.v) for (int k = 0; k < NTAN; ++k) r.d[k] = b.d[k];
+    return r;
+}
+__device__ __forceinline__ Dual fmax(double a, const Dual &b)
+{
+    Dual r(::fmax(a, b.v));
+    if (r.v == b.v) for (int k = 0; k < NTAN; ++k) r.d[k] = b.d[k];
+    return r;
+}
 __device__ __forceinline__ bool isnan(const Dual &a) { return ::isnan(a.v); }
 
 /* selection helpers of the reference (Max/Min/Wind), both scalar types */
@@ -162,7 +177,7 @@ template <class S> using TF = typename std::conditional<std::is_same<S, double>:
 /* fields whose values never depend on the parameters */
 __host__ __device__ constexpr bool tangent_field(int f)
 {
-    return !(f == SALEC_M_PTY || f == SALEC_E_EJT || f == SALEC_E_POS || f == SALEC_E_VOL || f == SALEC_E_GRAD ||
+    return !(f == SALEC_E_EJT || f == SALEC_E_POS || f == SALEC_E_VOL || f == SALEC_E_GRAD ||
              f == SALEC_E_SIGMA || f == SALEC_E_DAMPREF || f == SALEC_V_POS || f == SALEC_V_BF ||
              f == SALEC_V_VOL || f == SALEC_V_SUBVOL || f == SALEC_V_SUF || f == SALEC_V_LAPLACE ||
              f == SALEC_BX_NORMAL || f == SALEC_BY_NORMAL || f == SALEC_Y_PROFILE);

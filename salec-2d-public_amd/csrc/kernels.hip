@@ -79,6 +79,13 @@ template <> struct Mats<0> {
     static constexpr int cap = SALEC_MAXMAT;
     __device__ static int n(const Grid &g) { return g.nmat; }
 };
+/* porosity model on/off as a template parameter: its remap terms cost
+ * registers that the porosity-free kernels should not carry */
+template <class L> static void por_dispatch(int por, L launch)
+{
+    if (por) launch(std::integral_constant<bool, true>{});
+    else launch(std::integral_constant<bool, false>{});
+}
 template <class L> static void nm_dispatch(int nmat, L launch)
 {
     switch (nmat) {
@@ -856,8 +863,12 @@ void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const TFields &
 }
 
 /* -------------------------------------------------------- update_energy -- */
-/* src/sale2d.c:564-712 (porosity model off) */
-template <class S, int NM, class TT>
+/* src/sale2d.c:564-712; with a porosity model (POR) the distension of each
+ * full material grows by the material's rate (WunnemannPorosity, or zero
+ * for a material without one) outside the damping layer
+ * (EXCLUDE_PML_POROSITY, src/sale2d_constant.h:32) and is bounded to
+ * [1, alphamax2] */
+template <class S, int NM, bool POR, class TT>
 __device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, const TT &T, const DevMat *mats, S dt, int k, int j)
 {
     long c = k + (long)g.CW * j;
@@ -875,12 +886,27 @@ __device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, cons
         if (vf > TOLVOF) eg += vf * mats[m].gamfrac / CPL(M_DEN, m, c);
     }
     S dcvs = div - eg * (dEp + dEs);
+    bool outside_layer = true;
+    S cvs0 = 0., pre0 = 0., ev0 = 0., ev1 = 0.;
+    if (POR) {
+        outside_layer = CPL(E_SIGMA, 0, c) + CPL(E_SIGMA, 1, c) <= 0.;
+        cvs0 = CPL(E_CVS, 0, c); pre0 = CPL(E_PRE, 0, c); ev0 = CPL(E_VEL, 0, c); ev1 = CPL(E_VEL, 1, c);
+    }
     for (int m = 1; m < n; ++m) {
         if (CPL(M_VOF, m, c) <= TOLVOF || CPL(M_DEN, m, c) <= TOLRHO) {
             CPL(M_ENG, m, c) = 0.;
+            if (POR) CPL(M_PTY, m, c) = 1.;
         } else {
             double f = 1.0;
             CPL(M_ENG, m, c) = dMax(0., CPL(M_ENG, m, c) + (dEp + dEs) * dt * f);
+            if (POR) {
+                const DevMat &r = mats[m];
+                S pty = CPL(M_PTY, m, c);
+                S dp = 0.;
+                if (outside_layer && r.pfunc == SALEC_FN_WUNNEMANN) dp = porosity_rate(r, pty, cvs0, pre0, dcvs, ev0, ev1);
+                pty += dp * dt;
+                CPL(M_PTY, m, c) = dWind(pty, 1.0, r.alphamax2);
+            }
         }
     }
     CPL(E_CVS, 0, c) = dWind(CPL(E_CVS, 0, c) + dcvs * dt, -10.0, 10.0);
@@ -1113,9 +1139,10 @@ void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, const TFiel
  * A face carries flux iff its swept volume is non-zero; the donor is the
  * side on which it is negative (exactly antisymmetric, src/advect.c:1347). */
 #define CPN(fld, c, i) fa<S, SALEC_##fld>(Fn.f, Tn, (long)(c) * g.NC + (i))
-template <class S, int NM, class TT>
+template <class S, int NM, bool POR, class TT>
 __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const TT &T, const Fields &Fn, const TT &Tn,
-                                            const Params &P, const S *bx, const S *by, const S *top, S dt, int k, int j)
+                                            const Params &P, const DevMat *mats, const S *bx, const S *by, const S *top,
+                                            S dt, int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g), PL = 3 * n + 10;
@@ -1131,6 +1158,12 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     }
     const bool pmlc = pml3 && (sgx > 0. || sgy > 0.);
     const double pp[4] = {-sgy, -sgx, -sgy, -sgx};
+    /* porosity (advect_flux_default with porosity_model, src/advect.c:98-102,
+     * 179-186, 235-240, 253-293): cold volume strain and each material's
+     * distension travel with the mass; advect_flux_pml3 has no such terms */
+    const bool por = POR && !pml3;
+    S dcv = 0., dpt[Mats<NM>::cap];
+    if (POR) for (int m = 0; m < n; ++m) dpt[m] = 0.;
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
     double px[4], py[4];
     S dx[4], dy[4];
@@ -1172,6 +1205,10 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
             dphi[m + 2 * n] += mv * de;
             S pa = mv * dm;
             for (int q = 0; q < 10; ++q) dphi[3 * n + q] += pa * tail[q];
+            if (POR && por) {
+                dcv += S(CPL(E_CVS, 0, d)) * pa;
+                dpt[m] += S(CPL(M_PTY, m, d)) * pa;
+            }
             if (pmlc) {
                 beta[0] += mv * dm * pp[i];
                 beta[1] += mv * de * pp[i];
@@ -1196,7 +1233,16 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     sphi[3 * n + 7] = emass * CPL(E_TPS, 0, c);
     sphi[3 * n + 8] = emass * CPL(E_VIB, 0, c);
     sphi[3 * n + 9] = emass * CPL(E_EJT, 0, c);
+    S scv = 0., spt[Mats<NM>::cap];
+    if (POR && por) {
+        scv = emass * CPL(E_CVS, 0, c);
+        for (int m = 0; m < n; ++m) spt[m] = ev[m] * evol * md[m] * CPL(M_PTY, m, c);
+    }
     for (int q = 0; q < PL; ++q) sphi[q] += 1.0 * dphi[q];
+    if (POR && por) {
+        scv += 1.0 * dcv;
+        for (int m = 0; m < n; ++m) spt[m] += 1.0 * dpt[m];
+    }
     if (pml3) {
         if (sgx + sgy > 0.) {
             for (int q = 0; q < 4; ++q) beta[q] += -1.0 * (P.alpha_damp_x + P.alpha_damp_y) * beta[q] * dt;
@@ -1236,21 +1282,26 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
         double s1 = 1.0 / evol;
         for (int m = 0; m < n; ++m) ev[m] = s1 * sphi[m];
     }
+    S pt[Mats<NM>::cap];
+    if (POR && por) pt[0] = CPL(M_PTY, 0, c);
     for (int m = 1; m < n; ++m) {
         if (ev[m] < TOLVOF || (!pml3 && sphi[m + n] < TOLRHO * TOLVOF * evol)) {
             me[m] = 0.0; md[m] = 0.0; ev[m] = 0.0;
+            if (POR && por) pt[m] = 1.0;
         } else {
             me[m] = sphi[m + 2 * n] / (ev[m] * evol);
             md[m] = sphi[m + n] / (ev[m] * evol);
+            if (POR && por) pt[m] = spt[m] / sphi[m + n];
         }
     }
-    S dv0, dv1, ste[4], dam, tps, vib, ejt;
+    S dv0, dv1, ste[4], dam, tps, vib, ejt, cvs = 0.;
     if (emass > TOLRHO * evol) {
         S s1 = 1.0 / emass;
         dv0 = s1 * sphi[3 * n]; dv1 = s1 * sphi[3 * n + 1];
         for (int q = 0; q < 4; ++q) ste[q] = s1 * sphi[3 * n + 2 + q];
         dam = sphi[3 * n + 6] / emass; tps = sphi[3 * n + 7] / emass;
         vib = sphi[3 * n + 8] / emass; ejt = sphi[3 * n + 9] / emass;
+        if (POR && por) cvs = scv / emass;
     } else {
         dv0 = dv1 = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
         dam = tps = vib = ejt = 0.;
@@ -1259,11 +1310,19 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     tps = dWind(tps, 0.0, 10.0);
     dam = dWind(dam, 0.0, 1.0);
     ejt = dWind(ejt, 0.0, 1.0);
+    if (POR && por)
+        for (int m = 1; m < n; ++m) pt[m] = dWind(pt[m], 1.0, mats[m].alphamax2);
     if (eden < P.density_cutoff || (!pml3 && eden < 10.0 * P.density_cutoff * (1.0 - ev[0]))) {
         for (int m = 0; m < n; ++m) { ev[m] = 0.0; md[m] = 0.0; me[m] = 0.0; }
         ev[0] = 1.0;
         dv0 = dv1 = 0.0; ste[0] = ste[1] = ste[2] = ste[3] = 0.0;
         dam = tps = vib = ejt = 0.;
+        if (POR && por) { cvs = 0.; for (int m = 1; m < n; ++m) pt[m] = 1.0; }
+    }
+    if (POR) {
+        /* second buffer set: remapped, or carried over under pml3 */
+        for (int m = 0; m < n; ++m) CPN(M_PTY, m, c) = por ? pt[m] : S(CPL(M_PTY, m, c));
+        CPN(E_CVS, 0, c) = por ? cvs : S(CPL(E_CVS, 0, c));
     }
     for (int m = 0; m < n; ++m) { CPL(M_VOF, m, c) = ev[m]; CPN(M_DEN, m, c) = md[m]; CPN(M_ENG, m, c) = me[m]; }
     CPL(E_DVEL, 0, c) = dv0; CPL(E_DVEL, 1, c) = dv1;
@@ -1449,13 +1508,13 @@ void launch_advance_time(hipStream_t s, DevTime *tm) { k_advance_time<<<1, 64, 0
  * a cell's flux reads only its own energy, so the per-cell chain gives the
  * reference's values.  Cells of the top physical row also snapshot their
  * m_den and m_eng (see advect_cell). */
-template <class S, int NM>
+template <class S, int NM, bool POR>
 __global__ void __launch_bounds__(128) k_energy_flux(Grid g, Fields F, TF<S> T, const DevMat *mats, const DevTime *tm,
                                                      S *bx, S *by, S *top)
 {
     int k, j;
     if (!physical_cell(g, xcd_thread(), k, j)) return;
-    energy_cell<S, NM>(g, F, T, mats, tm_dt<S>(tm), k, j);
+    energy_cell<S, NM, POR>(g, F, T, mats, tm_dt<S>(tm), k, j);
     flux_cell<S, NM>(g, F, T, bx, by, k, j);
     if (j == g.CH - 3) {
         const int n = Mats<NM>::n(g);
@@ -1466,19 +1525,24 @@ __global__ void __launch_bounds__(128) k_energy_flux(Grid g, Fields F, TF<S> T, 
         }
     }
 }
-void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm,
-                        double *bx, double *by, double *top)
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats,
+                        const DevTime *tm, double *bx, double *by, double *top)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_energy_flux<double, decltype(I)::value><<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, NoTan{}, mats, tm,
-                                                                                                 bx, by, top);
+        por_dispatch(P.porosity, [&](auto Q) {
+            k_energy_flux<double, decltype(I)::value, decltype(Q)::value>
+                <<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, NoTan{}, mats, tm, bx, by, top);
+        });
     });
 }
-void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const DevMat *mats,
-                        const DevTime *tm, Dual *bx, Dual *by, Dual *top)
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P,
+                        const DevMat *mats, const DevTime *tm, Dual *bx, Dual *by, Dual *top)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_energy_flux<Dual, decltype(I)::value><<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, T, mats, tm, bx, by, top);
+        por_dispatch(P.porosity, [&](auto Q) {
+            k_energy_flux<Dual, decltype(I)::value, decltype(Q)::value>
+                <<<nblk8((long)g.nx * g.ny, 128), 128, 0, s>>>(g, F, T, mats, tm, bx, by, top);
+        });
     });
 }
 
@@ -1498,7 +1562,7 @@ void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFi
 #else
 #define REMAP_ATTR
 #endif
-template <class S, int NM>
+template <class S, int NM, bool POR>
 __device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F, const TF<S> &T, const Fields &Fn,
                                                  const TF<S> &Tn, const Params &P, const DevMat *mats, const S *bx,
                                                  const S *by, const S *top, const DevTime *tm)
@@ -1511,12 +1575,16 @@ __device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F,
     const S dt = tm_dt<S>(tm);
     const double t = tm->t;
     if (phys) {
-        advect_cell<S, NM>(g, F, T, Fn, Tn, P, bx, by, top, dt, k, j);
+        advect_cell<S, NM, POR>(g, F, T, Fn, Tn, P, mats, bx, by, top, dt, k, j);
     } else {
         for (int m = 0; m < n; ++m) { CPN(M_DEN, m, c) = CPL(M_DEN, m, c); CPN(M_ENG, m, c) = CPL(M_ENG, m, c); }
         for (int q = 0; q < 4; ++q) CPN(E_STE, q, c) = CPL(E_STE, q, c);
         CPN(E_DAM, 0, c) = CPL(E_DAM, 0, c); CPN(E_TPS, 0, c) = CPL(E_TPS, 0, c);
         CPN(E_VIB, 0, c) = CPL(E_VIB, 0, c); CPN(E_EJT, 0, c) = CPL(E_EJT, 0, c);
+        if (POR) {
+            for (int m = 0; m < n; ++m) CPN(M_PTY, m, c) = CPL(M_PTY, m, c);
+            CPN(E_CVS, 0, c) = CPL(E_CVS, 0, c);
+        }
     }
     if (!(P.damp_time <= t)) damp_cell<S, NM>(g, Fn, Tn, P, dt, c);
     if (phys) {
@@ -1526,25 +1594,28 @@ __device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F,
 }
 /* the double kernel may be given an occupancy target; the dual-number one
  * keeps the compiler's own register budget */
-template <int NM>
+template <int NM, bool POR>
 __global__ void __launch_bounds__(128) REMAP_ATTR k_remap_close(Grid g, Fields F, Fields Fn, Params P, const DevMat *mats,
                                                                 const double *bx, const double *by, const double *top,
                                                                 const DevTime *tm)
 {
-    remap_close_cell<double, NM>(g, F, NoTan{}, Fn, NoTan{}, P, mats, bx, by, top, tm);
+    remap_close_cell<double, NM, POR>(g, F, NoTan{}, Fn, NoTan{}, P, mats, bx, by, top, tm);
 }
-template <int NM>
+template <int NM, bool POR>
 __global__ void __launch_bounds__(128) k_remap_close_t(Grid g, Fields F, TFields T, Fields Fn, TFields Tn, Params P,
                                                        const DevMat *mats, const Dual *bx, const Dual *by, const Dual *top,
                                                        const DevTime *tm)
 {
-    remap_close_cell<Dual, NM>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
+    remap_close_cell<Dual, NM, POR>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
 }
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
                         const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_remap_close<decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
+        por_dispatch(P.porosity, [&](auto Q) {
+            k_remap_close<decltype(I)::value, decltype(Q)::value>
+                <<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
+        });
     });
 }
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Fields &Fn,
@@ -1552,7 +1623,10 @@ void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFi
                         const Dual *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        k_remap_close_t<decltype(I)::value><<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
+        por_dispatch(P.porosity, [&](auto Q) {
+            k_remap_close_t<decltype(I)::value, decltype(Q)::value>
+                <<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
+        });
     });
 }
 

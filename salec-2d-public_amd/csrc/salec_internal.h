@@ -26,7 +26,7 @@ struct AxisHint {
 
 /* material parameters as the kernels read them (state_reference + tables) */
 struct DevMat {
-    int eos_type, yfunc, dfunc, tfunc;
+    int eos_type, yfunc, dfunc, tfunc, pfunc, pad0_;
     int till_nden, tab_nden, tab_ntem, PolyLiq, PolySol, pad_;
     int sens_y0, sens_fr;           /* 1 + tangent direction of Yint0 / Yfricint, 0 = none */
     double TLRho0, TLCv, TLA, TLB, TLE0, TLa, TLb, TLAlpha, TLBeta, TLEiv, TLEcv, TLTref;
@@ -40,6 +40,7 @@ struct DevMat {
     double Tlidc1, Tlidc2, Tlidc3, Tsolc1, Tsolc2, Tsolc3;
     double Toff, Cvib, VibMax, Tdamp, Pvlim, Acvis;
     double alpha0, chi, gamfrac, melt_viscosity;
+    double alphax, epse0, kappa, avel, alphamax, epsex, epsec, alphae, alphamax2;
     AxisHint hden, heng;            /* search hints of the table axes       */
 };
 
@@ -53,6 +54,8 @@ struct Params {
     double avis1, avis2, anc, density_cutoff, velocity_max, velocity_min, acceleration_min;
     double alpha_damp_x, alpha_damp_y, damp_time, threshold_z, flow_resistance, flow_friction, max_dt;
     int partpres, tensile_failure, acoustic_fluid, damp_type;
+    int porosity;                   /* some material has a porosity model   */
+    int pad_;
 };
 
 struct Fields { double *f[SALEC_NFIELDS]; };
@@ -94,8 +97,8 @@ void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Pa
 void launch_advance_time(hipStream_t s, DevTime *tm);
 /* setup tail (src/salec2d.c:76-94) */
 void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, double *bx, double *by);
-void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const DevMat *mats, const DevTime *tm,
-                        double *bx, double *by, double *top);
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats,
+                        const DevTime *tm, double *bx, double *by, double *top);
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
                         const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm);
 void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part, int nblocks);
@@ -112,8 +115,8 @@ void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const TFields 
                    const DevTime *tm);
 void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, const DevTime *tm);
 void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const DevTime *tm);
-void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const DevMat *mats,
-                        const DevTime *tm, Dual *bx, Dual *by, Dual *top);
+void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P,
+                        const DevMat *mats, const DevTime *tm, Dual *bx, Dual *by, Dual *top);
 void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, Dual *bx, Dual *by);
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Fields &Fn,
                         const TFields &Tn, const Params &P, const DevMat *mats, const Dual *bx, const Dual *by,

@@ -648,14 +648,35 @@ int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out,
         r->Tdamp = gb * bs / 5000.0;
         r->melt_viscosity = inp_dk(f, "material.viscosityM", m, 0.0);
         inp_sk(f, "material.Porosity", m, pf, sizeof pf, "None");
-        r->alpha0 = 1.0; r->chi = 0.;
         r->yfunc = fn_by_name(yf); r->tfunc = fn_by_name(tf); r->dfunc = fn_by_name(df); r->pfunc = fn_by_name(pf);
         /* str2func accepts registry indices > 0 only (src/strength.c:590) */
         if (r->yfunc <= 0 || r->tfunc <= 0 || r->dfunc <= 0 || r->pfunc <= 0) {
             rc = SALEC_ENOSYM;
             ERR("undefined strength/damage/tensile/porosity function for material %d", m);
         }
-        if (r->pfunc != SALEC_FN_NONE) { rc = SALEC_EINVAL; ERR("porosity model '%s' is not supported", pf); }
+        /* epsilon-alpha parameters, src/sale2d_back.c:369-398 */
+        if (r->pfunc == SALEC_FN_WUNNEMANN) {
+            r->alpha0 = inp_dk(f, "material.alpha0", m, 1.0);
+            r->alphax = inp_dk(f, "material.alphax", m, 1.0);
+            r->epse0 = inp_dk(f, "material.epse0", m, -1.0e-5);
+            r->kappa = inp_dk(f, "material.kappa", m, 0.98);
+            r->chi = inp_dk(f, "material.chi", m, 1.0);
+            r->avel = inp_dk(f, "material.avel", m, 1e-4);
+            r->alphamax = r->alpha0;
+            r->epsex = r->epse0 + log(r->alphax / r->alpha0) / r->kappa;
+            r->epsec = r->epsex + 2.0 * (1.0 - r->alphax) / (r->kappa * r->alphax);
+            if (r->epsex < r->epsec) r->epsec = r->epsex;
+            r->alphae = r->alpha0 * (1.0 + (1.0 - r->chi * r->chi) * r->epse0);
+            r->alphamax2 = r->alpha0;
+        } else if (r->pfunc == SALEC_FN_NONE) {
+            /* the reference leaves alphamax2 unset here (undefined when another
+             * material is porous); 1 keeps the distension at 1 */
+            r->alpha0 = 1.0; r->epse0 = 0.; r->kappa = 0.; r->chi = 0.; r->alphax = 1.0; r->alphamax = 1.0;
+            r->alphamax2 = 1.0;
+        } else {
+            rc = SALEC_EINVAL;
+            ERR("undefined porosity model:%s", pf);
+        }
     }
     /* velocity / acceleration limits, src/sale2d.c:448-472 */
     h->p_vel[0] = inp_dk(f, "projectile.velocity", 0, 0.0);
@@ -1168,7 +1189,12 @@ static int build_profile(salec_host *h, tprof *P)
             if (r->PolySol > 0) { double p = P->pre[k] * 1.0e-9; mt = r->Tsolc1 + r->Tsolc2 * p + r->Tsolc3 * p * p; }
             double lt = Min(P->tem[k], mt);
             P->tem[k] = lt;
-            if (eos_tp(h, P->mat[k], lt, P->pre[k], &P->den[k], &P->eng[k]) != SALEC_OK) fail = 1;
+            /* CorrectionDensity1/2 (src/target_profile.c:215-290): the matrix
+             * state at alpha0 * p, divided by the distension alpha0 */
+            double pty = r->alpha0;
+            if (eos_tp(h, P->mat[k], lt, P->pre[k] * pty, &P->den[k], &P->eng[k]) != SALEC_OK) fail = 1;
+            P->den[k] = P->den[k] / pty;
+            P->eng[k] = P->eng[k] / pty;
         }
         if (fail) { fail_any = 1; break; }
         if (memcmp(prev, P->den, sizeof(double) * n) == 0) break;

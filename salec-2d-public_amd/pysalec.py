@@ -67,7 +67,8 @@ class Material(C.Structure):
         ("PolyLiq", C.c_int), ("PolySol", C.c_int)] + [(n, C.c_double) for n in (
         "Tlidc1", "Tlidc2", "Tlidc3", "Tsolc1", "Tsolc2", "Tsolc3", "Toff", "Cvib", "VibMax", "Tdamp", "Pvlim",
         "Acvis", "alpha0", "chi", "gamfrac", "den0", "melt_viscosity")] + [
-        ("yfunc", C.c_int), ("dfunc", C.c_int), ("tfunc", C.c_int), ("pfunc", C.c_int)]
+        ("yfunc", C.c_int), ("dfunc", C.c_int), ("tfunc", C.c_int), ("pfunc", C.c_int)] + [(n, C.c_double) for n in (
+        "alphax", "epse0", "kappa", "avel", "alphamax", "epsex", "epsec", "alphae", "alphamax2")]
 
 
 class Config(C.Structure):

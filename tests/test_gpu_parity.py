@@ -151,6 +151,15 @@ VARIANTS = {
     "damp_pml3_plain_hydro": ("c0", dict(damping="pml3")),
     # by step ~156 the wave carries momentum into the layer's flux terms
     "damp_pml3_wave_in_layer": ("c1", dict(damping="pml3"), 160, 25),
+    # epsilon-alpha porosity (WunnemannPorosity): porous regolith over solid basalt,
+    # both layers porous, and porosity under pml3 (distension not remapped there)
+    "porous_regolith": ("c1", dict(sections_over={"material.Porosity": "[, Wunnemann, None]", "material.alpha0": "[, 1.5, 1.0]",
+                                          "material.alphax": "[, 1.1, 1.0]", "material.chi": "[, 0.5, 1.0]"})),
+    "porous_all": ("c1", dict(sections_over={"material.Porosity": "[, Wunnemann, Wunnemann]",
+                                             "material.alpha0": "[, 1.5, 1.2]", "material.alphax": "[, 1.1, 1.05]",
+                                             "material.chi": "[, 0.5, 0.8]", "material.kappa": "[, 0.98, 0.9]"})),
+    "porous_pml3": ("c1", dict(damping="pml3", sections_over={"material.Porosity": "[, Wunnemann, None]", "material.alpha0": "[, 1.5, 1.0]",
+                                          "material.alphax": "[, 1.1, 1.0]", "material.chi": "[, 0.5, 1.0]"})),
     # the reference's sponge layer adds the reference state on top of the
     # damped one in y (src/velocity.c:746-747), so densities in the bottom
     # layer grow geometrically and dt collapses within ~20 steps: compare the

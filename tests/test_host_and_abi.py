@@ -95,7 +95,7 @@ def test_v_laplace_field_on_stretched_mesh(tmp_path):
 
 
 @pytest.mark.parametrize("over,what", [
-    ({"material.Porosity": "[, Wunnemann]"}, "orosity"),
+    ({"material.Porosity": "[, Ivanov]"}, "orosity"),
     ({"target.type": "sphere"}, "sphere"),
 ])
 def test_loader_rejects_unsupported(tmp_path, over, what):
@@ -275,3 +275,31 @@ def test_target_profile_states_lie_on_the_eos(eos_runs, which):
             assert ps.eos_tp(h.cfg.mat[m], T[j, k], Pr[j, k]) == (D[m, j, k], E[m, j, k]), (j, k)
             n += 1
     assert n > 20
+
+
+def test_porous_target_profile(tmp_path):
+    """A porous layer starts at distension alpha0 with the matrix state of
+    (T, alpha0 p) divided by alpha0 (CorrectionDensity1,
+    src/target_profile.c:215-232), and the epsilon-alpha parameters derive
+    as in src/sale2d_back.c:371-385."""
+    over = {"material.Porosity": "[, Wunnemann, None]", "material.alpha0": "[, 1.5, 1.0]",
+            "material.alphax": "[, 1.1, 1.0]", "material.chi": "[, 0.5, 1.0]"}
+    h = ps.HostRun(*ps.write_config(str(tmp_path / "c1"), "c1", nx=24, ny=36, sections_over=over))
+    m1, m2 = h.cfg.mat[1], h.cfg.mat[2]
+    assert m1.pfunc == ps.SALEC_FN_WUNNEMANN and m2.pfunc == ps.SALEC_FN_NONE
+    assert (m1.alpha0, m1.alphamax, m1.alphamax2, m2.alphamax2) == (1.5, 1.5, 1.5, 1.0)
+    assert m1.epsex < m1.epse0 and m1.epsec <= m1.epsex and m1.alphae < m1.alpha0
+    pty, vof = h.field("M_PTY"), h.field("M_VOF")
+    T, Pr, D, E = h.field("E_TEM")[0], h.field("E_PRE")[0], h.field("M_DEN"), h.field("M_ENG")
+    ey = h.field("E_POS")[1]
+    n = 0
+    for j in range(2, vof.shape[1] - 2):
+        for k in range(2, vof.shape[2] - 2, 2):
+            if vof[1, j, k] < 1.0 or ey[j, k] > -1.0:
+                continue
+            assert pty[1, j, k] == 1.5
+            d, e = ps.eos_tp(m1, T[j, k], Pr[j, k] * 1.5)
+            assert (D[1, j, k], E[1, j, k]) == (d / 1.5, e / 1.5), (j, k)
+            n += 1
+    assert n > 10
+    assert (pty[2][vof[2] >= 1.0] == 1.0).all()

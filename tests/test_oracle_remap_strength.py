@@ -10,7 +10,10 @@ models (src/strength.c:379-854) by what the mathematics fixes:
   is positive;
 * vofMSTACS: continuity at the switching points and boundedness
   (nD <= f <= min(nD/cd, 1) for Courant numbers below 1/3);
-* strength: limits, slopes, continuity and special cases of each model.
+* strength: limits, slopes, continuity and special cases of each model;
+* the epsilon-alpha porosity rate (WunnemannPorosity): its switches, and its
+  compaction laws integrated along a compressive path against their closed
+  forms (exponential above alpha_x, parabolic below).
 """
 import ctypes as C
 
@@ -230,3 +233,71 @@ def test_tensile_vonmises_johnsoncook_special_cases():
     y2 = Y(ps.SALEC_FN_JOHNSONCOOK1, m, rate=np.e)
     assert y2 / y0 == pytest.approx(1.0 + m.JCC, rel=1e-14)
     assert Y(ps.SALEC_FN_NONE, m, p=1e9) == 0.0
+
+
+def wunnemann_mat(alpha0=1.5, alphax=1.1, kappa=0.98, chi=0.5, epse0=-1e-5, avel=1e-4):
+    """material with the epsilon-alpha parameters and the derived ones of
+    src/sale2d_back.c:371-385"""
+    m = ps.Material()
+    m.alpha0, m.alphax, m.kappa, m.chi, m.epse0, m.avel = alpha0, alphax, kappa, chi, epse0, avel
+    m.alphamax = alpha0
+    m.epsex = epse0 + np.log(alphax / alpha0) / kappa
+    m.epsec = m.epsex + 2.0 * (1.0 - alphax) / (kappa * alphax)
+    m.epsec = min(m.epsec, m.epsex)
+    m.alphae = alpha0 * (1.0 + (1.0 - chi * chi) * epse0)
+    m.alphamax2 = alpha0
+    m.pfunc = ps.SALEC_FN_WUNNEMANN
+    return m
+
+
+def wrate(m, pty, cvs, pre, dcvs, u=10.0, w=0.0):
+    p = (C.c_double * 6)(pty, cvs, pre, dcvs, u, w)
+    return po.olib().oracle_porosity_wunnemann(p, C.byref(m))
+
+
+def test_porosity_rate_switches():
+    m = wunnemann_mat()
+    # no change of fully compacted, over-distended, tensile or resting cells
+    assert wrate(m, 1.0, -0.1, 1e8, -1.0) == 0.0
+    assert wrate(m, 1.6, -0.1, 1e8, -1.0) == 0.0
+    assert wrate(m, 1.3, -0.1, 0.0, -1.0) == 0.0
+    assert wrate(m, 1.3, -0.1, 1e8, -1.0, u=1e-5, w=0.0) == 0.0
+    # compaction removes distension under compression, and the rate is linear in the strain rate
+    r1, r2 = wrate(m, 1.3, -1.0, 1e8, -1.0), wrate(m, 1.3, -1.0, 1e8, -2.0)
+    assert r1 < 0.0 and r2 == 2.0 * r1
+    # above alpha_x the slope is kappa * alpha (exponential compaction), below it
+    # kappa * alpha_x * sqrt((alpha - 1)/(alpha_x - 1)); the two meet at alpha_x
+    assert np.isclose(r1, -m.kappa * 1.3, rtol=1e-15)
+    lo = wrate(m, m.alphax - 1e-12, -1.0, 1e8, -1.0)
+    hi = wrate(m, m.alphax + 1e-12, -1.0, 1e8, -1.0)
+    assert np.isclose(lo, hi, rtol=1e-5)
+    # the elastic regime (strain above the compaction threshold eps_c(alpha),
+    # here -0.146 at alpha 1.3) scales with 1 - c_ratio^2 and vanishes for a
+    # matrix sound speed ratio chi = 1
+    e = wrate(m, 1.3, -0.1, 1e8, -1.0)
+    assert wrate(m, 1.3, -0.15, 1e8, -1.0) == -m.kappa * 1.3
+    ratio = 1.0 + (m.chi - 1.0) * (1.3 - 1.0) / (m.alphamax - 1.0)
+    assert np.isclose(e, -1.3 * (1.0 - ratio * ratio), rtol=1e-15)
+    assert wrate(wunnemann_mat(chi=1.0), 1.3, 1.0, 1e8, -1.0) == 0.0
+
+
+def test_porosity_compaction_laws_integrate_to_closed_forms():
+    """Along a compressive path (strain rate -1, strain below the threshold),
+    d(alpha)/d(eps) = kappa alpha gives alpha = alpha_a exp(kappa (eps - eps_a))
+    above alpha_x, and kappa alpha_x sqrt((alpha-1)/(alpha_x-1)) gives
+    sqrt(alpha - 1) = sqrt(alpha_b - 1) + kappa alpha_x (eps - eps_b) / (2 sqrt(alpha_x - 1))
+    below it.  Integrated with classical RK4 in the strain."""
+    m = wunnemann_mat()
+    f = lambda a: wrate(m, a, -1.0, 1e8, -1.0) / -1.0      # d(alpha)/d(eps)
+    def rk4(a, h, n):
+        for _ in range(n):
+            k1 = f(a); k2 = f(a + 0.5 * h * k1); k3 = f(a + 0.5 * h * k2); k4 = f(a + h * k3)
+            a += h / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        return a
+    h, n = -1e-4, 1000            # eps decreases by 0.1
+    a = rk4(1.45, h, n)
+    assert np.isclose(a, 1.45 * np.exp(m.kappa * h * n), rtol=1e-10)
+    b0 = 1.08
+    b = rk4(b0, h, 200)
+    want = (np.sqrt(b0 - 1.0) + m.kappa * m.alphax * (h * 200) / (2.0 * np.sqrt(m.alphax - 1.0))) ** 2 + 1.0
+    assert np.isclose(b, want, rtol=1e-8)
