@@ -97,6 +97,10 @@ struct salec_ctx {
     unsigned long long *d_nf = nullptr;
     std::vector<DevMat> dm_host;         /* host copy of d_mats */
     SensState *sens = nullptr;
+    double *d_geo = nullptr;             /* geometry row pool (Grid::geo)  */
+    int *d_georow = nullptr;
+    int geo_dirty = 1;                   /* geometry uploaded since the pool was built */
+    long geo_rows = 0;
 };
 
 /* Captured step graphs bake in kernel arguments (damp time, tracer count,
@@ -603,6 +607,8 @@ void salec_destroy(salec_ctx *c)
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     sens_free(c);
+    if (c->d_geo) hipFree(c->d_geo);
+    if (c->d_georow) hipFree(c->d_georow);
     for (hipGraphExec_t &e : c->gexec) if (e) hipGraphExecDestroy(e);
     for (void *p : c->owned) hipFree(p);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -615,6 +621,7 @@ int salec_upload(salec_ctx *c, int field, const double *host)
     if (!c || !host) return SALEC_EINVAL;
     int grid, nc;
     if (salec_field_info(field, c->g.nmat, &grid, &nc)) return SALEC_EINVAL;
+    for (int gf = 0; gf < GF_COUNT; ++gf) if (kGeoField[gf] == field) c->geo_dirty = 1;
     size_t n = (size_t)salec_grid_points(grid, c->g.nx, c->g.ny) * nc;
     HIPCHK(hipMemcpyAsync(c->F.f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -704,6 +711,60 @@ int salec_get_time(salec_ctx *c, double *t, double *dt, long *step)
     return SALEC_OK;
 }
 
+/* Build the geometry row pool (Grid::geo) from the device fields: every
+ * row of every listed (field, component) maps to the first pool row with
+ * the same bytes.  Runs on the host at set-up; never inside a capture. */
+static int build_geo(salec_ctx *c)
+{
+    Grid &g = c->g;
+    const int PW = std::max(g.VW, g.CW), RH = std::max(g.VH, g.CH);
+    std::vector<double> pool;
+    std::vector<int> rowmap((size_t)kGeoComps * RH, 0);
+    std::vector<int> width;
+    std::unordered_map<uint64_t, std::vector<int>> buckets;
+    for (int gf = 0; gf < GF_COUNT; ++gf) {
+        int f = kGeoField[gf], grid, nc;
+        salec_field_info(f, g.nmat, &grid, &nc);
+        if (nc != kGeoNC[gf]) return SALEC_EINVAL;
+        int W = grid == SALEC_GRID_CELL ? g.CW : grid == SALEC_GRID_VERTEX ? g.VW : grid == SALEC_GRID_XFACE ? g.BXW : g.BYW;
+        int H = grid == SALEC_GRID_CELL ? g.CH : grid == SALEC_GRID_VERTEX ? g.VH : grid == SALEC_GRID_XFACE ? g.BXH : g.BYH;
+        long N = (long)W * H;
+        std::vector<double> h((size_t)nc * N);
+        HIPCHK(hipMemcpyAsync(h.data(), c->F.f[f], sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int comp = 0; comp < nc; ++comp)
+            for (int j = 0; j < H; ++j) {
+                const double *row = h.data() + comp * N + (long)j * W;
+                uint64_t hs = 1469598103934665603ull ^ (uint64_t)W;
+                const unsigned char *b = (const unsigned char *)row;
+                for (size_t q = 0; q < sizeof(double) * W; ++q) hs = (hs ^ b[q]) * 1099511628211ull;
+                int r = -1;
+                for (int cand : buckets[hs])
+                    if (width[cand] == W && !std::memcmp(pool.data() + (size_t)cand * PW, row, sizeof(double) * W)) { r = cand; break; }
+                if (r < 0) {
+                    r = (int)width.size();
+                    width.push_back(W);
+                    pool.resize((size_t)(r + 1) * PW, 0.0);
+                    std::memcpy(pool.data() + (size_t)r * PW, row, sizeof(double) * W);
+                    buckets[hs].push_back(r);
+                }
+                rowmap[(size_t)(kGeoBase[gf] + comp) * RH + j] = r;
+            }
+    }
+    if (c->d_geo) { hipFree(c->d_geo); c->d_geo = nullptr; }
+    if (c->d_georow) { hipFree(c->d_georow); c->d_georow = nullptr; }
+    HIPCHK(hipMalloc(&c->d_geo, sizeof(double) * pool.size()));
+    HIPCHK(hipMalloc(&c->d_georow, sizeof(int) * rowmap.size()));
+    HIPCHK(hipMemcpy(c->d_geo, pool.data(), sizeof(double) * pool.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_georow, rowmap.data(), sizeof(int) * rowmap.size(), hipMemcpyHostToDevice));
+    g.geo = c->d_geo; g.georow = c->d_georow; g.geoRH = RH; g.geoPW = PW;
+    c->geo_rows = (long)width.size();
+    c->geo_dirty = 0;
+    drop_graphs(c);
+    return SALEC_OK;
+}
+static int ensure_geo(salec_ctx *c) { return c->geo_dirty ? build_geo(c) : SALEC_OK; }
+
 /* setup tail of main(), src/salec2d.c:76-94 */
 int salec_init(salec_ctx *c, double *damp_time_io)
 {
@@ -732,6 +793,7 @@ int salec_init(salec_ctx *c, double *damp_time_io)
     HIPCHK(hipMemsetAsync(c->F.f[SALEC_E_DVEL], 0, sizeof(double) * 2 * c->g.NC, s));
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
+    if (int rg = ensure_geo(c)) return rg;   /* geometry is complete once the run is initialised */
     c->initialized = 1;
     return SALEC_OK;
 }
@@ -760,6 +822,7 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
 {
     if (!c) return SALEC_EINVAL;
     if (c->sens && c->sens->params) return SALEC_ESTATE;
+    if (c->geo_dirty) { int rc = build_geo(c); if (rc) return rc; }
     const bool to_out = flags & SALEC_STEP_TO_OUTPUT, from_out = flags & SALEC_STEP_FROM_OUTPUT;
     if (to_out && from_out) return SALEC_EINVAL;
     if (from_out != (bool)c->half_open) return SALEC_ESTATE;
@@ -920,6 +983,7 @@ int salec_sens_begin(salec_ctx *c, unsigned params, unsigned mat_mask, int surfa
     if (c->half_open || c->sens) return SALEC_ESTATE;
     if (surface_row < 2 || surface_row > c->g.VH - 3) return SALEC_EINVAL;
     if ((params & SALEC_SENS_VELOCITY) && !(impact_speed > 0.)) return SALEC_EINVAL;
+    if (int rg = ensure_geo(c)) return rg;
     const Grid &g = c->g;
     SensState *S = new SensState();
     c->sens = S;
@@ -1068,6 +1132,7 @@ int salec_checkpoint(salec_ctx *c, const char *path)
 int salec_restore(salec_ctx *c, const char *path)
 {
     if (!c || !path) return SALEC_EINVAL;
+    c->geo_dirty = 1;
     FILE *fp = std::fopen(path, "rb");
     if (!fp) return SALEC_EINVAL;
     CkptHeader h{};
@@ -1128,6 +1193,8 @@ struct salec_batch {
 static int capture_graphs(salec_ctx *c)
 {
     if (c->prof_mask) return SALEC_ESTATE;   /* phase events are not recorded into graphs */
+    int rg = ensure_geo(c);                  /* built outside the capture */
+    if (rg) return rg;
     if (c->gexec[0] && c->gexec[1]) return SALEC_OK;
     drop_graphs(c);
     HIPCHK(hipStreamSynchronize(c->stream));

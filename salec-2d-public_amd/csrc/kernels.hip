@@ -21,6 +21,10 @@ namespace salec {
 #define CPL(fld, c, i) fa<S, SALEC_##fld>(F.f, T, (long)(c) * g.NC + (i))
 #define VPL(fld, c, i) fa<S, SALEC_##fld>(F.f, T, (long)(c) * g.NV + (i))
 #define PRIMAL_SCOPE using S = double; const NoTan T{}; (void)T
+/* static geometry from the deduplicated row pool: component c of field fld
+ * at column k of grid row j (the grid of the field) */
+#define GEO(fld, c, k, j) \
+    (g.geo[(long)g.georow[(kGeoBase[GF_##fld] + (c)) * g.geoRH + (j)] * g.geoPW + (k)])
 
 /* dt of the step as the scalar type S (value and its derivatives) */
 template <class S> __device__ __forceinline__ S tm_dt(const DevTime *tm);
@@ -85,6 +89,17 @@ template <class L> static void por_dispatch(int por, L launch)
 {
     if (por) launch(std::integral_constant<bool, true>{});
     else launch(std::integral_constant<bool, false>{});
+}
+/* remap variant: bit 0 porosity, bit 1 pml3 */
+template <class L> static void var_dispatch(const Params &P, L launch)
+{
+    const int v = (P.porosity ? 1 : 0) | (P.damp_type == SALEC_DAMP_PML3 ? 2 : 0);
+    switch (v) {
+    case 0: launch(std::integral_constant<int, 0>{}); break;
+    case 1: launch(std::integral_constant<int, 1>{}); break;
+    case 2: launch(std::integral_constant<int, 2>{}); break;
+    default: launch(std::integral_constant<int, 3>{}); break;
+    }
 }
 template <class L> static void nm_dispatch(int nmat, L launch)
 {
@@ -330,7 +345,7 @@ __global__ void k_vertex_vof_vel(Grid g, Fields F, TF<S> T, Params P)
         if (CPL(E_WPT, voff[i], e[i]) > 0.) vc++;
         S den = CPL(E_DEN, 0, e[i]);
         if (den < TOLRHO) continue;
-        double lv = VPL(V_VOL, i + 1, v);
+        double lv = GEO(V_VOL, i + 1, k, j);
         vvol += lv;
         vmass += den * lv;
         S pa = den * lv;
@@ -478,11 +493,11 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T,
         S vx[4], vy[4];
         for (int i = 0; i < 4; ++i) { vx[i] = VPL(V_VEL, 0, vv[i]); vy[i] = VPL(V_VEL, 1, vv[i]); }
         S g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
-        for (int i = 0; i < 4; ++i) g0 += CPL(E_GRAD, i, c) * vx[i];
-        for (int i = 0; i < 4; ++i) g2 += CPL(E_GRAD, i, c) * vy[i];
-        for (int i = 0; i < 4; ++i) g1 += CPL(E_GRAD, 4 + i, c) * vx[i];
-        for (int i = 0; i < 4; ++i) g3 += CPL(E_GRAD, 4 + i, c) * vy[i];
-        for (int i = 0; i < 4; ++i) g4 += CPL(E_GRAD, 8 + i, c) * vx[i];
+        for (int i = 0; i < 4; ++i) g0 += GEO(E_GRAD, i, k, j) * vx[i];
+        for (int i = 0; i < 4; ++i) g2 += GEO(E_GRAD, i, k, j) * vy[i];
+        for (int i = 0; i < 4; ++i) g1 += GEO(E_GRAD, 4 + i, k, j) * vx[i];
+        for (int i = 0; i < 4; ++i) g3 += GEO(E_GRAD, 4 + i, k, j) * vy[i];
+        for (int i = 0; i < 4; ++i) g4 += GEO(E_GRAD, 8 + i, k, j) * vx[i];
         CPL(E_GRAD_VEL, 0, c) = g0; CPL(E_GRAD_VEL, 1, c) = g1; CPL(E_GRAD_VEL, 2, c) = g2;
         CPL(E_GRAD_VEL, 3, c) = g3; CPL(E_GRAD_VEL, 4, c) = g4;
         S div = g0 + g3 + g4;
@@ -492,7 +507,7 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T,
         if (v0 > TOLVOF || CPL(E_PRE, 0, c) <= 0.0) {
             q = 0.;
         } else {
-            double ea = CPL(E_VOL, 0, c) / CPL(E_POS, 0, c);
+            double ea = GEO(E_VOL, 0, k, j) / GEO(E_POS, 0, k, j);
             ea = fabs(ea);
             S ql = P.avis1 * csd * sqrt(ea);
             S qq = P.avis2 * div * ea;
@@ -503,8 +518,8 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T,
         if (!(v0 > 1.0 - TOLVOF)) {
             double cyl = 1.0;
             S ev0 = CPL(E_VEL, 0, c), ev1 = CPL(E_VEL, 1, c);
-            S dr = cyl * CPL(E_POS, 2, c) / (fabs(ev0) + csd);
-            S dz = 1.0 * CPL(E_POS, 3, c) / (fabs(ev1) + csd);
+            S dr = cyl * GEO(E_POS, 2, k, j) / (fabs(ev0) + csd);
+            S dz = 1.0 * GEO(E_POS, 3, k, j) / (fabs(ev1) + csd);
             cand = DTF * (dr < dz ? dr : dz);
             has = true;
         }
@@ -561,7 +576,8 @@ void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const TFie
 /* block_vibration (src/strength.c:625-695) then update_e_stress
  * (src/strength.c:7-349, avg_strength mixing) for one physical cell. */
 template <class S, int NM>
-__global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, TF<S> T, Params P, const DevMat *mats, const DevTime *tm)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(std::is_same<S, double>::value ? 3 : 1)))
+k_stress(Grid g, Fields F, TF<S> T, Params P, const DevMat *mats, const DevTime *tm)
 {
     int k, j;
     if (!physical_cell(g, xcd_thread(), k, j)) return;
@@ -691,7 +707,7 @@ __global__ void __launch_bounds__(256) k_stress(Grid g, Fields F, TF<S> T, Param
             S Tmax = dMax(0.5 * (Sxx + Szz) - Ts, 0.5 * (Sxx + Szz) + Ts);
             Tmax = dMax(Tmax, Sth);
             if ((Tmax - pre) > lts) {
-                S dinc = 0.4 * csd / dMin(CPL(E_POS, 1, c), CPL(E_POS, 2, c));
+                S dinc = 0.4 * csd / dMin(GEO(E_POS, 1, k, j), GEO(E_POS, 2, k, j));
                 S dold = dam;
                 S t3 = pow(dold, 1.0 / 3.);
                 t3 += dinc;
@@ -738,13 +754,17 @@ __global__ void k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
     double vs0 = 0., vs1 = 0.;
     S al0 = 0., al1 = 0.;
     bool damp = false;
+    const int ek[4] = {k - 1, k, k, k - 1}, ej[4] = {j - 1, j - 1, j, j};
     if (PML) {
-        for (int i = 0; i < 4; ++i) { vs0 += 0.25 * CPL(E_SIGMA, 0, e[i]); vs1 += 0.25 * CPL(E_SIGMA, 1, e[i]); }
+        for (int i = 0; i < 4; ++i) {
+            vs0 += 0.25 * GEO(E_SIGMA, 0, ek[i], ej[i]);
+            vs1 += 0.25 * GEO(E_SIGMA, 1, ek[i], ej[i]);
+        }
         al0 = VPL(V_ALPHA, 0, v); al1 = VPL(V_ALPHA, 1, v);
         damp = (vs0 > 0. || vs1 > 0.);
     }
     for (int i = 0; i < 4; ++i) {
-        double su0 = VPL(V_SUF, 2 * i, v), su1 = VPL(V_SUF, 2 * i + 1, v), sv = VPL(V_SUBVOL, i, v);
+        double su0 = GEO(V_SUF, 2 * i, k, j), su1 = GEO(V_SUF, 2 * i + 1, k, j), sv = GEO(V_SUBVOL, i, k, j);
         S lp = CPL(E_PRE, 0, e[i]) + CPL(E_Q, 0, e[i]);
         S s0 = CPL(E_STE, 0, e[i]), s1 = CPL(E_STE, 1, e[i]), s2 = CPL(E_STE, 2, e[i]), s3 = CPL(E_STE, 3, e[i]);
         if (PML) {
@@ -775,7 +795,7 @@ __global__ void k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
             a0 += -1.0 * f0; a1 += -1.0 * f1;
             a0 += 1.0 * fp0; a1 += 1.0 * fp1;
         }
-        double lv = VPL(V_VOL, 1 + i, v);
+        double lv = GEO(V_VOL, 1 + i, k, j);
         vm += lv * CPL(E_DEN, 0, e[i]);
         vv += lv;
     }
@@ -799,7 +819,7 @@ __global__ void k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
                 L0 += wt * VPL(V_VEL, 0, w);
                 L1 += wt * VPL(V_VEL, 1, w);
             }
-        double sq = sqrt(fabs(VPL(V_SUBVOL, 0, v)) + fabs(VPL(V_SUBVOL, 2, v)));
+        double sq = sqrt(fabs(GEO(V_SUBVOL, 0, k, j)) + fabs(GEO(V_SUBVOL, 2, k, j)));
         n0 = 5.0e3 * sq * L0;
         n1 = 5.0e3 * sq * L1;
     }
@@ -807,10 +827,10 @@ __global__ void k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
     if (vm > TOLRHO * vv) {
         S px = 1.0 / vm;
         a0 *= px; a1 *= px;
-        a0 += 1.0 * VPL(V_BF, 0, v); a1 += 1.0 * VPL(V_BF, 1, v);
+        a0 += 1.0 * GEO(V_BF, 0, k, j); a1 += 1.0 * GEO(V_BF, 1, k, j);
         a0 += P.anc * n0; a1 += P.anc * n1;
     } else if (PML) {
-        a0 = VPL(V_BF, 0, v); a1 = VPL(V_BF, 1, v);
+        a0 = GEO(V_BF, 0, k, j); a1 = GEO(V_BF, 1, k, j);
     } else {
         a0 = 0.0; a1 = 0.0;
     }
@@ -889,7 +909,7 @@ __device__ __forceinline__ void energy_cell(const Grid &g, const Fields &F, cons
     bool outside_layer = true;
     S cvs0 = 0., pre0 = 0., ev0 = 0., ev1 = 0.;
     if (POR) {
-        outside_layer = CPL(E_SIGMA, 0, c) + CPL(E_SIGMA, 1, c) <= 0.;
+        outside_layer = GEO(E_SIGMA, 0, k, j) + GEO(E_SIGMA, 1, k, j) <= 0.;
         cvs0 = CPL(E_CVS, 0, c); pre0 = CPL(E_PRE, 0, c); ev0 = CPL(E_VEL, 0, c); ev1 = CPL(E_VEL, 1, c);
     }
     for (int m = 1; m < n; ++m) {
@@ -987,7 +1007,7 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
     S lbru[4] = {lv, bv, rv, uv};
     S ev[Mats<NM>::cap];
     for (int m = 0; m < n; ++m) ev[m] = CPL(M_VOF, m, c);
-    double evol = CPL(E_VOL, 0, c);
+    double evol = GEO(E_VOL, 0, k, j);
     S fv[Mats<NM>::cap][4];
     int pm = 0;
     S evpm = ev[0];
@@ -1003,12 +1023,12 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
         for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = w;
     } else {
         double gr[8];
-        for (int i = 0; i < 8; ++i) gr[i] = CPL(E_GRAD, i, c);
+        for (int i = 0; i < 8; ++i) gr[i] = GEO(E_GRAD, i, k, j);
         S g0 = 0, g1 = 0;
         for (int i = 0; i < 4; ++i) g0 += VPL(V_VOF, pm, vv[i]) * gr[i];
         for (int i = 0; i < 4; ++i) g1 += VPL(V_VOF, pm, vv[i]) * gr[4 + i];
         vnorm2(g0, g1);
-        S ca = CPL(E_POS, 2, c) * fabs(g0), cb = CPL(E_POS, 3, c) * fabs(g1);
+        S ca = GEO(E_POS, 2, k, j) * fabs(g0), cb = GEO(E_POS, 3, k, j) * fabs(g1);
         if (ca < cb) { S t = ca; ca = cb; cb = t; }
         S dstar = cut_length(evpm, ca, cb);
         S w[4];
@@ -1028,10 +1048,10 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
         long nb[4] = {k - 1 + (long)g.CW * j, k + (long)g.CW * (j - 1), k + 1 + (long)g.CW * j, k + (long)g.CW * (j + 1)};
         bool nbok[4] = {k - 1 >= 0, j - 1 >= 0, k + 1 < g.CW, j + 1 < g.CH};
         double bn[4][2];
-        bn[0][0] = F.f[SALEC_BX_NORMAL][k + (long)g.BXW * j]; bn[0][1] = F.f[SALEC_BX_NORMAL][g.NBX + k + (long)g.BXW * j];
-        bn[2][0] = F.f[SALEC_BX_NORMAL][k + 1 + (long)g.BXW * j]; bn[2][1] = F.f[SALEC_BX_NORMAL][g.NBX + k + 1 + (long)g.BXW * j];
-        bn[1][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * j]; bn[1][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * j];
-        bn[3][0] = F.f[SALEC_BY_NORMAL][k + (long)g.BYW * (j + 1)]; bn[3][1] = F.f[SALEC_BY_NORMAL][g.NBY + k + (long)g.BYW * (j + 1)];
+        bn[0][0] = GEO(BX_NORMAL, 0, k, j); bn[0][1] = GEO(BX_NORMAL, 1, k, j);
+        bn[2][0] = GEO(BX_NORMAL, 0, k + 1, j); bn[2][1] = GEO(BX_NORMAL, 1, k + 1, j);
+        bn[1][0] = GEO(BY_NORMAL, 0, k, j); bn[1][1] = GEO(BY_NORMAL, 1, k, j);
+        bn[3][0] = GEO(BY_NORMAL, 0, k, j + 1); bn[3][1] = GEO(BY_NORMAL, 1, k, j + 1);
         S fnp[Mats<NM>::cap][4];
         S snp[4] = {0., 0., 0., 0.};
         for (int m = 0; m < n; ++m) fnp[m][0] = fnp[m][1] = fnp[m][2] = fnp[m][3] = 0.;
@@ -1139,21 +1159,22 @@ void launch_flux_ring(hipStream_t s, const Grid &g, const Fields &F, const TFiel
  * A face carries flux iff its swept volume is non-zero; the donor is the
  * side on which it is negative (exactly antisymmetric, src/advect.c:1347). */
 #define CPN(fld, c, i) fa<S, SALEC_##fld>(Fn.f, Tn, (long)(c) * g.NC + (i))
-template <class S, int NM, bool POR, class TT>
+template <class S, int NM, int VAR, class TT>
 __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, const TT &T, const Fields &Fn, const TT &Tn,
                                             const Params &P, const DevMat *mats, const S *bx, const S *by, const S *top,
                                             S dt, int k, int j)
 {
     long c = k + (long)g.CW * j;
     const int n = Mats<NM>::n(g), PL = 3 * n + 10;
+    constexpr bool POR = VAR & 1;
     /* pml3 (advect_flux_pml3, src/advect.c:304-593): the flux terms of the
      * damping layer accumulate into e_beta, which relaxes and feeds the
      * mass, energy and momentum of the cell */
-    const bool pml3 = P.damp_type == SALEC_DAMP_PML3;
+    constexpr bool pml3 = VAR & 2;
     double sgx = 0., sgy = 0.;
     S beta[4];
     if (pml3) {
-        sgx = CPL(E_SIGMA, 0, c); sgy = CPL(E_SIGMA, 1, c);
+        sgx = GEO(E_SIGMA, 0, k, j); sgy = GEO(E_SIGMA, 1, k, j);
         for (int q = 0; q < 4; ++q) beta[q] = CPL(E_BETA, q, c);
     }
     const bool pmlc = pml3 && (sgx > 0. || sgy > 0.);
@@ -1161,7 +1182,7 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
     /* porosity (advect_flux_default with porosity_model, src/advect.c:98-102,
      * 179-186, 235-240, 253-293): cold volume strain and each material's
      * distension travel with the mass; advect_flux_pml3 has no such terms */
-    const bool por = POR && !pml3;
+    constexpr bool por = POR && !pml3;
     S dcv = 0., dpt[Mats<NM>::cap];
     if (POR) for (int m = 0; m < n; ++m) dpt[m] = 0.;
     long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
@@ -1217,7 +1238,7 @@ __device__ __forceinline__ void advect_cell(const Grid &g, const Fields &F, cons
             }
         }
     }
-    double evol = CPL(E_VOL, 0, c);
+    double evol = GEO(E_VOL, 0, k, j);
     S emass = evol * CPL(E_DEN, 0, c);
     S sphi[3 * Mats<NM>::cap + 10];
     S ev[Mats<NM>::cap], md[Mats<NM>::cap], me[Mats<NM>::cap];
@@ -1338,7 +1359,8 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
 {
     const int n = Mats<NM>::n(g);
     if (CPL(M_VOF, 0, c) > TOLVOF) return;
-    double sx = CPL(E_SIGMA, 0, c), sy = CPL(E_SIGMA, 1, c);
+    const int k = (int)(c % g.CW), j = (int)(c / g.CW);
+    double sx = GEO(E_SIGMA, 0, k, j), sy = GEO(E_SIGMA, 1, k, j);
     double adx = P.alpha_damp_x, ady = P.alpha_damp_y;
     if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3) {
         /* pml2 and pml3 (damp_on_cell_pml3, src/velocity.c:591-686) differ
@@ -1431,9 +1453,10 @@ __device__ __forceinline__ void ejecta_cell(const Grid &g, const Fields &F, cons
     long c = k + (long)g.CW * j;
     long v0 = k - 1 + (long)g.VW * (j - 1), v1 = k + (long)g.VW * (j - 1);
     double gy = 0.;
-    gy += 0.5 * VPL(V_BF, 1, v0);
-    gy += 0.5 * VPL(V_BF, 1, v1);
-    if (CPL(E_POS, 1, c) <= F.f[SALEC_Y_PROFILE][k] && CPL(E_EJT, 0, c) > 0.1) {
+    gy += 0.5 * GEO(V_BF, 1, k - 1, j - 1);
+    gy += 0.5 * GEO(V_BF, 1, k, j - 1);
+    (void)v0; (void)v1;
+    if (GEO(E_POS, 1, k, j) <= F.f[SALEC_Y_PROFILE][k] && CPL(E_EJT, 0, c) > 0.1) {
         if (P.flow_resistance > 0.) {
             S pa = -P.flow_resistance * dt;
             CPL(E_DVEL, 0, c) += pa * CPL(E_VEL, 0, c);
@@ -1562,7 +1585,7 @@ void launch_energy_flux(hipStream_t s, const Grid &g, const Fields &F, const TFi
 #else
 #define REMAP_ATTR
 #endif
-template <class S, int NM, bool POR>
+template <class S, int NM, int VAR>
 __device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F, const TF<S> &T, const Fields &Fn,
                                                  const TF<S> &Tn, const Params &P, const DevMat *mats, const S *bx,
                                                  const S *by, const S *top, const DevTime *tm)
@@ -1575,13 +1598,13 @@ __device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F,
     const S dt = tm_dt<S>(tm);
     const double t = tm->t;
     if (phys) {
-        advect_cell<S, NM, POR>(g, F, T, Fn, Tn, P, mats, bx, by, top, dt, k, j);
+        advect_cell<S, NM, VAR>(g, F, T, Fn, Tn, P, mats, bx, by, top, dt, k, j);
     } else {
         for (int m = 0; m < n; ++m) { CPN(M_DEN, m, c) = CPL(M_DEN, m, c); CPN(M_ENG, m, c) = CPL(M_ENG, m, c); }
         for (int q = 0; q < 4; ++q) CPN(E_STE, q, c) = CPL(E_STE, q, c);
         CPN(E_DAM, 0, c) = CPL(E_DAM, 0, c); CPN(E_TPS, 0, c) = CPL(E_TPS, 0, c);
         CPN(E_VIB, 0, c) = CPL(E_VIB, 0, c); CPN(E_EJT, 0, c) = CPL(E_EJT, 0, c);
-        if (POR) {
+        if (VAR & 1) {
             for (int m = 0; m < n; ++m) CPN(M_PTY, m, c) = CPL(M_PTY, m, c);
             CPN(E_CVS, 0, c) = CPL(E_CVS, 0, c);
         }
@@ -1594,25 +1617,25 @@ __device__ __forceinline__ void remap_close_cell(const Grid &g, const Fields &F,
 }
 /* the double kernel may be given an occupancy target; the dual-number one
  * keeps the compiler's own register budget */
-template <int NM, bool POR>
+template <int NM, int VAR>
 __global__ void __launch_bounds__(128) REMAP_ATTR k_remap_close(Grid g, Fields F, Fields Fn, Params P, const DevMat *mats,
                                                                 const double *bx, const double *by, const double *top,
                                                                 const DevTime *tm)
 {
-    remap_close_cell<double, NM, POR>(g, F, NoTan{}, Fn, NoTan{}, P, mats, bx, by, top, tm);
+    remap_close_cell<double, NM, VAR>(g, F, NoTan{}, Fn, NoTan{}, P, mats, bx, by, top, tm);
 }
-template <int NM, bool POR>
+template <int NM, int VAR>
 __global__ void __launch_bounds__(128) k_remap_close_t(Grid g, Fields F, TFields T, Fields Fn, TFields Tn, Params P,
                                                        const DevMat *mats, const Dual *bx, const Dual *by, const Dual *top,
                                                        const DevTime *tm)
 {
-    remap_close_cell<Dual, NM, POR>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
+    remap_close_cell<Dual, NM, VAR>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
 }
 void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const Fields &Fn, const Params &P,
                         const DevMat *mats, const double *bx, const double *by, const double *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        por_dispatch(P.porosity, [&](auto Q) {
+        var_dispatch(P, [&](auto Q) {
             k_remap_close<decltype(I)::value, decltype(Q)::value>
                 <<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, Fn, P, mats, bx, by, top, tm);
         });
@@ -1623,7 +1646,7 @@ void launch_remap_close(hipStream_t s, const Grid &g, const Fields &F, const TFi
                         const Dual *top, const DevTime *tm)
 {
     nm_dispatch(g.nmat, [&](auto I) {
-        por_dispatch(P.porosity, [&](auto Q) {
+        var_dispatch(P, [&](auto Q) {
             k_remap_close_t<decltype(I)::value, decltype(Q)::value>
                 <<<nblk8(g.NC, 128), 128, 0, s>>>(g, F, T, Fn, Tn, P, mats, bx, by, top, tm);
         });

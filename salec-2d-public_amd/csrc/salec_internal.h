@@ -44,10 +44,26 @@ struct DevMat {
     AxisHint hden, heng;            /* search hints of the table axes       */
 };
 
+/* Static geometry as deduplicated rows.  Each (field, component) of the
+ * list below maps every grid row to a row of one shared pool; equal rows
+ * (bit for bit) share a pool row, so the kernels read the same values from
+ * a table small enough to stay in cache.  On a uniform axisymmetric mesh
+ * most geometry repeats row to row. */
+enum { GF_V_SUF = 0, GF_V_SUBVOL, GF_V_VOL, GF_V_BF, GF_E_GRAD, GF_E_VOL, GF_E_SIGMA, GF_BX_NORMAL, GF_BY_NORMAL,
+       GF_E_POS, GF_COUNT };
+constexpr int kGeoField[GF_COUNT] = {SALEC_V_SUF, SALEC_V_SUBVOL, SALEC_V_VOL, SALEC_V_BF, SALEC_E_GRAD, SALEC_E_VOL,
+                                     SALEC_E_SIGMA, SALEC_BX_NORMAL, SALEC_BY_NORMAL, SALEC_E_POS};
+constexpr int kGeoNC[GF_COUNT] = {8, 4, 5, 2, 12, 1, 2, 2, 2, 4};
+constexpr int kGeoBase[GF_COUNT] = {0, 8, 12, 17, 19, 31, 32, 34, 36, 38};
+constexpr int kGeoComps = 42;
+
 struct Grid {
     int nx, ny, nmat, PL;           /* PL = face packet length 3*nmat+10    */
     int CW, CH, VW, VH, BXW, BXH, BYW, BYH;
     long NC, NV, NBX, NBY;
+    const double *geo;              /* pool of distinct geometry rows, width geoPW */
+    const int *georow;              /* [kGeoComps][geoRH] pool row of each grid row */
+    int geoRH, geoPW;
 };
 
 struct Params {
