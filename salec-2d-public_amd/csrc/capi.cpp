@@ -63,6 +63,8 @@ struct SensState {
 struct salec_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t tstream = nullptr;       /* tracer update, overlapped with energy/flux */
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     salec_config cfg{};
     Grid g{};
     Params P{};
@@ -507,6 +509,9 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     P.porosity = 0;
     for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].pfunc != SALEC_FN_NONE) P.porosity = 1;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SALEC_EDEVICE; }
+    if (hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) { salec_destroy(c); return SALEC_EDEVICE; }
 #define TRY(x) do { rc = (x); if (rc != SALEC_OK) { salec_destroy(c); return rc; } } while (0)
     for (int f = 0; f < SALEC_NFIELDS; ++f) {
         int grid, nc;
@@ -612,6 +617,9 @@ void salec_destroy(salec_ctx *c)
     for (hipGraphExec_t &e : c->gexec) if (e) hipGraphExecDestroy(e);
     for (void *p : c->owned) hipFree(p);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->tstream) { hipStreamSynchronize(c->tstream); hipStreamDestroy(c->tstream); }
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -622,6 +630,7 @@ int salec_upload(salec_ctx *c, int field, const double *host)
     int grid, nc;
     if (salec_field_info(field, c->g.nmat, &grid, &nc)) return SALEC_EINVAL;
     for (int gf = 0; gf < GF_COUNT; ++gf) if (kGeoField[gf] == field) c->geo_dirty = 1;
+    if (field == SALEC_M_PTY) c->geo_dirty = 1;
     size_t n = (size_t)salec_grid_points(grid, c->g.nx, c->g.ny) * nc;
     HIPCHK(hipMemcpyAsync(c->F.f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -759,6 +768,13 @@ static int build_geo(salec_ctx *c)
     HIPCHK(hipMemcpy(c->d_georow, rowmap.data(), sizeof(int) * rowmap.size(), hipMemcpyHostToDevice));
     g.geo = c->d_geo; g.georow = c->d_georow; g.geoRH = RH; g.geoPW = PW;
     c->geo_rows = (long)width.size();
+    /* without a porosity model m_pty never changes: note whether it is 1 */
+    c->P.pty_one = 0;
+    if (!c->P.porosity) {
+        std::vector<double> p((size_t)g.nmat * g.NC);
+        HIPCHK(hipMemcpy(p.data(), c->F.f[SALEC_M_PTY], sizeof(double) * p.size(), hipMemcpyDeviceToHost));
+        c->P.pty_one = std::all_of(p.begin(), p.end(), [](double x) { return x == 1.0; }) ? 1 : 0;
+    }
     c->geo_dirty = 0;
     drop_graphs(c);
     return SALEC_OK;
@@ -861,10 +877,23 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     PHASE(PH_RST_ACC, run_program(c, c->rstA));
     PHASE(PH_VDIS, launch_v_dis(s, g, F, c->d_time));
     PHASE(PH_RST_DIS, run_program(c, c->rstD));
-    PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
+    /* the tracer update reads v_dis/v_vel/v_acc and the cells' m_vof, e_pre
+     * and e_tem, none of which energy/flux/ghost/flux-ring write: it runs on
+     * its own stream beside them and joins before the output point.  When
+     * its own phase is timed, it runs in line so that time is its own. */
+    const bool overlap = !(mask & (1u << PH_TRACERS)) && c->T.n > 0;
+    if (overlap) {
+        HIPCHK(hipEventRecord(c->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(c->tstream, c->ev_fork, 0));
+        launch_tracers(c->tstream, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time);
+        HIPCHK(hipEventRecord(c->ev_join, c->tstream));
+    } else {
+        PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
+    }
     PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, P, c->d_mats, c->d_time, c->bx, c->by, c->d_top));
     PHASE(PH_GF_C, run_program(c, c->gfC); run_program(c, c->gfP));
     PHASE(PH_FLUX_RING, launch_flux_ring(s, g, F, c->bx, c->by));
+    if (overlap) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
     }
     c->half_open = to_out;
     if (!to_out) {
@@ -940,7 +969,8 @@ static void run_program_sens(salec_ctx *c, const Program &P)
     run_program(c, P);
 }
 
-/* one loop iteration on Dual values (salec_step's sequence without tracers) */
+/* one loop iteration on Dual values (salec_step's sequence without
+ * tracers); phases are timed under the same mask as salec_step's */
 static void sens_step_once(salec_ctx *c)
 {
     SensState *S = c->sens;
@@ -948,33 +978,45 @@ static void sens_step_once(salec_ctx *c)
     const Grid &g = c->g;
     const Params &P = c->P;
     const TFields &T = S->T;
-    launch_trace_profile(s, g, c->F, P, c->d_prof, c->prof_chunks);
-    launch_profile_finish(s, g, c->F, P, c->d_prof, c->prof_chunks);
-    run_program_sens(c, c->gfA);
-    launch_vertex_vof_vel(s, g, c->F, T, P);
-    run_program_sens(c, c->rstV);
-    launch_update_e_vel(s, g, c->F, T);
-    run_program_sens(c, c->gfE);
-    launch_check_v_vel(s, g, c->F, T);
-    launch_gradvel_dt(s, g, c->F, T, P, c->d_time, S->part, c->gv_blocks);
-    launch_stress(s, g, c->F, T, P, c->d_mats, c->d_time);
-    run_program_sens(c, c->gfB);
-    launch_acc(s, g, c->F, T, P, c->d_time);
-    run_program_sens(c, c->rstA);
-    launch_v_dis(s, g, c->F, T, c->d_time);
-    run_program_sens(c, c->rstD);
-    launch_energy_flux(s, g, c->F, T, P, c->d_mats, c->d_time, S->bx, S->by, S->top);
-    run_program_sens(c, c->gfC);
-    run_program_sens(c, c->gfP);
-    launch_flux_ring(s, g, c->F, T, S->bx, S->by);
+    const unsigned mask = c->prof_mask;
+    auto T0 = [&](int ph) {
+        if (!(mask & (1u << ph))) return;
+        size_t i = c->ev_used;
+        hipEvent_t e = next_event(c);
+        if (e) { hipEventRecord(e, s); c->ev_pending.push_back({ph, i}); }
+    };
+    auto T1 = [&](int ph) {
+        if (!(mask & (1u << ph))) return;
+        hipEvent_t e = next_event(c);
+        if (e) hipEventRecord(e, s);
+    };
+#define PHASE(ph, ...) do { T0(ph); __VA_ARGS__; T1(ph); } while (0)
+    PHASE(PH_PROFILE, launch_trace_profile(s, g, c->F, P, c->d_prof, c->prof_chunks);
+                      launch_profile_finish(s, g, c->F, P, c->d_prof, c->prof_chunks));
+    PHASE(PH_GF_A, run_program_sens(c, c->gfA));
+    PHASE(PH_VERTEX, launch_vertex_vof_vel(s, g, c->F, T, P));
+    PHASE(PH_RST_VEL, run_program_sens(c, c->rstV));
+    PHASE(PH_E_VEL, launch_update_e_vel(s, g, c->F, T));
+    PHASE(PH_GF_EVEL, run_program_sens(c, c->gfE));
+    PHASE(PH_CHECK_V, launch_check_v_vel(s, g, c->F, T));
+    PHASE(PH_GRADVEL_DT, launch_gradvel_dt(s, g, c->F, T, P, c->d_time, S->part, c->gv_blocks));
+    PHASE(PH_STRESS, launch_stress(s, g, c->F, T, P, c->d_mats, c->d_time));
+    PHASE(PH_GF_B, run_program_sens(c, c->gfB));
+    PHASE(PH_ACC, launch_acc(s, g, c->F, T, P, c->d_time));
+    PHASE(PH_RST_ACC, run_program_sens(c, c->rstA));
+    PHASE(PH_VDIS, launch_v_dis(s, g, c->F, T, c->d_time));
+    PHASE(PH_RST_DIS, run_program_sens(c, c->rstD));
+    PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, c->F, T, P, c->d_mats, c->d_time, S->bx, S->by, S->top));
+    PHASE(PH_GF_C, run_program_sens(c, c->gfC); run_program_sens(c, c->gfP));
+    PHASE(PH_FLUX_RING, launch_flux_ring(s, g, c->F, T, S->bx, S->by));
     Fields Fn = c->F;
     TFields Tn = S->T;
     for (int f : c->swapped) { Fn.f[f] = c->Falt.f[f]; Tn.t[f] = S->Talt.t[f]; }
-    launch_remap_close(s, g, c->F, T, Fn, Tn, P, c->d_mats, S->bx, S->by, S->top, c->d_time);
+    PHASE(PH_REMAP_CLOSE, launch_remap_close(s, g, c->F, T, Fn, Tn, P, c->d_mats, S->bx, S->by, S->top, c->d_time));
     for (int f : c->swapped) { std::swap(c->F.f[f], c->Falt.f[f]); std::swap(S->T.t[f], S->Talt.t[f]); }
     c->parity ^= 1;
-    launch_advance_time(s, c->d_time);
-    launch_sens_accum(s, g, c->F, S->T, S->acc, S->row);
+    PHASE(PH_TIME, launch_advance_time(s, c->d_time); launch_sens_accum(s, g, c->F, S->T, S->acc, S->row));
+#undef PHASE
 }
 
 int salec_sens_begin(salec_ctx *c, unsigned params, unsigned mat_mask, int surface_row, double impact_speed)

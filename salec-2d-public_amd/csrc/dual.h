@@ -13,7 +13,9 @@
  * field element; with S = Dual it is a DRef that reads and writes the value
  * plane and the NTAN tangent planes of the field together.  Fields whose
  * values never depend on the parameters (mesh geometry, PML profiles,
- * the ejecta flag, the crater profile) have no tangent planes and
+ * the ejecta flag, the crater profile), or whose values enter no other
+ * value (the face weights e_wpt, read only by comparisons, and the
+ * output-only e_strength), have no tangent planes and
  * are read as plain doubles in both modes.
  */
 #pragma once
@@ -177,7 +179,7 @@ template <class S> using TF = typename std::conditional<std::is_same<S, double>:
 /* fields whose values never depend on the parameters */
 __host__ __device__ constexpr bool tangent_field(int f)
 {
-    return !(f == SALEC_E_EJT || f == SALEC_E_POS || f == SALEC_E_VOL || f == SALEC_E_GRAD ||
+    return !(f == SALEC_E_EJT || f == SALEC_E_WPT || f == SALEC_E_STRENGTH || f == SALEC_E_POS || f == SALEC_E_VOL || f == SALEC_E_GRAD ||
              f == SALEC_E_SIGMA || f == SALEC_E_DAMPREF || f == SALEC_V_POS || f == SALEC_V_BF ||
              f == SALEC_V_VOL || f == SALEC_V_SUBVOL || f == SALEC_V_SUF || f == SALEC_V_LAPLACE ||
              f == SALEC_BX_NORMAL || f == SALEC_BY_NORMAL || f == SALEC_Y_PROFILE);

@@ -720,7 +720,7 @@ k_stress(Grid g, Fields F, TF<S> T, Params P, const DevMat *mats, const DevTime 
     }
     for (int i = 0; i < 4; ++i) CPL(E_STE, i, c) = ste[i];
     CPL(E_STE2S, 0, c) = ste2s; CPL(E_STA2S, 0, c) = sta2s; CPL(E_TPS, 0, c) = tps;
-    CPL(E_DAM, 0, c) = dam; CPL(E_STRENGTH, 0, c) = strength;
+    CPL(E_DAM, 0, c) = dam; CPL(E_STRENGTH, 0, c) = val(strength);
 }
 void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm)
 {
@@ -742,7 +742,7 @@ void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const TFields 
  * vertices after the e_pre/e_ste/e_q ghost fills (internal vertices never
  * read the cells those fills write). */
 template <class S, bool PML>
-__global__ void k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
+__global__ void __launch_bounds__(256) k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
 {
     int k, j;
     if (!active_vertex(g, xcd_thread(), k, j)) return;
@@ -1037,7 +1037,7 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
         S mw2 = w[2] > w[3] ? w[2] : w[3];
         mw = mw > mw2 ? mw : mw2;
         for (int i = 0; i < 4; ++i) w[i] = w[i] - mw + dstar;
-        for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = (pm == 0) ? w[i] : S(-0.25);
+        for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = (pm == 0) ? val(w[i]) : -0.25;
         S fp[4];
         fp[0] = geo_ratio(w[3], w[0]);
         fp[2] = geo_ratio(w[2], w[1]);
@@ -1389,8 +1389,9 @@ __device__ __forceinline__ void damp_cell(const Grid &g, const Fields &F, const 
             }
             S dd = (-(sx + sy) * (CPL(M_DEN, m, c) - r0) - r0 * a0) * dt;
             S de = (-(sx + sy) * (CPL(M_ENG, m, c) - r1) - a1) * dt;
-            CPL(M_DEN, m, c) += dd / CPL(M_PTY, m, c);
-            CPL(M_ENG, m, c) += de / CPL(M_PTY, m, c);
+            S pty = P.pty_one ? S(1.0) : S(CPL(M_PTY, m, c));
+            CPL(M_DEN, m, c) += dd / pty;
+            CPL(M_ENG, m, c) += de / pty;
             break;
         }
     } else if (P.damp_type == SALEC_DAMP_PML) {
@@ -1485,7 +1486,10 @@ __device__ __forceinline__ void state_cell(const Grid &g, const Fields &F, const
         if (!(vf < TOLVOF)) {
             const DevMat &r = mats[m];
             S pmin = (1.0 - dam) * r.minPint + dam * r.minPdam;
-            eos_state_capped<S>(r, md, CPL(M_ENG, m, c), pmin, CPL(M_PTY, m, c), &mt, &mp, &mc);
+            /* a distension of exactly 1 changes nothing in the EOS (x * 1 and
+             * x / 1 are exact), so it is not read when it is known to be 1 */
+            S pty = P.pty_one ? S(1.0) : S(CPL(M_PTY, m, c));
+            eos_state_capped<S>(r, md, CPL(M_ENG, m, c), pmin, pty, &mt, &mp, &mc);
         }
         pre += vf * mp;
         csd += vf / dMax(1.0, mc);

@@ -71,7 +71,7 @@ struct Params {
     double alpha_damp_x, alpha_damp_y, damp_time, threshold_z, flow_resistance, flow_friction, max_dt;
     int partpres, tensile_failure, acoustic_fluid, damp_type;
     int porosity;                   /* some material has a porosity model   */
-    int pad_;
+    int pty_one;                    /* no porosity model and every m_pty is 1 */
 };
 
 struct Fields { double *f[SALEC_NFIELDS]; };
