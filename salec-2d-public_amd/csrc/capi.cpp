@@ -63,8 +63,6 @@ struct SensState {
 struct salec_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t tstream = nullptr;       /* tracer update, overlapped with energy/flux */
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     salec_config cfg{};
     Grid g{};
     Params P{};
@@ -509,9 +507,6 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     P.porosity = 0;
     for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].pfunc != SALEC_FN_NONE) P.porosity = 1;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SALEC_EDEVICE; }
-    if (hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) { salec_destroy(c); return SALEC_EDEVICE; }
 #define TRY(x) do { rc = (x); if (rc != SALEC_OK) { salec_destroy(c); return rc; } } while (0)
     for (int f = 0; f < SALEC_NFIELDS; ++f) {
         int grid, nc;
@@ -617,9 +612,6 @@ void salec_destroy(salec_ctx *c)
     for (hipGraphExec_t &e : c->gexec) if (e) hipGraphExecDestroy(e);
     for (void *p : c->owned) hipFree(p);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-    if (c->tstream) { hipStreamSynchronize(c->tstream); hipStreamDestroy(c->tstream); }
-    if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -877,23 +869,10 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     PHASE(PH_RST_ACC, run_program(c, c->rstA));
     PHASE(PH_VDIS, launch_v_dis(s, g, F, c->d_time));
     PHASE(PH_RST_DIS, run_program(c, c->rstD));
-    /* the tracer update reads v_dis/v_vel/v_acc and the cells' m_vof, e_pre
-     * and e_tem, none of which energy/flux/ghost/flux-ring write: it runs on
-     * its own stream beside them and joins before the output point.  When
-     * its own phase is timed, it runs in line so that time is its own. */
-    const bool overlap = !(mask & (1u << PH_TRACERS)) && c->T.n > 0;
-    if (overlap) {
-        HIPCHK(hipEventRecord(c->ev_fork, s));
-        HIPCHK(hipStreamWaitEvent(c->tstream, c->ev_fork, 0));
-        launch_tracers(c->tstream, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time);
-        HIPCHK(hipEventRecord(c->ev_join, c->tstream));
-    } else {
-        PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
-    }
+    PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
     PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, P, c->d_mats, c->d_time, c->bx, c->by, c->d_top));
     PHASE(PH_GF_C, run_program(c, c->gfC); run_program(c, c->gfP));
     PHASE(PH_FLUX_RING, launch_flux_ring(s, g, F, c->bx, c->by));
-    if (overlap) HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
     }
     c->half_open = to_out;
     if (!to_out) {
