@@ -18,6 +18,9 @@ static double Max(double a, double b) { return a > b ? a : b; }
 static double Min(double a, double b) { return a < b ? a : b; }
 static double Sqrt_trunc(double x) { return x < 1.0e-2 ? -sqrt(fabs(x)) : sqrt(x); }
 
+enum { TEM_CONST, TEM_CONDUCTIVE, TEM_CONVECTIVE, TEM_LUNAR };
+enum { ADD_SPHERE, ADD_CUBE };
+#define MAX_ADD 32
 struct salec_host {
     salec_config cfg;
     salec_material mat[SALEC_MAXMAT];
@@ -51,9 +54,16 @@ struct salec_host {
     /* projectile */
     int p_mat;
     double p_radius, p_center[2], p_vel[2], p_dam, p_pre, p_tem;
-    /* target */
+    /* target (target_info, src/sale2d.h:262-284) */
     int t_plane, n_layer, t_mat[10], tem_type, n_profile;
     double t_depth[10], max_depth, surf_pre, surf_tem, surf_dtdz, r_planet;
+    double surf_gravity, t_lith, d_lith, d_mantle, t_mantle, t_center[2];
+    /* StateReference thermexp and c_heat of each material (the convective
+     * and lunar temperature profiles use them) */
+    double thermexp[SALEC_MAXMAT], c_heat[SALEC_MAXMAT];
+    /* target additions (target_addition_info, src/sale2d.h:288-310) */
+    int n_add, add_type[MAX_ADD], add_mat[MAX_ADD];
+    double add_p[MAX_ADD][12];
     /* fields */
     double *f[SALEC_NFIELDS];
     double x0, y0;
@@ -315,7 +325,7 @@ static int sd_interp_tp(const salec_material *M, const sd_ranges *R, double T, d
  * density at (Pnorm, Tnorm) and the state_reference values derived from
  * the table around it.  The sound speed keeps the reference's two-term sum
  * (a stray ';' at :244 drops the other two corners). */
-static int sd_state_ref(salec_material *M, double pnorm, double tnorm)
+static int sd_state_ref(salec_material *M, double pnorm, double tnorm, double *thermexp, double *c_heat)
 {
     int ti, di;
     double xl[2];
@@ -333,9 +343,9 @@ static int sd_state_ref(salec_material *M, double pnorm, double tnorm)
     M->MeltDen = 0.85 * dnorm;
     M->VaporDen = 0.04 * dnorm;
     M->den0 = dnorm;
-    double thermexp = (dpde / dtde) / (cs * cs * dnorm);
-    double c_heat = 1.0 / dtde;
-    M->gamfrac = thermexp / c_heat;
+    *thermexp = (dpde / dtde) / (cs * cs * dnorm);
+    *c_heat = 1.0 / dtde;
+    M->gamfrac = *thermexp / *c_heat;
     return SALEC_OK;
 }
 
@@ -401,6 +411,8 @@ static int load_tillotson(salec_host *h, int m, const char *path)
     M->VaporDen = 1.0e-4 * p[RHO0] * 10.0;
     M->den0 = p[RHO0];
     M->gamfrac = (p[Lb] + p[La]) / (p[TA] / p[RHO0]);
+    h->c_heat[m] = p[CV];
+    h->thermexp[m] = p[CV] * (p[Lb] + p[La]) / (p[TA] / p[RHO0]);
     return SALEC_OK;
 }
 
@@ -435,7 +447,7 @@ static int load_sd_table(salec_host *h, int m, const char *path)
     M->tab_nden = nden; M->tab_ntem = nene;
     h->tab_yden[m] = yd; h->tab_xtem[m] = xt; h->tab_data[m] = dd;
     M->tab_yden = yd; M->tab_xtem = xt; M->tab_data = dd;
-    int rc = sd_state_ref(M, pn, tn);
+    int rc = sd_state_ref(M, pn, tn, &h->thermexp[m], &h->c_heat[m]);
     return rc ? rc : sd_build_ranges(h, m);
 }
 
@@ -709,13 +721,21 @@ int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out,
     h->p_pre = inp_dk(f, "projectile.pressure", 1, 1.0);
     h->p_tem = inp_dk(f, "projectile.temperature", 1, 293.0);
     c->threshold_z = ej_thr * h->p_radius;
-    /* target, src/target_profile.c:345-483 (plane; const/surface/conductive) */
+    /* target, load_target_info (src/target_profile.c:345-483) */
     h->n_profile = 3001;
     h->n_layer = inp_i(f, "target.number", 1);
+    if (h->n_layer < 1 || h->n_layer > 10) ERR("target.number out of range");
     inp_s(f, "target.type", buf, sizeof buf, "plane");
-    if (strcasecmp(buf, "plane") != 0) ERR("target.type '%s' is not supported", buf);
-    h->t_plane = 1;
-    for (int l = 0; l < h->n_layer && l < 10; ++l) {
+    if (!strcasecmp(buf, "plane")) {
+        h->t_plane = 1;
+    } else if (!strcasecmp(buf, "sphere")) {
+        h->t_plane = 0;
+        h->r_planet = inp_d(f, "target.r_planet", 1738.0e3);
+        h->t_center[0] = 0.0; h->t_center[1] = -h->r_planet;
+    } else {
+        ERR("target.type '%s' is not supported", buf);
+    }
+    for (int l = 0; l < h->n_layer; ++l) {
         h->t_depth[l] = inp_dk(f, "target.depth", l, 1.);
         char tn[80];
         inp_sk(f, "target.material", l, tn, sizeof tn, "unknown");
@@ -724,14 +744,66 @@ int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out,
         if (h->t_mat[l] < 0) ERR("cannot get material id of %s in target layer", tn);
     }
     h->max_depth = inp_d(f, "target.max_depth", 1.);
+    if (!h->t_plane) h->max_depth = h->r_planet;
     inp_sk(f, "target.temperature", 0, buf, sizeof buf, "const");
     h->surf_pre = inp_d(f, "target.surf_pressure", 1.0e-5);
     h->surf_tem = inp_d(f, "target.surf_temperature", -1.0);
     if (h->surf_tem < 0.) h->surf_tem = inp_dk(f, "target.temperature", 1, 293.0);
-    if (!strcasecmp(buf, "const") || !strcasecmp(buf, "surface")) h->tem_type = 0;
-    else if (!strcasecmp(buf, "conductive")) { h->tem_type = 1; h->surf_dtdz = inp_d(f, "target.surf_dtdz", 0.0); h->r_planet = inp_d(f, "target.r_planet", 1738.0e3); }
-    else ERR("target temperature profile '%s' is not supported", buf);
-    if (inp_i(f, "target.addition", 0) > 0 || inp_has(f, "addition.number")) ERR("target additions are not supported");
+    if (!strcasecmp(buf, "const")) {
+        h->tem_type = TEM_CONST;
+    } else if (!strcasecmp(buf, "surface")) {
+        h->tem_type = TEM_CONST;
+        h->surf_tem = inp_dk(f, "target.temperature", 1, 293.0);
+    } else if (!strcasecmp(buf, "conductive")) {
+        /* the reference reads no radius for a plane conductive target and
+         * divides by an unset r_planet; the host reads target.r_planet */
+        h->tem_type = TEM_CONDUCTIVE;
+        h->surf_dtdz = inp_d(f, "target.surf_dtdz", 0.0);
+        h->r_planet = inp_d(f, "target.r_planet", 1738.0e3);
+    } else if (!strcasecmp(buf, "convective") || !strcasecmp(buf, "lunar")) {
+        int lunar = !strcasecmp(buf, "lunar");
+        h->tem_type = lunar ? TEM_LUNAR : TEM_CONVECTIVE;
+        h->surf_gravity = inp_d(f, "target.surf_gravity", -1.63);
+        h->surf_dtdz = inp_d(f, "target.surf_dtdz", 0.0);
+        h->t_lith = inp_d(f, "target.t_lith", -1.0);
+        h->d_lith = inp_d(f, "target.d_lith", -1.0);
+        h->r_planet = inp_d(f, "target.r_planet", 1738.0e3);
+        h->d_mantle = inp_d(f, "target.d_mantle", lunar ? 560.0e3 : -1.0);
+        h->t_mantle = inp_d(f, "target.t_mantle", -1.0);
+        h->t_center[0] = 0.0; h->t_center[1] = -h->r_planet;
+        /* ConsistentLith, src/target_profile.c:67-83: one of t_lith and
+         * d_lith gives the other; with both or neither set the values stay */
+        if (h->t_lith >= 0 && h->d_lith < 0) {
+            double D = 1.0 - 2.0 * (h->t_lith - h->surf_tem) / (h->r_planet * h->surf_dtdz);
+            D = Max(0.0, D);
+            h->d_lith = h->r_planet * (1.0 - sqrt(D));
+        } else if (h->t_lith < 0 && h->d_lith >= 0) {
+            h->t_lith = h->surf_tem + h->surf_dtdz * h->d_lith * (1.0 - 0.5 * h->d_lith / h->r_planet);
+        } else {
+            fprintf(stderr, "salec: target.t_lith and target.d_lith: exactly one must be set\n");
+        }
+    } else {
+        ERR("target temperature profile '%s' is not supported", buf);
+    }
+    /* target additions, load_target_addition_info (src/target_profile.c:486-568) */
+    h->n_add = inp_i(f, "addition.nobject", 0);
+    if (h->n_add > MAX_ADD) ERR("addition.nobject above %d", MAX_ADD);
+    for (int o = 0; o < h->n_add; ++o) {
+        char tn[80], ty[80], pn[80], key[100];
+        inp_sk(f, "addition.material", o, tn, sizeof tn, "unknown");
+        h->add_mat[o] = -1;
+        for (int m = 0; m < c->nmat; ++m) if (!strcasecmp(tn, h->name[m])) { h->add_mat[o] = m; break; }
+        if (h->add_mat[o] < 0) ERR("cannot get material id of %s in target addition", tn);
+        inp_sk(f, "addition.type", o, ty, sizeof ty, "unknown");
+        inp_sk(f, "addition.parameter", o, pn, sizeof pn, "unknown");
+        snprintf(key, sizeof key, "addition.%s", pn);
+        if (!inp_has(f, key)) ERR("can not find parameter:%s", key);
+        int np;
+        if (!strcasecmp(ty, "cube")) { h->add_type[o] = ADD_CUBE; np = 12; }
+        else if (!strcasecmp(ty, "sphere")) { h->add_type[o] = ADD_SPHERE; np = 9; }
+        else ERR("no method to initialize addition %d of type '%s'", o, ty);
+        for (int q = 0; q < 12; ++q) h->add_p[o][q] = q < np ? inp_dk(f, key, q, -1.0) : -1.0;
+    }
     inp_close(f);
     f = NULL;
     c->mat = h->mat;
@@ -1136,8 +1208,85 @@ static void init_mesh(salec_host *h)
         }
 }
 
-/* build_target_profile for a plane target, src/target_profile.c:86-343,1161-1196 */
-typedef struct { int n; double dz; double *pre, *rad, *den, *tem, *eng; int *mat; } tprof;
+/* build_target_profile, src/target_profile.c:1161-1196, with
+ * SetTargetProfile_Mat/_Tem/_Gra (:86-188), IntegrateGravity (:294-343) and
+ * CorrectionDensity1/2 (:203-290).  rad[k] runs from the bottom of the
+ * profile (the planet's centre for a sphere) to the surface at k = n-1. */
+typedef struct { int n; double dz; double *pre, *rad, *den, *tem, *eng, *gra; int *mat; } tprof;
+
+/* SolidusTem, src/strength.c:529-565 */
+static double solidus(const salec_material *r, double pre)
+{
+    double mt = r->Tmelt0 * pow(r->Asimon * pre + 1.0, r->Csimon);
+    if (r->PolySol > 0) { double p = pre * 1.0e-9; mt = r->Tsolc1 + r->Tsolc2 * p + r->Tsolc3 * p * p; }
+    return mt;
+}
+
+/* IntegrateGravity: a sphere integrates g from the centre (trapezoid in
+ * density, midpoint radius squared) and then p from the surface; a plane
+ * keeps g = condition.gravity[1] and integrates p. */
+static void integrate_gravity(const salec_host *h, tprof *P)
+{
+    int n = P->n;
+    double *den = P->den, *rad = P->rad, *gra = P->gra, *pre = P->pre;
+    if (!h->t_plane) {
+        gra[0] = 0.0;
+        for (int k = 1; k < n; ++k)
+            gra[k] = gra[k - 1] - 4.0 * M_PI * 6.674e-11 * (den[k] + den[k - 1]) * (rad[k] + rad[k - 1]) *
+                                  (rad[k] + rad[k - 1]) / 8.0 * (rad[k] - rad[k - 1]);
+        for (int k = 1; k < n; ++k) gra[k] = gra[k] / (rad[k] * rad[k]);
+        for (int k = n - 2; k >= 0; --k)
+            pre[k] = pre[k + 1] + (den[k] + den[k + 1]) * (gra[k] + gra[k + 1]) * (rad[k] - rad[k + 1]) / 4.0;
+    } else {
+        for (int k = n - 2; k >= 0; --k) {
+            double drg = 0.25 * (den[k + 1] + den[k]) * (gra[k] + gra[k + 1]);
+            pre[k] = pre[k + 1] + drg * (rad[k] - rad[k + 1]);
+        }
+    }
+}
+
+/* CorrectDensity: the temperature cap (CorrectionDensity1) or the
+ * conduction-lithosphere/convection-mantle temperature (CorrectionDensity2,
+ * convective profiles), then the EOS state of (T, alpha0 p) / alpha0. */
+static int correct_density(const salec_host *h, tprof *P)
+{
+    int n = P->n;
+    if (h->tem_type == TEM_CONVECTIVE) {
+        /* sequential in k: T[k] is built on the capped T[k+1] */
+        for (int k = n - 1; k >= 0; --k) {
+            const salec_material *r = &h->mat[P->mat[k]];
+            double kd = P->rad[n - 1] - P->rad[k];
+            double gc = h->surf_dtdz * (1.0 + kd / h->r_planet);
+            P->tem[k] = h->surf_tem;
+            if (k != n - 1) {
+                double cc = h->thermexp[P->mat[k]] * fabs(P->gra[k]) / h->c_heat[P->mat[k]];
+                double gv = P->tem[k + 1] * cc;
+                double cond = 1.0 - (kd - 0.9 * h->d_lith) / (0.1 * h->d_lith);
+                cond = Wind(cond, 0.0, 1.0);
+                if (h->d_mantle >= h->d_lith && kd < h->d_mantle) cond = 1.0;
+                P->tem[k] = P->tem[k + 1] + (gc * cond + gv * (1.0 - cond)) * (P->rad[k + 1] - P->rad[k]);
+            }
+            double mt = solidus(r, P->pre[k]);
+            if (!(h->d_mantle >= h->d_lith && kd > h->d_mantle)) P->tem[k] = Min(P->tem[k], mt);
+        }
+    } else {
+        for (int k = 0; k < n; ++k) P->tem[k] = Min(P->tem[k], solidus(&h->mat[P->mat[k]], P->pre[k]));
+    }
+    int fail = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : fail)
+    for (int k = 0; k < n; ++k) {
+        double pty = h->mat[P->mat[k]].alpha0;
+        if (eos_tp(h, P->mat[k], P->tem[k], P->pre[k] * pty, &P->den[k], &P->eng[k]) != SALEC_OK) fail = 1;
+        P->den[k] = P->den[k] / pty;
+        P->eng[k] = P->eng[k] / pty;
+    }
+    return fail ? SALEC_EINVAL : SALEC_OK;   /* a profile state lies outside an EOS table */
+}
+
+static void free_profile(tprof *P)
+{
+    free(P->pre); free(P->rad); free(P->den); free(P->tem); free(P->eng); free(P->gra); free(P->mat);
+}
 
 static int build_profile(salec_host *h, tprof *P)
 {
@@ -1146,8 +1295,9 @@ static int build_profile(salec_host *h, tprof *P)
     int n = P->n;
     P->pre = (double *)calloc(n, sizeof(double)); P->rad = (double *)calloc(n, sizeof(double));
     P->den = (double *)calloc(n, sizeof(double)); P->tem = (double *)calloc(n, sizeof(double));
-    P->eng = (double *)calloc(n, sizeof(double)); P->mat = (int *)calloc(n, sizeof(int));
-    if (!P->pre || !P->rad || !P->den || !P->tem || !P->eng || !P->mat) return SALEC_ENOMEM;
+    P->eng = (double *)calloc(n, sizeof(double)); P->gra = (double *)calloc(n, sizeof(double));
+    P->mat = (int *)calloc(n, sizeof(int));
+    if (!P->pre || !P->rad || !P->den || !P->tem || !P->eng || !P->gra || !P->mat) return SALEC_ENOMEM;
     /* SetTargetProfile_Mat */
     int cur = n - 1;
     for (int l = 0; l < h->n_layer; l++) {
@@ -1155,67 +1305,130 @@ static int build_profile(salec_host *h, tprof *P)
         for (int j = 0; j < ti && cur > 0; ++j) P->mat[cur--] = h->t_mat[l];
     }
     for (; cur >= 0; --cur) P->mat[cur] = h->t_mat[h->n_layer - 1];
-    /* SetTargetProfile_Tem */
+    /* SetTargetProfile_Tem (max_depth is r_planet for a sphere) */
     for (int k = 0; k < n; ++k) {
         P->pre[k] = h->surf_pre;
         P->den[k] = h->mat[P->mat[k]].den0;
         P->rad[k] = h->max_depth * k / (n - 1);
     }
-    if (h->tem_type == 0) {
+    if (h->tem_type == TEM_CONST) {
         for (int k = 0; k < n; ++k) P->tem[k] = h->surf_tem;
-    } else {
+    } else if (h->tem_type == TEM_CONDUCTIVE) {
         for (int k = n - 1; k >= 0; --k) {
             double kd = P->rad[n - 1] - P->rad[k];
             if (n == k + 1) { P->tem[k] = h->surf_tem; continue; }
             double g = h->surf_dtdz * (1.0 + kd / h->r_planet);
             P->tem[k] = P->tem[k + 1] + g * (P->rad[k + 1] - P->rad[k]);
         }
+    } else {
+        for (int k = n - 1; k >= 0; --k) {
+            double kd = P->rad[n - 1] - P->rad[k];
+            if (n == k + 1) { P->tem[k] = h->surf_tem; continue; }
+            int m = P->mat[k];
+            double gc = h->surf_dtdz * (1.0 + kd / h->r_planet);
+            double cc = h->thermexp[m] * h->surf_gravity / h->c_heat[m];
+            double gv = h->t_lith * cc * exp(cc * (kd - h->d_lith));
+            double cond = 1.0 - (P->tem[k + 1] - 0.9 * h->t_lith) / (0.11 * h->t_lith);
+            cond = Wind(cond, 0.0, 1.0);
+            P->tem[k] = P->tem[k + 1] + (gc * cond + gv * (1.0 - cond)) * (P->rad[k + 1] - P->rad[k]);
+        }
     }
-    double gz = h->gravity[1];
-    /* IntegrateGravity + CorrectionDensity1, iterated to the fixed point */
-    double *prev = (double *)malloc(sizeof(double) * n);
-    int fail_any = 0;
+    /* SetTargetProfile_Gra */
+    for (int k = 0; k < n; ++k) P->gra[k] = h->gravity[1];
+    /* 2000 rounds of IntegrateGravity + CorrectDensity.  A round maps
+     * (den, tem) to the next pair; once a round leaves both unchanged every
+     * later round repeats it, so the loop stops there with the same result. */
+    double *prev = (double *)malloc(sizeof(double) * 2 * n);
+    if (!prev) return SALEC_ENOMEM;
+    int rc = SALEC_OK;
     for (int it = 0; it < 2000; ++it) {
-        for (int k = n - 2; k >= 0; --k) {
-            double drg = 0.25 * (P->den[k + 1] + P->den[k]) * (gz + gz);
-            P->pre[k] = P->pre[k + 1] + drg * (P->rad[k] - P->rad[k + 1]);
-        }
+        integrate_gravity(h, P);
         memcpy(prev, P->den, sizeof(double) * n);
-        int fail = 0;
-#pragma omp parallel for schedule(dynamic, 16) reduction(| : fail)
-        for (int k = 0; k < n; ++k) {
-            const salec_material *r = &h->mat[P->mat[k]];
-            double mt = r->Tmelt0 * pow(r->Asimon * P->pre[k] + 1.0, r->Csimon);
-            if (r->PolySol > 0) { double p = P->pre[k] * 1.0e-9; mt = r->Tsolc1 + r->Tsolc2 * p + r->Tsolc3 * p * p; }
-            double lt = Min(P->tem[k], mt);
-            P->tem[k] = lt;
-            /* CorrectionDensity1/2 (src/target_profile.c:215-290): the matrix
-             * state at alpha0 * p, divided by the distension alpha0 */
-            double pty = r->alpha0;
-            if (eos_tp(h, P->mat[k], lt, P->pre[k] * pty, &P->den[k], &P->eng[k]) != SALEC_OK) fail = 1;
-            P->den[k] = P->den[k] / pty;
-            P->eng[k] = P->eng[k] / pty;
-        }
-        if (fail) { fail_any = 1; break; }
-        if (memcmp(prev, P->den, sizeof(double) * n) == 0) break;
+        memcpy(prev + n, P->tem, sizeof(double) * n);
+        if ((rc = correct_density(h, P)) != SALEC_OK) break;
+        if (memcmp(prev, P->den, sizeof(double) * n) == 0 && memcmp(prev + n, P->tem, sizeof(double) * n) == 0) break;
     }
     free(prev);
-    if (fail_any) return SALEC_EINVAL;   /* a profile state lies outside an EOS table */
-    for (int k = n - 2; k >= 0; --k) {
-        double drg = 0.25 * (P->den[k + 1] + P->den[k]) * (gz + gz);
-        P->pre[k] = P->pre[k + 1] + drg * (P->rad[k] - P->rad[k + 1]);
+    if (rc != SALEC_OK) return rc;
+    integrate_gravity(h, P);
+    return SALEC_OK;
+}
+
+/* the target-cell state at profile index li */
+static void set_target_cell(salec_host *h, const tprof *P, long c, long NC, int li)
+{
+    int lm = P->mat[li];
+    h->f[SALEC_E_PRE][c] = P->pre[li]; h->f[SALEC_E_TEM][c] = P->tem[li]; h->f[SALEC_E_DAM][c] = 0.;
+    h->f[SALEC_M_VOF][lm * NC + c] = 1.0;
+    h->f[SALEC_M_DEN][lm * NC + c] = P->den[li];
+    h->f[SALEC_M_ENG][lm * NC + c] = P->eng[li];
+    h->f[SALEC_M_PTY][lm * NC + c] = h->mat[lm].alpha0;
+    for (int q = 0; q < 4; ++q) h->f[SALEC_E_WPT][q * NC + c] = -0.25;
+}
+
+/* init_material_target_addition, src/target_profile.c:1000-1157: objects
+ * overwrite the cells they hold, in input order. */
+static int apply_additions(salec_host *h)
+{
+    int nmat = h->cfg.nmat;
+    int CW = h->nx + 4, CH = h->ny + 4;
+    long NC = (long)CW * CH;
+    for (long c = 0; c < NC; ++c) {
+        double ep[2] = {h->f[SALEC_E_POS][c], h->f[SALEC_E_POS][NC + c]};
+        for (int o = 0; o < h->n_add; ++o) {
+            const double *q = h->add_p[o];
+            int oid = h->add_mat[o];
+            double tem, pre, dam, pty, vel[2];
+            if (h->add_type[o] == ADD_CUBE) {
+                double x0[2] = {ep[0] - q[0], ep[1] - q[1]};
+                double lxa = sqrt(0. + q[2] * q[2] + q[3] * q[3]);
+                double lya = sqrt(0. + q[4] * q[4] + q[5] * q[5]);
+                double xl = (0. + x0[0] * q[2] + x0[1] * q[3]) / lxa / lxa;
+                double yl = (0. + x0[0] * q[4] + x0[1] * q[5]) / lya / lya;
+                if (xl * (1 - xl) < 0 || yl * (1 - yl) < 0) continue;
+                vel[0] = q[6]; vel[1] = q[7]; tem = q[8]; pre = q[9]; dam = q[10]; pty = q[11];
+            } else {
+                double d0 = q[0] - ep[0], d1 = q[1] - ep[1];
+                if (sqrt(0. + d0 * d0 + d1 * d1) > q[2]) continue;
+                vel[0] = q[3]; vel[1] = q[4]; tem = q[5]; pre = q[6]; dam = q[7]; pty = q[8];
+            }
+            tem = tem >= 0 ? tem : h->f[SALEC_E_TEM][c];
+            pre = pre >= 0 ? pre : h->f[SALEC_E_PRE][c];
+            pty = pty >= 0 ? pty : h->f[SALEC_M_PTY][oid * NC + c];
+            dam = dam >= 0 ? dam : h->f[SALEC_E_DAM][c];
+            for (int m = 0; m < nmat; ++m) {
+                h->f[SALEC_M_VOF][m * NC + c] = 0.;
+                h->f[SALEC_M_DEN][m * NC + c] = 0.;
+                h->f[SALEC_M_ENG][m * NC + c] = 0.;
+            }
+            if (oid == 0) {
+                h->f[SALEC_M_VOF][c] = 1.0;
+                for (int k = 0; k < 4; ++k) h->f[SALEC_E_WPT][k * NC + c] = 1.0;
+                continue;
+            }
+            double den, eng;
+            if (eos_tp(h, oid, tem, pre, &den, &eng) != SALEC_OK) return SALEC_EINVAL;
+            h->f[SALEC_E_PRE][c] = pre; h->f[SALEC_E_TEM][c] = tem; h->f[SALEC_E_DAM][c] = dam;
+            h->f[SALEC_M_VOF][oid * NC + c] = 1.0;
+            h->f[SALEC_M_DEN][oid * NC + c] = den / pty;
+            h->f[SALEC_M_ENG][oid * NC + c] = eng / pty;
+            h->f[SALEC_M_PTY][oid * NC + c] = pty;
+            for (int k = 0; k < 4; ++k) h->f[SALEC_E_WPT][k * NC + c] = -0.25;
+            h->f[SALEC_E_VEL][c] = vel[0]; h->f[SALEC_E_VEL][NC + c] = vel[1];
+        }
     }
     return SALEC_OK;
 }
 
-int salec_host_setup(salec_host *h)
+/* init_material_plane_distribution (src/target_profile.c:588-831) and
+ * init_material_sphere_distribution (:835-999): cells in the projectile,
+ * in the target (below z = 0, or within r_planet of the planet's centre) or
+ * vacuum; vertex body force (uniform gravity, or central gravity from the
+ * profile falling off as 1/r^2 above the surface) and the ANC taper of the
+ * Laplacian weights (switched off for a sphere). */
+static int distribute(salec_host *h, const tprof *Pp)
 {
-    int rc = alloc_fields(h);
-    if (rc) return rc;
-    init_mesh(h);
-    tprof P;
-    memset(&P, 0, sizeof P);
-    if ((rc = build_profile(h, &P)) != SALEC_OK) return rc;
+    const tprof P = *Pp;
     int nmat = h->cfg.nmat;
     int CW = h->nx + 4, CH = h->ny + 4, VW = h->nx + 5, VH = h->ny + 5;
     long NC = (long)CW * CH, NV = (long)VW * VH;
@@ -1223,7 +1436,6 @@ int salec_host_setup(salec_host *h)
     const double p_pty = h->mat[h->p_mat].alpha0;
     double p_den = 0., p_eng = 0.;
     if (eos_tp(h, h->p_mat, h->p_tem, h->p_pre * p_pty, &p_den, &p_eng) != SALEC_OK) return SALEC_EINVAL;
-    /* init_material_plane_distribution, src/target_profile.c:588-713 */
     for (int j = 0; j < CH; ++j)
         for (int k = 0; k < CW; ++k) {
             long c = k + (long)CW * j;
@@ -1240,6 +1452,8 @@ int salec_host_setup(salec_host *h)
             double dxp = ex - h->p_center[0], dyp = ey - h->p_center[1];
             double pd = sqrt(0. + dxp * dxp + dyp * dyp);
             double *wpt = h->f[SALEC_E_WPT];
+            double dxc = ex - h->t_center[0], dyc = ey - h->t_center[1];
+            double cd = h->t_plane ? 0. : sqrt(0. + dxc * dxc + dyc * dyc);
             if (pd <= h->p_radius) {
                 int pm = h->p_mat;
                 h->f[SALEC_M_VOF][pm * NC + c] = 1.0;
@@ -1250,49 +1464,76 @@ int salec_host_setup(salec_host *h)
                 h->f[SALEC_E_VEL][c] = h->p_vel[0]; h->f[SALEC_E_VEL][NC + c] = h->p_vel[1];
                 for (int q = 0; q < 4; ++q) wpt[q * NC + c] = -0.25;
                 h->f[SALEC_M_PTY][pm * NC + c] = pty;
-            } else if (ey <= 0.) {
+            } else if (h->t_plane && ey <= 0.) {
                 int li = (int)floor(fabs(ey) / h->max_depth * (P.n - 1)) + 1;
                 li = P.n - 1 - li;
                 if (li > P.n - 1) li = P.n - 1;
                 if (li < 0) li = 0;
-                int lm = P.mat[li];
-                h->f[SALEC_E_PRE][c] = P.pre[li]; h->f[SALEC_E_TEM][c] = P.tem[li]; h->f[SALEC_E_DAM][c] = 0.;
-                h->f[SALEC_M_VOF][lm * NC + c] = 1.0;
-                h->f[SALEC_M_DEN][lm * NC + c] = P.den[li];
-                h->f[SALEC_M_ENG][lm * NC + c] = P.eng[li];
-                h->f[SALEC_M_PTY][lm * NC + c] = h->mat[lm].alpha0;
-                for (int q = 0; q < 4; ++q) wpt[q * NC + c] = -0.25;
+                set_target_cell(h, &P, c, NC, li);
+            } else if (!h->t_plane && cd <= h->r_planet) {
+                int li = (int)floor(cd / P.dz);
+                if (li > P.n - 1) li = P.n - 1;
+                if (li < 0) li = 0;
+                set_target_cell(h, &P, c, NC, li);
             } else {
                 h->f[SALEC_M_VOF][c] = 1.0;
                 for (int q = 0; q < 4; ++q) wpt[q * NC + c] = 1.0;
             }
         }
-    /* gravity and ANC taper, src/target_profile.c:715-831 */
     for (int j = 0; j < VH; ++j)
         for (int k = 0; k < VW; ++k) {
             long v = k + (long)VW * j;
-            h->f[SALEC_V_BF][v] = h->gravity[0];
-            h->f[SALEC_V_BF][NV + v] = h->gravity[1];
-            if (j <= NPAD) h->f[SALEC_V_BF][NV + v] = 0.;
             double ar = 0.;
-            if (!strcasecmp(h->anc_pattern, "pml")) {
-                if (h->damp_y >= 1 && j <= 2 * h->damp_y) {
-                    double lx = 0.5 * j / h->damp_y;
-                    ar = 2.0 * Max(ar, 2.0 * hanning(0.5 * (1.0 - lx)));
+            if (!h->t_plane) {
+                double vx = h->f[SALEC_V_POS][v], vy = h->f[SALEC_V_POS][NV + v];
+                double g0 = vx - h->t_center[0], g1 = vy - h->t_center[1];
+                double d = sqrt(0. + g0 * g0 + g1 * g1);
+                if (d == 0.) return SALEC_EINVAL;   /* a vertex on the centre has no direction (0/0 in the reference) */
+                g0 /= d; g1 /= d;
+                int li = (int)floor(d / P.dz);
+                if (li > P.n - 1) li = P.n - 1;
+                if (li < 0) li = 0;
+                double bx = P.gra[li] * g0, by = P.gra[li] * g1;
+                if (d >= h->r_planet) {
+                    double s = (h->r_planet / d) * (h->r_planet / d);
+                    bx *= s; by *= s;
                 }
-                if (h->damp_x >= 1 && k >= VW - 2 * h->damp_x) {
-                    double lx = 0.5 * (VW - 2 * h->damp_x - k) / h->damp_x;
-                    ar = 2.0 * Max(ar, 2.0 * hanning(0.5 * lx));
-                }
-            } else if (!strcasecmp(h->anc_pattern, "uniform")) {
-                ar = 1.0;
+                h->f[SALEC_V_BF][v] = bx;
+                h->f[SALEC_V_BF][NV + v] = by;
             } else {
-                ar = 0.;
+                h->f[SALEC_V_BF][v] = h->gravity[0];
+                h->f[SALEC_V_BF][NV + v] = h->gravity[1];
+                if (j <= NPAD) h->f[SALEC_V_BF][NV + v] = 0.;
+                if (!strcasecmp(h->anc_pattern, "pml")) {
+                    if (h->damp_y >= 1 && j <= 2 * h->damp_y) {
+                        double lx = 0.5 * j / h->damp_y;
+                        ar = 2.0 * Max(ar, 2.0 * hanning(0.5 * (1.0 - lx)));
+                    }
+                    if (h->damp_x >= 1 && k >= VW - 2 * h->damp_x) {
+                        double lx = 0.5 * (VW - 2 * h->damp_x - k) / h->damp_x;
+                        ar = 2.0 * Max(ar, 2.0 * hanning(0.5 * lx));
+                    }
+                } else if (!strcasecmp(h->anc_pattern, "uniform")) {
+                    ar = 1.0;
+                }
             }
             for (int q = 0; q < 9; ++q) h->f[SALEC_V_LAPLACE][q * NV + v] *= ar;
         }
-    free(P.pre); free(P.rad); free(P.den); free(P.tem); free(P.eng); free(P.mat);
     return SALEC_OK;
+}
+
+int salec_host_setup(salec_host *h)
+{
+    int rc = alloc_fields(h);
+    if (rc) return rc;
+    init_mesh(h);
+    tprof P;
+    memset(&P, 0, sizeof P);
+    if ((rc = build_profile(h, &P)) != SALEC_OK) { free_profile(&P); return rc; }
+    rc = distribute(h, &P);
+    free_profile(&P);
+    if (rc != SALEC_OK) return rc;
+    return apply_additions(h);
 }
 
 /* fl_tracer_init, src/tracer.c:380-450 */

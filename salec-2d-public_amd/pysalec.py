@@ -532,11 +532,22 @@ class Batch:
 GRANITE = [2680.0, 8.96e2, 1.8e10, 1.8e10, 1.6e7, 0.5, 1.3, 5.0, 5.0, 3.5e6, 1.8e7, 293.0]
 
 
+TILL_KEYS = ("rho0", "cv", "A", "B", "E0", "a", "b", "alpha", "beta", "Eiv", "Ecv", "Tref")
+
+
 def tillotson(rho0=None, **kw):
+    """Tillotson constants in the order of salec_write_tillotson: the granite
+    set with rho0 and any of TILL_KEYS replaced."""
     p = list(GRANITE)
     if rho0 is not None:
         p[0] = rho0
+    for k, v in kw.items():
+        p[TILL_KEYS.index(k)] = v
     return p
+
+
+# dunite-like mantle: rho0 3320 kg/m3, A = B = 130 GPa
+DUNITE = dict(rho0=3320.0, A=1.3e11, B=1.3e11, E0=5.5e8, a=0.5, b=1.5, Eiv=4.5e6, Ecv=1.45e7)
 
 
 def write_eos(eos_dir, name, params, table=None):
@@ -572,6 +583,13 @@ def write_config(path_dir, which, nx=None, ny=None, table_n=(256, 256), tracer_s
                   basalt (rho0 2860), both 256 x 256 log-spaced (rho, e) tables
                   filled from Tillotson, ROCK + Ivanov damage, 1000 x 1500
                   cells, projectile radius 20 cells at 12 km/s, ~1M tracers.
+    which = "c4": complex crater on a spherical target (central gravity,
+                  r_planet 1738 km, conductive temperature 10 K/km): regolith
+                  (40 cells) and basalt crust (800 cells) tables over a
+                  dunite-like Tillotson mantle (rho0 3320, A = B = 130 GPa),
+                  ROCK with thermal softening, block-model acoustic
+                  fluidisation (Toff 16 s), 3000 x 4000 cells of 50 m,
+                  dunite projectile radius 60 cells at 15 km/s, ~4M tracers.
     Cells are 10 m.  nx, ny shrink the mesh for tests; the projectile radius
     and regolith thickness (in cells) shrink with nx unless given.  Keyword
     arguments add or replace [numerical] keys; sections_over maps
@@ -620,6 +638,35 @@ def write_config(path_dir, which, nx=None, ny=None, table_n=(256, 256), tracer_s
                   ("max_depth", ny * dx * 1.5), ("temperature", "[const, 293.0]")]
         strip = 1 if tracer_strip is None else tracer_strip
         cycle = [("dt0", 1.0e-5), ("maxdt", 1.0e-2), ("time0", 0.0), ("time1", 1000.0), ("maxstep", 100000)]
+    elif which == "c4":
+        nx = nx or 3000
+        ny = ny or 4000
+        dx = 50.0
+        rc = proj_cells if proj_cells is not None else max(3.0, 60.0 * nx / 3000.0)
+        reg = reg_cells if reg_cells is not None else max(2.0, 40.0 * nx / 3000.0)
+        crust = max(4.0, 800.0 * nx / 3000.0)
+        vel = velocity if velocity is not None else 15000.0
+        tbl = (table_n[0], table_n[1], 1.0e-2, 1.2e4, 1.0e-1, 1.0e9)
+        write_eos(eos_dir, "regolith", tillotson(1800.0), table=tbl)
+        write_eos(eos_dir, "basalt", tillotson(2860.0), table=tbl)
+        write_eos(eos_dir, "dunite", tillotson(**DUNITE))
+        mats = [("nm", 4), ("name", "[vacuum_, regolith, basalt, dunite]"),
+                ("postfix", "[null, aneos.sd, aneos.sd, tillotson]"),
+                ("yshear", "[, RockStrengthT, RockStrengthT, RockStrengthT]"), ("damage", "[, Ivanov, Ivanov, Ivanov]"),
+                ("ytens", "[, SimpleTensile, SimpleTensile, SimpleTensile]"), ("Porosity", "[, None, None, None]"),
+                ("yint0", "[, 1.0e7, 1.0e7, 1.0e7]"), ("yintfri", "[, 1.2, 1.2, 1.2]"),
+                ("yintlim", "[, 2.5e9, 2.5e9, 3.5e9]"), ("ydam0", "[, 1.0e4, 1.0e4, 1.0e4]"),
+                ("ydamfri", "[, 0.6, 0.6, 0.6]"), ("ydamlim", "[, 2.5e9, 2.5e9, 3.5e9]"),
+                ("OhnakaXi", "[, 1.2, 1.2, 1.2]"), ("SimonT0", "[, 1373.0, 1373.0, 1436.0]"),
+                ("SimonA", "[, 4.5e9, 4.5e9, 1.52e9]"), ("SimonC", "[, 3.0, 3.0, 4.05]"),
+                ("Toff", "[, 16.0, 16.0, 16.0]"), ("Cvib", "[, 0.1, 0.1, 0.1]")]
+        r = rc * dx
+        proj = [("material", "dunite"), ("radiu", r), ("center", "[0.0, %g]" % r), ("velocity", "[0.0, %g]" % -vel)]
+        target = [("type", "sphere"), ("r_planet", 1738.0e3), ("number", 3), ("material", "[regolith, basalt, dunite]"),
+                  ("depth", "[%g, %g, %g]" % (reg * dx, crust * dx, 1738.0e3)),
+                  ("temperature", "[conductive, 293.0]"), ("surf_dtdz", 0.01)]
+        strip = 2 if tracer_strip is None else tracer_strip
+        cycle = [("dt0", 1.0e-5), ("maxdt", 1.0e-2), ("time0", 0.0), ("time1", 1000.0), ("maxstep", 20000)]
     else:
         raise ValueError(which)
     dc = damp_cells if damp_cells is not None else (30 if nx >= 200 else max(4, nx // 6))

@@ -10,7 +10,8 @@ for bit; the tests assert the bar, and report the worst difference seen.
 Cases: the two benchmark configurations on small meshes (c0: one Tillotson
 material, plain hydro; c1: two tabulated materials with ROCK strength,
 Ivanov damage, pml2 and tracers), the committed golden run, and variants
-that switch on the other models and boundary/damping options.  At the full
+that switch on the other models, boundary/damping options, a spherical
+target with central gravity (the complex-crater deck) and target additions.  At the full
 c1 size the tests check size-independent properties (determinism, finite
 state, volume fractions summing to one).
 """
@@ -187,6 +188,16 @@ VARIANTS = {
     "tensile_failure_partpres": ("c1", dict(tensile_failure=1, partpres=1)),
     "ejecta_resistance": ("c1", dict(sections_over={"ejecta.resistance": "0.5", "ejecta.friction": "0.2",
                                                    "ejecta.threshold": "1.0"})),
+    # the complex-crater deck scaled down: four materials (two tables, one
+    # Tillotson mantle), a spherical target whose body force points at the
+    # planet's centre, ANC off, a conductive profile, thermal softening and
+    # block-model acoustic fluidisation
+    "sphere_target_complex_crater": ("c4", dict()),
+    # a granite ball moving through vacuum and a vacuum box cut into the target
+    "target_additions": ("c1", dict(sections_over={
+        "addition.nobject": 2, "addition.material": "[basalt, vacuum_]", "addition.type": "[sphere, cube]",
+        "addition.parameter": "[ball, box]", "addition.ball": "[60.0, 120.0, 25.0, 300.0, -900.0, 293.0, 1.0e5, 0.2, -1]",
+        "addition.box": "[80.0, -60.0, 60.0, 0.0, 0.0, 30.0, 0.0, 0.0, -1, -1, -1, -1]"})),
     "anc_uniform_stretched_mesh": ("c0", dict(ANCPAT="uniform", ANC=0.3,
                                               sections_over={"mesh.ext": "on", "mesh.ex": "[1.04, 6, 6]",
                                                              "mesh.ey": "[1.04, 6, 6]"})),
@@ -198,7 +209,7 @@ def test_variant_parity(name):
     which, kw = VARIANTS[name][:2]
     ahead, steps = VARIANTS[name][2:] if len(VARIANTS[name]) > 2 else (15, 25)
     say("variant %s (%s 32x48, %d steps after %d)" % (name, which, steps, ahead))
-    h, o, dev, tr = setup_pair(which, 32, 48, tracers=(which == "c1"), **kw)
+    h, o, dev, tr = setup_pair(which, 32, 48, tracers=(which != "c0"), **kw)
     if tr is not None:
         dev.upload_tracers(tr)
     # run the oracle ahead so that the compared steps have developed

@@ -96,7 +96,8 @@ def test_v_laplace_field_on_stretched_mesh(tmp_path):
 
 @pytest.mark.parametrize("over,what", [
     ({"material.Porosity": "[, Ivanov]"}, "orosity"),
-    ({"target.type": "sphere"}, "sphere"),
+    ({"target.type": "cylinder"}, "cylinder"),
+    ({"target.temperature": "[condconv, 293.0]"}, "condconv"),
 ])
 def test_loader_rejects_unsupported(tmp_path, over, what):
     inp, eos = ps.write_config(str(tmp_path / "c0"), "c0", nx=16, ny=16, sections_over=over)
