@@ -14,7 +14,10 @@ its steps in.  The timed steps follow.
 Prints one JSON line: cell-steps/s (physical cells x steps / time of the timed
 steps, inputs resident in HBM), the roofline of the dominant kernel measured
 with HIP events on the solver's stream, and the CPU baseline (the single-thread
-C oracle on a bounded sample of the same workload).
+C oracle on a bounded sample of the same workload).  Side legs in the same
+line: the batched 64-run ensemble against the same runs stepped serially, the
+sensitivity (dual-number) step against the plain step, and the 3000 x 4000
+complex-crater deck (spherical target, four materials) stepped plainly.
 """
 import argparse
 import json
@@ -179,6 +182,50 @@ def ensemble_leg(nruns, steps, seed, wd):
     return out
 
 
+def complex_leg(advance, steps, wd, nx=3000, ny=4000):
+    """The complex-crater deck at its full size: 3000 x 4000 cells of 50 m,
+    vacuum + regolith + basalt crust (tables) + dunite-like Tillotson mantle,
+    spherical target with central gravity, ROCK with thermal softening,
+    block-model acoustic fluidisation, pml2, tracers, dunite projectile of
+    60 cells at 15 km/s.  Plain steps timed after `advance` steps from
+    contact, wall clock around the step loop."""
+    t0 = time.time()
+    inp, eos = ps.write_config(os.path.join(wd, "c4"), "c4", nx=nx, ny=ny)
+    h = ps.HostRun(inp, eos)
+    log("complex crater: host setup %dx%d in %.1fs" % (nx, ny, time.time() - t0))
+    ntr = ps.lib().salec_host_count_tracers(h.h)
+    d = ps.Device(h.cfg, max_tracers=ntr)
+    d.upload_state(h.state())
+    h.cfg.damp_time = d.init(h.damp_t_in)
+    tr = h.tracers(d.download("V_VEL"))
+    d.upload_tracers(tr)
+    d.set_time(h.time0, h.dt0)
+    h.close()
+    for k in range(advance):
+        d.step(sync=False)
+        if k % 500 == 499:
+            d.sync()
+            log("  complex crater: advanced %d steps" % (k + 1))
+    d.sync()
+    t_adv = d.get_time()[0]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.step(sync=False)
+    d.sync()
+    el = time.perf_counter() - t0
+    rc, badf, badi = d.check_finite()
+    d.close()
+    cells = nx * ny
+    out = {"workload": "complex crater: %dx%d cells of 50 m, vacuum+regolith+basalt (tables)+dunite (Tillotson), "
+                       "spherical target r=1738 km with central gravity, ROCK + thermal softening, block-model "
+                       "acoustic fluidisation, pml2 30 layers, %d tracers, r=60 cells at 15 km/s" % (nx, ny, tr.n),
+           "cells": cells, "tracers": int(tr.n), "nmat": 4, "advance": advance, "start_time": t_adv,
+           "steps": steps, "ms_per_step": round(el / steps * 1e3, 4),
+           "cell_steps_per_s": round(cells * steps / el, 1), "finite": rc == 0}
+    log("complex crater: %.3f ms/step, %.3e cell-steps/s" % (out["ms_per_step"], out["cell_steps_per_s"]))
+    return out
+
+
 def sensitivity_leg(nx, ny, steps, wd):
     """The differentiated step: the strength workload's physics on nx x ny
     cells over a window of `steps` steps from contact.  Forward: the plain
@@ -243,6 +290,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--sens-steps", type=int, default=400,
                     help="window of the sensitivity leg (512x768 cells; 0 = skip)")
+    ap.add_argument("--complex-steps", type=int, default=200,
+                    help="timed steps of the 3000x4000 complex-crater leg (0 = skip)")
+    ap.add_argument("--complex-advance", type=int, default=500)
     ap.add_argument("--checkpoint-out", default=None, help="save the advanced state here (salec_checkpoint)")
     ap.add_argument("--checkpoint-in", default=None,
                     help="start from this checkpoint instead of contact + advance (for profiler runs)")
@@ -374,6 +424,7 @@ def main():
     dev.close()
     ens = ensemble_leg(args.ensemble, args.ensemble_steps, 0, wd) if args.ensemble > 0 else None
     sens = sensitivity_leg(512, 768, args.sens_steps, wd) if args.sens_steps > 0 else None
+    cplx = complex_leg(args.complex_advance, args.complex_steps, wd) if args.complex_steps > 0 else None
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -386,7 +437,8 @@ def main():
                       "start_time": t_adv,
                       "cells": cells, "tracers": int(tr.n), "nmat": nmat,
                       "sim_time_after": t_now},
-           "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens, "sensitivity": sens}
+           "roofline": roofline, "cpu_baseline": cpu, "ensemble": ens, "sensitivity": sens,
+           "complex_crater": cplx}
     print(json.dumps(out), flush=True)
 
 
