@@ -148,6 +148,10 @@ def lib():
     L.salec_write_pvtp.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int]
     L.salec_write_mass_tracers.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(Tracers),
                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.salec_isale_to_inp.argtypes = [C.c_char_p, C.c_char_p, vp]
+    L.salec_pack_vts.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p]
+    L.salec_unpack_vts.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int)]
+    L.salec_merge_vts_blocks.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_char_p]
     L.salec_checkpoint.argtypes = [vp, C.c_char_p]
     L.salec_restore.argtypes = [vp, C.c_char_p]
     L.salec_write_tillotson.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int]
@@ -447,6 +451,47 @@ def write_mass_tracers(path, tr, nx, ny, e_den, e_vol):
     a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (e_den, e_vol)]
     check(lib().salec_write_mass_tracers(path.encode(), nx, ny, C.byref(tr.struct()), _dp(a[0]), _dp(a[1])),
           "salec_write_mass_tracers")
+
+
+def isale_to_inp(asteroid, material):
+    """The .inp lines an iSALE-2D input pair translates to
+    (salec_isale_to_inp), as text."""
+    libc = C.CDLL(None)
+    libc.tmpfile.restype = C.c_void_p
+    for fn in ("rewind", "fclose"):
+        getattr(libc, fn).argtypes = [C.c_void_p]
+    libc.fread.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, C.c_void_p]
+    libc.fread.restype = C.c_size_t
+    fp = libc.tmpfile()
+    try:
+        check(lib().salec_isale_to_inp(asteroid.encode(), material.encode(), fp), "salec_isale_to_inp")
+        libc.rewind(fp)
+        out, buf = b"", C.create_string_buffer(1 << 16)
+        while True:
+            n = libc.fread(buf, 1, len(buf), fp)
+            if n == 0:
+                break
+            out += buf.raw[:n]
+        return out.decode()
+    finally:
+        libc.fclose(fp)
+
+
+def pack_vts(vts_dir, prefix, nproc, min_cycle, max_cycle, nthreads, out_dir):
+    check(lib().salec_pack_vts(vts_dir.encode(), prefix.encode(), nproc, min_cycle, max_cycle, nthreads,
+                               out_dir.encode()), "salec_pack_vts")
+
+
+def unpack_vts(pack, dirname=None):
+    n = C.c_int(0)
+    check(lib().salec_unpack_vts(pack.encode(), dirname.encode() if dirname else None, C.byref(n)),
+          "salec_unpack_vts")
+    return n.value
+
+
+def merge_vts_blocks(vts_dir, prefix, nproc, min_cycle, max_cycle, out_dir):
+    check(lib().salec_merge_vts_blocks(vts_dir.encode(), prefix.encode(), nproc, min_cycle, max_cycle,
+                                       out_dir.encode()), "salec_merge_vts_blocks")
 
 
 class CraterSensitivity:

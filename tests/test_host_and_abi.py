@@ -14,6 +14,7 @@
   closed box without damping.
 """
 import ctypes as C
+import glob
 import os
 import re
 import tempfile
@@ -27,8 +28,8 @@ import pysalec as ps
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
-HEADERS = [os.path.join(ROOT, "include", "salec.h"),
-           os.path.join(ROOT, "salec-2d-public_amd", "host", "setup.h")]
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))) + [
+    os.path.join(ROOT, "salec-2d-public_amd", "host", "setup.h")]
 
 
 def declared_functions(path):
