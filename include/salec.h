@@ -38,7 +38,7 @@ extern "C" {
 #endif
 
 #define SALEC_MAXMAT 10   /* MAXMAT, src/sale2d_constant.h:305 */
-#define SALEC_ABI_VERSION 2
+#define SALEC_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 #define SALEC_OK            0
@@ -126,7 +126,8 @@ enum { SALEC_GRID_CELL = 0, SALEC_GRID_VERTEX = 1, SALEC_GRID_XFACE = 2,
     X(BX_NORMAL,  SALEC_GRID_XFACE,   2)                                        \
     X(BY_NORMAL,  SALEC_GRID_YFACE,   2)                                        \
     X(Y_PROFILE,  SALEC_GRID_COLUMN,  1)                                        \
-    X(E_BETA,     SALEC_GRID_CELL,    4)  /* pml3 flux auxiliaries (e_beta)  */
+    X(E_BETA,     SALEC_GRID_CELL,    4)  /* pml3 flux auxiliaries (e_beta)  */ \
+    X(E_CND,      SALEC_GRID_CELL,    1)  /* lubricating tracers in the cell */
 
 #define SALEC_FIELD_ENUM_(name, grid, nc) SALEC_##name,
 enum { SALEC_FIELD_LIST(SALEC_FIELD_ENUM_) SALEC_NFIELDS };
@@ -176,6 +177,14 @@ typedef struct salec_config {
     double max_dt;                 /* cycle.maxdt                            */
     int    go_list[4];             /* left, right, bottom, top: SALEC_GHT_*  */
     long   max_tracers;            /* capacity of the tracer arrays          */
+    /* tracer lubrication (tracer.lubrication, tracer.lub_y0, tracer.lub_frac;
+     * src/sale2d.c:522-527, src/construct.c:76-92): with L >= 1, a cell that
+     * holds a live tracer created in vertex rows [lub_gy0, lub_gy1) (the
+     * rows from L cells below the surface up to it) has its shear strength
+     * capped at min(p lub_frac + lub_y0, 0) (src/strength.c:265-275). */
+    int    tracer_lubrication;
+    int    lub_gy0, lub_gy1;
+    double lub_y0, lub_frac;
 } salec_config;
 
 /* ---- tracers (tracer_t, src/mpi_vec/linked_list.h:18-28) --------------- */

@@ -878,6 +878,11 @@ static void update_e_stress(oracle_state *s, const salec_config *cfg, const grid
                 double mtn = oracle_strength_fn(r->tfunc, pa, r);
                 lts += CF(M_VOF, m, c) * mtn;
             }
+            /* lubricating tracers cap the strength, src/strength.c:265-275 */
+            if (cfg->tracer_lubrication >= 1 && CF(E_CND, 0, c) >= 1.0) {
+                double lub = Min(pre * cfg->lub_frac + cfg->lub_y0, 0.0);
+                lss = Min(lss, lub);
+            }
             strength = lss;
             if (lss < ste2s) {
                 sta2s = sta2s / Max(0.5, (ste2s - s2o) / (ste2s - lss));
@@ -1732,6 +1737,19 @@ static void tracer_update(oracle_state *s, const salec_config *cfg, const grid_t
     }
 }
 
+/* update_strength_condition, src/strength.c:351-376: e_cnd counts, per
+ * cell, the live tracers created in vertex rows [lub_gy0, lub_gy1) */
+static void strength_condition(oracle_state *s, const salec_config *cfg, const grid_t g)
+{
+    if (cfg->tracer_lubrication <= 0) return;
+    for (long c = 0; c < g.NC; ++c) CF(E_CND, 0, c) = 0.;
+    const salec_tracers *t = &s->tr;
+    for (long i = 0; i < t->n; ++i) {
+        if (!t->alive[i]) continue;
+        if (t->gy[i] < cfg->lub_gy1 && t->gy[i] >= cfg->lub_gy0) CF(E_CND, 0, CI(t->idx[i], t->idy[i])) += 1.0;
+    }
+}
+
 /* --------------------------------------------------------------- driver -- */
 int oracle_attach(oracle_state *s, const salec_config *cfg)
 {
@@ -1856,6 +1874,7 @@ void oracle_step_to_output(oracle_state *s, const salec_config *cfg)
         oracle_ghost_fill(s, cfg, SALEC_E_CVS, O_GHT_ALWAYS_COPY);
     }
     calculate_flux_mixed(s, g, 0, porous(cfg));
+    strength_condition(s, cfg, g);
     (void)cell_receive;
 }
 void oracle_step_from_output(oracle_state *s, const salec_config *cfg)

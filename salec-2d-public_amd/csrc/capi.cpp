@@ -501,6 +501,8 @@ int salec_create(const salec_config *cfg, salec_ctx **out)
     P.threshold_z = cfg->threshold_z; P.flow_resistance = cfg->flow_resistance; P.flow_friction = cfg->flow_friction;
     P.max_dt = cfg->max_dt; P.partpres = cfg->partpres; P.tensile_failure = cfg->tensile_failure;
     P.damp_type = cfg->damp_type;
+    P.lub = cfg->tracer_lubrication; P.lub_gy0 = cfg->lub_gy0; P.lub_gy1 = cfg->lub_gy1;
+    P.lub_y0 = cfg->lub_y0; P.lub_frac = cfg->lub_frac;
     P.acoustic_fluid = 0;
     for (int m = 1; m < cfg->nmat; ++m) if (cfg->mat[m].Toff > 0.0) P.acoustic_fluid = 1;
     /* porosity_model, src/sale2d.c:490-513 */
@@ -869,7 +871,8 @@ int salec_step(salec_ctx *c, unsigned flags, salec_stats *st)
     PHASE(PH_RST_ACC, run_program(c, c->rstA));
     PHASE(PH_VDIS, launch_v_dis(s, g, F, c->d_time));
     PHASE(PH_RST_DIS, run_program(c, c->rstD));
-    PHASE(PH_TRACERS, launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
+    PHASE(PH_TRACERS, if (P.lub >= 1) launch_lub_clear(s, g, F);
+                      launch_tracers(s, g, F, P, c->T, c->d_vx, c->d_vy, c->d_time));
     PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, F, P, c->d_mats, c->d_time, c->bx, c->by, c->d_top));
     PHASE(PH_GF_C, run_program(c, c->gfC); run_program(c, c->gfP));
     PHASE(PH_FLUX_RING, launch_flux_ring(s, g, F, c->bx, c->by));
@@ -985,6 +988,10 @@ static void sens_step_once(salec_ctx *c)
     PHASE(PH_RST_ACC, run_program_sens(c, c->rstA));
     PHASE(PH_VDIS, launch_v_dis(s, g, c->F, T, c->d_time));
     PHASE(PH_RST_DIS, run_program_sens(c, c->rstD));
+    /* tracers change the state only through lubrication; then they run
+     * (their positions carry no derivatives, the cap's switch has none) */
+    if (P.lub >= 1) PHASE(PH_TRACERS, launch_lub_clear(s, g, c->F);
+                          launch_tracers(s, g, c->F, P, c->T, c->d_vx, c->d_vy, c->d_time));
     PHASE(PH_ENERGY_FLUX, launch_energy_flux(s, g, c->F, T, P, c->d_mats, c->d_time, S->bx, S->by, S->top));
     PHASE(PH_GF_C, run_program_sens(c, c->gfC); run_program_sens(c, c->gfP));
     PHASE(PH_FLUX_RING, launch_flux_ring(s, g, c->F, T, S->bx, S->by));

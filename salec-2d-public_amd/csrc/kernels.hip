@@ -689,6 +689,8 @@ k_stress(Grid g, Fields F, TF<S> T, Params P, const DevMat *mats, const DevTime 
             lss += vf * dMax(my, 0.0);
             if (P.tensile_failure) lts += vf * strength_fn<S>(r.tfunc, pre, dam, tps, sta2s, tem, r);
         }
+        /* lubricating tracers in the cell cap the strength (src/strength.c:265-275) */
+        if (P.lub >= 1 && CPL(E_CND, 0, c) >= 1.0) lss = dMin(lss, dMin(pre * P.lub_frac + P.lub_y0, 0.0));
         strength = lss;
         if (lss < ste2s) {
             sta2s = sta2s / dMax(0.5, (ste2s - s2o) / (ste2s - lss));
@@ -976,7 +978,21 @@ __global__ void k_tracers(Grid g, Fields F, Params P, Tracers Tr, const double *
     Tr.pos[i] = p0; Tr.pos[Tr.n + i] = p1;
     Tr.vel[i] = u0; Tr.vel[Tr.n + i] = u1;
     Tr.idx[i] = k; Tr.idy[i] = j;
-    if (k < 2 || k > g.CW - 3 || j < 2 || j > g.CH - 3) Tr.alive[i] = 0;
+    if (k < 2 || k > g.CW - 3 || j < 2 || j > g.CH - 3) { Tr.alive[i] = 0; return; }
+    /* update_strength_condition, src/strength.c:351-376: count the live
+     * tracers created in the lubricating rows, per cell they are now in
+     * (exact integer sums, so the order of the additions does not matter) */
+    if (P.lub >= 1 && Tr.gy[i] < P.lub_gy1 && Tr.gy[i] >= P.lub_gy0)
+        atomicAdd(&F.f[SALEC_E_CND][k + (long)g.CW * j], 1.0);
+}
+__global__ void k_lub_clear(Grid g, Fields F)
+{
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c < g.NC) F.f[SALEC_E_CND][c] = 0.0;
+}
+void launch_lub_clear(hipStream_t s, const Grid &g, const Fields &F)
+{
+    k_lub_clear<<<nblk(g.NC, 256), 256, 0, s>>>(g, F);
 }
 void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const Tracers &T,
                     const double *vx, const double *vy, const DevTime *tm)

@@ -72,6 +72,8 @@ struct Params {
     int partpres, tensile_failure, acoustic_fluid, damp_type;
     int porosity;                   /* some material has a porosity model   */
     int pty_one;                    /* no porosity model and every m_pty is 1 */
+    int lub, lub_gy0, lub_gy1;      /* tracer lubrication (salec_config)    */
+    double lub_y0, lub_frac;
 };
 
 struct Fields { double *f[SALEC_NFIELDS]; };
@@ -107,6 +109,7 @@ int  gradvel_dt_blocks(const Grid &g);
 void launch_stress(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats, const DevTime *tm);
 void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm);
 void launch_v_dis(hipStream_t s, const Grid &g, const Fields &F, const DevTime *tm);
+void launch_lub_clear(hipStream_t s, const Grid &g, const Fields &F);
 void launch_tracers(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const Tracers &T,
                     const double *vx, const double *vy, const DevTime *tm);
 void launch_update_state(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevMat *mats);

@@ -709,6 +709,10 @@ int salec_host_load(const char *inp_path, const char *eos_dir, salec_host **out,
     }
     if (h->ox < 1e-6) c->go_list[0] = SALEC_GHT_MIRROR;
     h->tracer_strip = inp_i(f, "numerical.tarcer_strip", 2);
+    /* init_sale2d_other, src/sale2d.c:522-527 */
+    c->tracer_lubrication = inp_i(f, "tracer.lubrication", 0);
+    c->lub_y0 = inp_d(f, "tracer.lub_y0", 0.);
+    c->lub_frac = inp_d(f, "tracer.lub_frac", 0.);
     /* projectile, src/sale2d_back.c:29-62 */
     inp_s(f, "projectile.material", buf, sizeof buf, "unknown");
     h->p_mat = -1;
@@ -1522,11 +1526,36 @@ static int distribute(salec_host *h, const tprof *Pp)
     return SALEC_OK;
 }
 
+/* the vertex rows of lubricating tracers, src/construct.c:76-92: lub_gy0 is
+ * the first padded vertex row at or above L cells below the surface, lub_gy1
+ * the first one from there at or above the surface (row j's z is that of
+ * every vertex of the row) */
+static void lubrication_rows(salec_host *h)
+{
+    salec_config *c = &h->cfg;
+    if (c->tracer_lubrication < 1) return;
+    int VW = h->nx + 5, NY = h->ny + 4;
+    long NV = (long)VW * (h->ny + 5);
+    const double *vy = h->f[SALEC_V_POS] + NV;
+    c->lub_gy0 = 0;
+    c->lub_gy1 = NY;
+    int j = 0;
+    for (; j < NY + 1; ++j) {
+        c->lub_gy0 = j;
+        if (vy[(long)VW * j] >= c->tracer_lubrication * h->dy * (-1.0)) break;
+    }
+    for (; j < NY + 1; ++j) {
+        c->lub_gy1 = j;
+        if (vy[(long)VW * j] >= 0.0) break;
+    }
+}
+
 int salec_host_setup(salec_host *h)
 {
     int rc = alloc_fields(h);
     if (rc) return rc;
     init_mesh(h);
+    lubrication_rows(h);
     tprof P;
     memset(&P, 0, sizeof P);
     if ((rc = build_profile(h, &P)) != SALEC_OK) { free_profile(&P); return rc; }

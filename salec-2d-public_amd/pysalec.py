@@ -24,7 +24,7 @@ FIELDS = [
     ("V_VEL", 1, 2), ("V_MASS", 1, 2), ("V_ALPHA", 1, 2), ("V_VOF", 1, -1), ("V_ACC", 1, 2),
     ("V_DIS", 1, 2), ("V_POS", 1, 2), ("V_BF", 1, 2), ("V_VOL", 1, 5), ("V_SUBVOL", 1, 4),
     ("V_SUF", 1, 8), ("V_LAPLACE", 1, 9), ("BX_NORMAL", 2, 2), ("BY_NORMAL", 3, 2), ("Y_PROFILE", 4, 1),
-    ("E_BETA", 0, 4),
+    ("E_BETA", 0, 4), ("E_CND", 0, 1),
 ]
 FIELD_ID = {name: i for i, (name, _, _) in enumerate(FIELDS)}
 NFIELDS = len(FIELDS)
@@ -32,7 +32,7 @@ NFIELDS = len(FIELDS)
 # persistent state carried from step to step (reference section 2.2)
 PERSISTENT = ["M_VOF", "M_DEN", "M_ENG", "E_STE", "E_DAM", "E_TPS", "E_VIB", "E_EJT", "E_STE2S",
               "E_DVEL", "E_ALPHA", "E_WPT", "E_DEN", "E_PRE", "E_CSD", "E_TEM", "E_MU", "E_CVS",
-              "V_VEL", "V_MASS", "V_ALPHA", "E_BETA"]
+              "V_VEL", "V_MASS", "V_ALPHA", "E_BETA", "E_CND"]
 TRANSIENT = ["E_VEL", "E_GRAD_VEL", "E_DIV_VEL", "E_Q", "E_STA2S", "E_STRENGTH", "V_VOF", "V_ACC",
              "V_DIS", "Y_PROFILE"]
 
@@ -78,7 +78,9 @@ class Config(C.Structure):
         ("partpres", C.c_int), ("tensile_failure", C.c_int), ("damp_type", C.c_int)] + [
         (n, C.c_double) for n in ("alpha_damp_x", "alpha_damp_y", "damp_time", "threshold_z", "flow_resistance",
                                   "flow_friction", "max_dt")] + [
-        ("go_list", C.c_int * 4), ("max_tracers", C.c_long)]
+        ("go_list", C.c_int * 4), ("max_tracers", C.c_long),
+        ("tracer_lubrication", C.c_int), ("lub_gy0", C.c_int), ("lub_gy1", C.c_int), ("lub_y0", C.c_double),
+        ("lub_frac", C.c_double)]
 
 
 class Tracers(C.Structure):

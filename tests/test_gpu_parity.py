@@ -198,6 +198,10 @@ VARIANTS = {
         "addition.nobject": 2, "addition.material": "[basalt, vacuum_]", "addition.type": "[sphere, cube]",
         "addition.parameter": "[ball, box]", "addition.ball": "[60.0, 120.0, 25.0, 300.0, -900.0, 293.0, 1.0e5, 0.2, -1]",
         "addition.box": "[80.0, -60.0, 60.0, 0.0, 0.0, 30.0, 0.0, 0.0, -1, -1, -1, -1]"})),
+    # tracers created in the four rows below the surface cap the strength of
+    # the cells they are in at min(0.5 p + 1e5, 0)
+    "tracer_lubrication": ("c1", dict(sections_over={"tracer.lubrication": "4", "tracer.lub_frac": "0.5",
+                                                     "tracer.lub_y0": "1.0e5"})),
     "anc_uniform_stretched_mesh": ("c0", dict(ANCPAT="uniform", ANC=0.3,
                                               sections_over={"mesh.ext": "on", "mesh.ex": "[1.04, 6, 6]",
                                                              "mesh.ey": "[1.04, 6, 6]"})),
