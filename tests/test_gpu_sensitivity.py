@@ -141,3 +141,36 @@ def test_sensitivity_calls_out_of_order(crater):
         dev.step()
     finally:
         dev.close()
+
+
+def test_values_equal_plain_run_with_lubrication():
+    """With tracer lubrication the tracers change the state, so the
+    sensitivity step runs them too: its value part still equals the plain
+    run, lubricated cells included."""
+    d = tempfile.mkdtemp(prefix="sens_lub_")
+    over = {"tracer.lubrication": "4", "tracer.lub_frac": "0.5", "tracer.lub_y0": "1.0e5"}
+    h = ps.HostRun(*ps.write_config(os.path.join(d, "c1"), "c1", nx=64, ny=96, sections_over=over))
+    cs = ps.CraterSensitivity(h, 30)
+    ntr = ps.lib().salec_host_count_tracers(h.h)
+    out = []
+    for params in (0, 7):
+        dev = ps.Device(h.cfg, max_tracers=ntr)
+        try:
+            dev.upload_state(cs.st0)
+            h.cfg.damp_time = dev.init(h.damp_t_in)
+            dev.upload_tracers(h.tracers(dev.download("V_VEL")))
+            dev.set_time(h.time0, h.dt0)
+            dev.sens_begin(params, cs.mask, cs.row, cs.speed0)
+            dev.sens_step(30)
+            L, g = dev.sens_result()
+            st = dev.state(["E_CND", "E_STRENGTH", "M_VOF", "E_STE", "V_VEL"])
+            dev.sens_end()
+        finally:
+            dev.close()
+        out.append((L, g, st))
+    say("lubrication: loss %.17g / %.17g, lubricated cells %d" % (out[0][0], out[1][0], int((out[0][2]["E_CND"] >= 1).sum())))
+    assert out[0][0] == out[1][0]
+    assert (out[0][2]["E_CND"] >= 1).sum() > 10
+    for k in out[0][2]:
+        assert np.array_equal(out[0][2][k], out[1][2][k], equal_nan=True), k
+    assert all(np.isfinite(out[1][1]))
