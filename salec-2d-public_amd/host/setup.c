@@ -1492,8 +1492,11 @@ static int distribute(salec_host *h, const tprof *Pp)
                 double vx = h->f[SALEC_V_POS][v], vy = h->f[SALEC_V_POS][NV + v];
                 double g0 = vx - h->t_center[0], g1 = vy - h->t_center[1];
                 double d = sqrt(0. + g0 * g0 + g1 * g1);
-                if (d == 0.) return SALEC_EINVAL;   /* a vertex on the centre has no direction (0/0 in the reference) */
+                /* vec_norm (src/geometry/linear.c:116-131): a 0/0 component
+                 * becomes 1 (a vertex on the centre; its g is 0) */
                 g0 /= d; g1 /= d;
+                if (isnan(g0)) g0 = 1.0;
+                if (isnan(g1)) g1 = 1.0;
                 int li = (int)floor(d / P.dz);
                 if (li > P.n - 1) li = P.n - 1;
                 if (li < 0) li = 0;

@@ -217,3 +217,15 @@ def test_complex_crater_config_sets_up(tmp_path):
     g = np.hypot(bf[0], bf[1])
     assert 1.4 < g.min() and g.max() < 1.7                # lunar surface gravity
     assert np.any(bf[0] != 0.0)                           # not uniform: it points at the centre
+
+
+def test_vertex_on_the_planet_centre(tmp_path):
+    """vec_norm turns the 0/0 direction of a vertex on the centre into
+    (1, 1) (src/geometry/linear.c:116-131); g there is 0, so its body force
+    is zero and finite."""
+    R = 2000.0                                  # the centre (0, -2000) is a vertex of the 10 m mesh
+    h = stiff_sphere(tmp_path, R=R)
+    vp, bf = h.field("V_POS"), h.field("V_BF")
+    at = (vp[0] == 0.0) & (vp[1] == -R)
+    assert at.sum() == 1
+    assert np.all(bf[:, at] == 0.0) and np.all(np.isfinite(bf))
