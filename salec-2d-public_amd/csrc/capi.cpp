@@ -34,6 +34,7 @@ struct Program {
     int nops = 0;
     int two_phase = 0;
     int maxc = 1;
+    int nconf = 0;                        /* leading ops with ordering hazards */
 };
 
 enum {
@@ -343,6 +344,7 @@ struct SlotSpec { int field; int vt; };
 static int build_program(salec_ctx *c, Program &P, const std::vector<SlotSpec> &specs)
 {
     std::vector<GhostOp> ops;
+    std::vector<char> conf;
     P.nslots = (int)specs.size();
     P.two_phase = 0;
     P.maxc = 1;
@@ -374,11 +376,29 @@ static int build_program(salec_ctx *c, Program &P, const std::vector<SlotSpec> &
                 if (x >= 2 && x < W - 2 && y >= 2 && y < H - 2) P.phys_dst.push_back(L);
             }
         }
+        /* hazards within the slot: an op reading a point another op writes,
+         * and an op writing a point another op reads */
+        std::unordered_map<int, int> read_by_other;
         for (size_t i = first; i < ops.size(); ++i) {
             const GhostOp &o = ops[i];
             bool reads = !(o.m0 >= GM_ZERO && o.m1 >= GM_ZERO && o.mr >= GM_ZERO);
-            if (reads && o.src != o.dst && dsts.count(o.src)) P.two_phase = 1;
+            if (reads && o.src != o.dst) read_by_other[o.src] = 1;
         }
+        for (size_t i = first; i < ops.size(); ++i) {
+            const GhostOp &o = ops[i];
+            bool reads = !(o.m0 >= GM_ZERO && o.m1 >= GM_ZERO && o.mr >= GM_ZERO);
+            bool hz = (reads && o.src != o.dst && dsts.count(o.src)) || read_by_other.count(o.dst);
+            if (reads && o.src != o.dst && dsts.count(o.src)) P.two_phase = 1;
+            conf.push_back(hz ? 1 : 0);
+        }
+    }
+    /* hazardous ops first (the order of independent ops does not matter) */
+    {
+        std::vector<GhostOp> re;
+        for (size_t i = 0; i < ops.size(); ++i) if (conf[i]) re.push_back(ops[i]);
+        P.nconf = (int)re.size();
+        for (size_t i = 0; i < ops.size(); ++i) if (!conf[i]) re.push_back(ops[i]);
+        ops.swap(re);
     }
     P.nops = (int)ops.size();
     if (P.nops) {
@@ -395,7 +415,7 @@ static void run_program(salec_ctx *c, const Program &P)
 {
     GhostSlots sl = P.slots;   /* field buffers swap between steps */
     for (int s = 0; s < P.nslots; ++s) sl.base[s] = c->F.f[P.field[s]];
-    launch_ghost(c->stream, sl, P.d_ops, P.nops, c->d_stage, P.two_phase, P.maxc);
+    launch_ghost(c->stream, sl, P.d_ops, P.nops, c->d_stage, P.two_phase, P.maxc, P.nconf);
 }
 
 /* Fields the remap writes into the second buffer set (k_remap_close); with

@@ -209,11 +209,53 @@ __global__ void k_ghost_scatter(GhostSlots sl, const GhostOp *ops, int nops, con
     int nc = sl.ncomp[o.slot];
     for (int c = 0; c < nc; ++c) b[c * pl + o.dst] = stage[(long)c * nops + i];
 }
-void launch_ghost(hipStream_t s, const GhostSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase, int maxc)
+/* one launch for a program with ordering hazards: workgroup 0 applies the
+ * nconf hazardous ops through LDS (every read before any write), the other
+ * workgroups the independent ops, which read and write points no other op
+ * touches */
+__global__ void __launch_bounds__(256) k_ghost_fused(GhostSlots sl, const GhostOp *ops, int nops, int nconf)
+{
+    if (blockIdx.x == 0) {
+        __shared__ double st[GHOST_LDS];
+        for (int i = threadIdx.x; i < nconf; i += blockDim.x) {
+            GhostOp o = ops[i];
+            const double *b = sl.base[o.slot];
+            long pl = sl.plane[o.slot];
+            int nc = sl.ncomp[o.slot];
+            for (int c = 0; c < nc; ++c) {
+                int m = c == 0 ? o.m0 : (c == 1 ? o.m1 : o.mr);
+                st[c * nconf + i] = gmode(b[c * pl + o.src], m);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nconf; i += blockDim.x) {
+            GhostOp o = ops[i];
+            double *b = sl.base[o.slot];
+            long pl = sl.plane[o.slot];
+            int nc = sl.ncomp[o.slot];
+            for (int c = 0; c < nc; ++c) b[c * pl + o.dst] = st[c * nconf + i];
+        }
+        return;
+    }
+    int i = nconf + (blockIdx.x - 1) * blockDim.x + threadIdx.x;
+    if (i >= nops) return;
+    GhostOp o = ops[i];
+    double *b = sl.base[o.slot];
+    long pl = sl.plane[o.slot];
+    int nc = sl.ncomp[o.slot];
+    for (int c = 0; c < nc; ++c) {
+        int m = c == 0 ? o.m0 : (c == 1 ? o.m1 : o.mr);
+        b[c * pl + o.dst] = gmode(b[c * pl + o.src], m);
+    }
+}
+void launch_ghost(hipStream_t s, const GhostSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase,
+                  int maxc, int nconf)
 {
     if (nops <= 0) return;
     if (!two_phase) {
         k_ghost_direct<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops);
+    } else if ((long)nconf * maxc <= GHOST_LDS) {
+        k_ghost_fused<<<1 + nblk(nops - nconf, 256), 256, 0, s>>>(sl, ops, nops, nconf);
     } else {
         k_ghost_gather<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, maxc);
         k_ghost_scatter<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage);

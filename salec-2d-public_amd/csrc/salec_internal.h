@@ -99,7 +99,14 @@ struct Tracers {
 /* launchers (kernels.hip) */
 void launch_trace_profile(hipStream_t s, const Grid &g, const Fields &F, const Params &P, double *part, int nchunk);
 void launch_profile_finish(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const double *part, int nchunk);
-void launch_ghost(hipStream_t s, const GhostSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase, int maxc);
+/* nconf: the first nconf ops read a point another op writes, or write a
+ * point another op reads; with nconf * maxc <= GHOST_LDS they are applied by
+ * one workgroup (all reads, a barrier, all writes) and the rest directly by
+ * the others, in one launch; otherwise the program is staged through
+ * `stage` in two launches */
+#define GHOST_LDS 4096
+void launch_ghost(hipStream_t s, const GhostSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase,
+                  int maxc, int nconf);
 void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const Params &P);
 void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F);
 void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F);
