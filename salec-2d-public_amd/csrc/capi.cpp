@@ -644,7 +644,7 @@ int salec_upload(salec_ctx *c, int field, const double *host)
     int grid, nc;
     if (salec_field_info(field, c->g.nmat, &grid, &nc)) return SALEC_EINVAL;
     for (int gf = 0; gf < GF_COUNT; ++gf) if (kGeoField[gf] == field) c->geo_dirty = 1;
-    if (field == SALEC_M_PTY) c->geo_dirty = 1;
+    if (field == SALEC_M_PTY || field == SALEC_V_LAPLACE) c->geo_dirty = 1;
     size_t n = (size_t)salec_grid_points(grid, c->g.nx, c->g.ny) * nc;
     HIPCHK(hipMemcpyAsync(c->F.f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -774,6 +774,25 @@ static int build_geo(salec_ctx *c)
                 rowmap[(size_t)(kGeoBase[gf] + comp) * RH + j] = r;
             }
     }
+    /* per vertex row, the columns whose ANC weights are not all zero: a
+     * vertex outside them adds a Laplacian of +0 (finite velocities), which
+     * leaves its acceleration as it is, so the kernel skips the term there */
+    size_t ranc = rowmap.size();
+    {
+        std::vector<double> lap((size_t)9 * g.NV);
+        HIPCHK(hipMemcpyAsync(lap.data(), c->F.f[SALEC_V_LAPLACE], sizeof(double) * lap.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        rowmap.resize(ranc + 2 * (size_t)g.VH);
+        for (int j = 0; j < g.VH; ++j) {
+            int lo = g.VW, hi = -1;
+            for (int k = 0; k < g.VW; ++k)
+                for (int q = 0; q < 9; ++q)
+                    if (lap[(size_t)q * g.NV + k + (size_t)g.VW * j] != 0.0) { lo = std::min(lo, k); hi = k; break; }
+            rowmap[ranc + 2 * j] = lo;
+            rowmap[ranc + 2 * j + 1] = hi;
+        }
+    }
     if (c->d_geo) { hipFree(c->d_geo); c->d_geo = nullptr; }
     if (c->d_georow) { hipFree(c->d_georow); c->d_georow = nullptr; }
     HIPCHK(hipMalloc(&c->d_geo, sizeof(double) * pool.size()));
@@ -781,6 +800,7 @@ static int build_geo(salec_ctx *c)
     HIPCHK(hipMemcpy(c->d_geo, pool.data(), sizeof(double) * pool.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_georow, rowmap.data(), sizeof(int) * rowmap.size(), hipMemcpyHostToDevice));
     g.geo = c->d_geo; g.georow = c->d_georow; g.geoRH = RH; g.geoPW = PW;
+    g.ancr = c->d_georow + ranc;
     c->geo_rows = (long)width.size();
     /* without a porosity model m_pty never changes: note whether it is 1 */
     c->P.pty_one = 0;

@@ -804,15 +804,15 @@ __global__ void __launch_bounds__(256) k_acc(Grid g, Fields F, TF<S> T, Params P
             vs0 += 0.25 * GEO(E_SIGMA, 0, ek[i], ej[i]);
             vs1 += 0.25 * GEO(E_SIGMA, 1, ek[i], ej[i]);
         }
-        al0 = VPL(V_ALPHA, 0, v); al1 = VPL(V_ALPHA, 1, v);
         damp = (vs0 > 0. || vs1 > 0.);
+        /* outside the damping layer v_alpha is neither used nor changed */
+        if (damp) { al0 = VPL(V_ALPHA, 0, v); al1 = VPL(V_ALPHA, 1, v); }
     }
     for (int i = 0; i < 4; ++i) {
         double su0 = GEO(V_SUF, 2 * i, k, j), su1 = GEO(V_SUF, 2 * i + 1, k, j), sv = GEO(V_SUBVOL, i, k, j);
         S lp = CPL(E_PRE, 0, e[i]) + CPL(E_Q, 0, e[i]);
         S s0 = CPL(E_STE, 0, e[i]), s1 = CPL(E_STE, 1, e[i]), s2 = CPL(E_STE, 2, e[i]), s3 = CPL(E_STE, 3, e[i]);
         if (PML) {
-            double lpr = CPL(E_DAMPREF, 2, e[i]);
             S Sxx = s0 - lp, Szz = s2 - lp, Sxz = s1, Sth = s3 - lp;
             S f0 = 0, f1 = 0;
             f0 += su0 * Sxx; f0 += su1 * Sxz;
@@ -820,6 +820,7 @@ __global__ void __launch_bounds__(256) k_acc(Grid g, Fields F, TF<S> T, Params P
             f0 += sv * Sth;
             a0 += -1.0 * f0; a1 += -1.0 * f1;
             if (damp) {
+                double lpr = CPL(E_DAMPREF, 2, e[i]);
                 Sxx += lpr; Szz += lpr; Sth += lpr;
                 S b0 = 0, b1 = 0;
                 b0 += su0 * (vs1 * Sxx); b0 += su1 * (vs0 * Sxz);
@@ -848,12 +849,12 @@ __global__ void __launch_bounds__(256) k_acc(Grid g, Fields F, TF<S> T, Params P
             al0 -= P.alpha_damp_x * al0 * dt;
             al1 -= P.alpha_damp_y * al1 * dt;
             a0 += -1.0 * al0; a1 += -1.0 * al1;
+            VPL(V_ALPHA, 0, v) = al0; VPL(V_ALPHA, 1, v) = al1;
         }
-        VPL(V_ALPHA, 0, v) = al0; VPL(V_ALPHA, 1, v) = al1;
     }
     int xd = min(k, g.VW - 1 - k), yd = min(j, g.VH - 1 - j);
     S n0 = 0., n1 = 0.;
-    if (!(xd <= 1 || yd <= 1)) {
+    if (!(xd <= 1 || yd <= 1) && k >= g.ancr[2 * j] && k <= g.ancr[2 * j + 1]) {
         S L0 = 0, L1 = 0;
         int q = 0;
         for (int dj = -1; dj <= 1; ++dj)

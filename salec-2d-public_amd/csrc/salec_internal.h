@@ -63,6 +63,8 @@ struct Grid {
     long NC, NV, NBX, NBY;
     const double *geo;              /* pool of distinct geometry rows, width geoPW */
     const int *georow;              /* [kGeoComps][geoRH] pool row of each grid row */
+    const int *ancr;                /* [VH][2] first and last vertex column whose
+                                     * v_laplace weights are not all zero */
     int geoRH, geoPW;
 };
 
