@@ -19,14 +19,24 @@
  * geometry (swept volume, cut length, edge ratio, MSTACS) and the strength
  * models against closed forms and limits.
  *
- * Parity unpinned: update_v_vof, update_v_vel_inc, update_e_vel,
- * check_v_vel, update_e_gradvel + calculate_dt, block_vibration, the stress
- * update around the strength functions, calculate_acc, calculate_v_dis,
- * update_energy, the composition of calculate_flux_mixed and
- * advect_flux_default, damp_on_cell, ejecta_flow, the mixture rules of
- * update_state, ghost fills/restrictions, fl_update_node and trace_profile
- * have no reference vector or closed form; they are checked only as a whole
- * (rest state, lithostatic equilibrium, mass conservation, a recorded run).
+ * Pinned one function at a time (tests/test_oracle_step_pins.py, through
+ * oracle_phase) by closed forms: update_e_gradvel (exact on linear fields,
+ * hoop term, viscosity scaling), calculate_dt (CFL), update_e_vel,
+ * update_v_vof, update_v_vel_inc, calculate_acc (uniform and linear
+ * pressure, axisymmetric divergence of a uniform deviator),
+ * calculate_v_dis, update_energy (pressure work, deviatoric power),
+ * block_vibration, update_e_stress (Jaumann rotation keeps J2, Hooke's law
+ * for the deviator, radial return), update_state (mixture averages, vacuum
+ * rule), damp_on_cell pml2 (velocity relaxation), ejecta_flow (drag,
+ * friction), fl_update_node, trace_profile, the axis/outflow/clean ghost
+ * fills and the vertex restrictions.  tests/test_oracle_lubrication.py pins
+ * update_strength_condition and the lubrication cap.
+ *
+ * Parity unpinned: check_v_vel, the pml/sbc damping variants and the pml
+ * auxiliaries (e_alpha, v_alpha, e_beta), and the composition of
+ * calculate_flux_mixed with advect_flux_default beyond the geometry
+ * helpers above; they are checked as a whole (rest state, lithostatic
+ * equilibrium, mass conservation in a closed box, a recorded run).
  */
 #ifndef SALEC_ORACLE_H
 #define SALEC_ORACLE_H
@@ -77,6 +87,16 @@ double oracle_till_temp(const salec_material *m, double eng, double rho);
 void   oracle_till_cold_energy(const salec_material *m, int nden, double *xden, double *yeng);
 int    oracle_bisearch(double x, const double *xl, int n);
 void   oracle_ghost_fill(oracle_state *s, const salec_config *cfg, int field, int var_type);
+
+/* One function of the step over the whole grid, for the tests that pin it
+ * on its own (dt and t are s->dt and s->t; ORACLE_PH_DT stores the new dt in
+ * s->dt).  Returns -1 for an unknown phase. */
+enum {
+    ORACLE_PH_V_VOF = 0, ORACLE_PH_V_VEL_INC, ORACLE_PH_E_VEL, ORACLE_PH_CHECK_V_VEL, ORACLE_PH_GRADVEL,
+    ORACLE_PH_DT, ORACLE_PH_VIBRATION, ORACLE_PH_STRESS, ORACLE_PH_ACC, ORACLE_PH_V_DIS, ORACLE_PH_ENERGY,
+    ORACLE_PH_TRACERS, ORACLE_PH_PROFILE, ORACLE_PH_EJECTA, ORACLE_PH_DAMP
+};
+int    oracle_phase(oracle_state *s, const salec_config *cfg, int phase);
 void   oracle_restrict(oracle_state *s, const salec_config *cfg, int field);
 
 /* ght_var_type, src/mpi_vec/mpi_vec_types.h */

@@ -42,6 +42,8 @@ def olib():
     L.oracle_init.argtypes = [P, CP, C.POINTER(C.c_double)]
     L.oracle_init.restype = C.c_double
     L.oracle_step.argtypes = [P, CP]
+    L.oracle_phase.argtypes = [P, CP, C.c_int]
+    L.oracle_phase.restype = C.c_int
     L.oracle_step_to_output.argtypes = [P, CP]
     L.oracle_step_from_output.argtypes = [P, CP]
     L.oracle_update_state.argtypes = [P, CP]
@@ -105,6 +107,28 @@ class Oracle:
 
     def step(self):
         self.L.oracle_step(C.byref(self.s), C.byref(self.cfg))
+
+    PHASES = ("V_VOF", "V_VEL_INC", "E_VEL", "CHECK_V_VEL", "GRADVEL", "DT", "VIBRATION", "STRESS", "ACC", "V_DIS",
+              "ENERGY", "TRACERS", "PROFILE", "EJECTA", "DAMP")
+
+    def phase(self, name, dt=None, t=None):
+        """Run one function of the step (oracle_phase) on the held state."""
+        if dt is not None:
+            self.s.dt = dt
+        if t is not None:
+            self.s.t = t
+        if self.L.oracle_phase(C.byref(self.s), C.byref(self.cfg), self.PHASES.index(name)) != 0:
+            raise ValueError(name)
+
+    def update_state(self):
+        """The mixture closure of every physical cell (oracle_update_state)."""
+        self.L.oracle_update_state(C.byref(self.s), C.byref(self.cfg))
+
+    def ghost_fill(self, field, var_type):
+        self.L.oracle_ghost_fill(C.byref(self.s), C.byref(self.cfg), ps.FIELD_ID[field], var_type)
+
+    def restrict(self, field):
+        self.L.oracle_restrict(C.byref(self.s), C.byref(self.cfg), ps.FIELD_ID[field])
 
     def step_to_output(self):
         self.L.oracle_step_to_output(C.byref(self.s), C.byref(self.cfg))

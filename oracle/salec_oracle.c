@@ -12,8 +12,8 @@
  * bilinear in between, strength closed forms, MSTACS piecewise form, swept
  * volume against brute-force quadrature, cut_length as the inverse of the
  * area fraction, conservation of mass/volume by the remap, rest state under
- * gravity, ghost-table brute force.  Functions without such a pin are listed
- * as "parity unpinned" in DESIGN.md.
+ * gravity, and the step functions one at a time (oracle_phase) against
+ * closed forms; oracle.h lists what stays unpinned.
  */
 #include "oracle.h"
 #include <math.h>
@@ -1894,6 +1894,40 @@ void oracle_step_from_output(oracle_state *s, const salec_config *cfg)
     s->t += s->dt;
     s->step++;
 }
+int oracle_phase(oracle_state *s, const salec_config *cfg, int phase)
+{
+    grid_t g = mkgrid(s);
+    double dt = s->dt, t = s->t;
+    int pmlacc = (cfg->damp_type == SALEC_DAMP_PML2 || cfg->damp_type == SALEC_DAMP_PML3);
+    switch (phase) {
+    case ORACLE_PH_V_VOF: update_v_vof(s, g, 1); update_v_vof(s, g, 2); break;
+    case ORACLE_PH_V_VEL_INC: update_v_vel_inc(s, cfg, g, 1); update_v_vel_inc(s, cfg, g, 2); break;
+    case ORACLE_PH_E_VEL: update_e_vel(s, g); break;
+    case ORACLE_PH_CHECK_V_VEL: check_v_vel(s, g); break;
+    case ORACLE_PH_GRADVEL: update_e_gradvel(s, cfg, g); break;
+    case ORACLE_PH_DT: s->dt = Min(cfg->max_dt, calculate_dt(s, g, s->dt)); break;
+    case ORACLE_PH_VIBRATION: block_vibration(s, cfg, g, dt, t); break;
+    case ORACLE_PH_STRESS: update_e_stress(s, cfg, g, dt); break;
+    case ORACLE_PH_ACC:
+        if (pmlacc) { calculate_acc_pml(s, cfg, g, dt, 1); calculate_acc_pml(s, cfg, g, dt, 2); }
+        else { calculate_acc_default(s, cfg, g, 1); calculate_acc_default(s, cfg, g, 2); }
+        break;
+    case ORACLE_PH_V_DIS: calculate_v_dis(s, g, dt); break;
+    case ORACLE_PH_ENERGY: update_energy(s, cfg, g, dt); break;
+    case ORACLE_PH_TRACERS: tracer_update(s, cfg, g, dt); break;
+    case ORACLE_PH_PROFILE: trace_profile(s, cfg, g); break;
+    case ORACLE_PH_EJECTA: ejecta_flow(s, cfg, g, dt); break;
+    case ORACLE_PH_DAMP:
+        if (cfg->damp_type == SALEC_DAMP_PML2 || cfg->damp_type == SALEC_DAMP_PML3)
+            damp_on_cell_pml2(s, cfg, g, dt, t, cfg->damp_type == SALEC_DAMP_PML3);
+        else if (cfg->damp_type == SALEC_DAMP_PML) damp_on_cell_pml(s, cfg, g, dt, t);
+        else if (cfg->damp_type == SALEC_DAMP_SBC) damp_on_cell_sbc(s, cfg, g, t);
+        break;
+    default: return -1;
+    }
+    return 0;
+}
+
 void oracle_step(oracle_state *s, const salec_config *cfg)
 {
     oracle_step_to_output(s, cfg);

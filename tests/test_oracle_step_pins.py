@@ -1,0 +1,534 @@
+"""Closed-form pins of the oracle's step functions, one function at a time
+(oracle_phase), on the real mesh geometry of the host set-up.
+
+Each test builds a state whose answer the mathematics fixes and checks the
+function against it; the cases are chosen so that a dropped term, a wrong
+sign, a swapped component or a wrong index fails:
+
+* velocity gradient (update_e_gradvel, src/sale2d.c:1144-1235): exact for
+  linear velocity fields on uniform and stretched meshes; the hoop term is
+  v_r / r at the cell centre; artificial viscosity zero in expansion,
+  quadratic (linear) in the compression rate for the quadratic (linear)
+  coefficient alone;
+* CFL time step (calculate_dt, src/sale2d.c:1959-2016): 0.2 h / (|u| + c)
+  on a uniform mesh, with the 1.1 growth cap;
+* cell velocity (update_e_vel, src/sale2d.c:838-927): the vertex value of a
+  uniform field, the centre value of a linear one;
+* vertex volume fractions (update_v_vof): the average of the four cells, a
+  partition of unity;
+* vertex momentum (update_v_vel_inc, src/sale2d.c:930-1048): a uniform
+  increment adds to a uniform velocity;
+* nodal acceleration (calculate_acc_default, src/velocity.c:9-155): zero
+  net force of a uniform pressure, -dp/dz / rho of a linear one, and the
+  axisymmetric divergence ((S_rr - S_tt) / r, S_rz / r) / rho of a uniform
+  deviatoric stress, exactly on a uniform mesh and to 1 % on a stretched one;
+* displacement (calculate_v_dis): constant-acceleration kinematics summed
+  over steps;
+* internal energy (update_energy, src/sale2d.c:564-712): pressure work
+  -(p + q) div v and deviatoric power S_rz (dv_r/dz + dv_z/dr);
+* block vibration (block_vibration, src/strength.c:625-695): decay close to
+  exp(-2 dt / T_d), excitation to (C_vib |v|)^2 capped at VibMax^2;
+* tracers (fl_update_node, src/tracer.c:52-175): constant-acceleration
+  kinematics in a cell open to vacuum, the mesh displacement of a linear
+  field inside the material;
+* crater profile (trace_profile, src/ejecta.c:8-63): a flat surface gives
+  the centre of the first vacuum row in every column;
+* ejecta flow (src/ejecta.c:65-134): linear drag and Coulomb friction
+  increments.
+"""
+import ctypes as C
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import pysalec as ps
+
+IN = (slice(2, -2), slice(2, -2))          # physical cells
+VIN = (slice(3, -3), slice(3, -3))         # vertices away from the padding
+STRETCH = {"mesh.ext": "on", "mesh.ex": "[1.04, 6, 6]", "mesh.ey": "[1.04, 6, 6]"}
+
+
+def fresh(which="c0", nx=24, ny=36, **over):
+    h = ps.HostRun(*ps.write_config(os.path.join(tempfile.mkdtemp(prefix="pin_"), which), which, nx=nx, ny=ny,
+                                    sections_over=over))
+    return h, po.Oracle(h.cfg, h.state())
+
+
+@pytest.mark.parametrize("mesh", ["uniform", "stretched"])
+def test_gradient_of_linear_velocity(mesh):
+    h, o = fresh(**(STRETCH if mesh == "stretched" else {}))
+    S = o.state
+    r, z = S["V_POS"][0], S["V_POS"][1]
+    a, b, c0, d, e, c1 = 0.3, -0.7, 2.0, 1.1, 0.5, -3.0
+    S["V_VEL"][0] = a * r + b * z + c0
+    S["V_VEL"][1] = d * r + e * z + c1
+    o.phase("GRADVEL")
+    g = S["E_GRAD_VEL"]
+    for i, x in enumerate((a, b, d, e)):          # dvr/dr, dvr/dz, dvz/dr, dvz/dz
+        np.testing.assert_allclose(g[i][IN], x, rtol=0, atol=1e-13)
+    rc, zc = S["E_POS"][0][IN], S["E_POS"][1][IN]
+    hoop = (a * rc + b * zc + c0) / rc            # v_r / r at the centre
+    np.testing.assert_allclose(g[4][IN], hoop, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(S["E_DIV_VEL"][0][IN], a + e + hoop, rtol=1e-12, atol=1e-13)
+
+
+def test_artificial_viscosity_scaling():
+    def q_of(rate, avis1, avis2):
+        h, o = fresh(**{"numerical.AVISL": str(avis1), "numerical.AVISQ": str(avis2)})
+        S = o.state
+        S["M_VOF"][0] = 0.0
+        S["M_VOF"][1] = 1.0
+        S["E_PRE"][0] = 1.0e8
+        S["E_CSD"][0] = 3000.0
+        S["E_DEN"][0] = 2500.0
+        r, z = S["V_POS"][0], S["V_POS"][1]
+        S["V_VEL"][0] = 0.0 * r
+        S["V_VEL"][1] = rate * z                  # uniaxial strain rate, div = rate
+        o.phase("GRADVEL")
+        return S["E_Q"][0][IN].copy()
+    assert np.all(q_of(+5.0, 0.5, 1.2) == 0.0)    # expansion: no viscosity
+    q1, q2 = q_of(-5.0, 0.0, 1.2), q_of(-10.0, 0.0, 1.2)
+    assert np.all(q1 > 0)
+    np.testing.assert_allclose(q2, 4.0 * q1, rtol=1e-12)      # quadratic term ~ div^2
+    l1, l2 = q_of(-5.0, 0.5, 0.0), q_of(-10.0, 0.5, 0.0)
+    np.testing.assert_allclose(l2, 2.0 * l1, rtol=1e-12)      # linear term ~ |div|
+
+
+def test_cfl_time_step():
+    h, o = fresh(**{"cycle.maxdt": "10.0"})
+    S = o.state
+    c, u = 2500.0, 400.0
+    S["E_CSD"][0] = c
+    S["E_VEL"][0] = u
+    S["E_VEL"][1] = 0.0
+    o.phase("DT", dt=1.0)
+    assert o.dt == pytest.approx(0.2 * 10.0 / (u + c), rel=1e-15)    # uniform 10 m cells
+    o.phase("DT", dt=1.0e-4)
+    assert o.dt == pytest.approx(1.1e-4, rel=1e-15)                   # growth cap
+
+
+def test_cell_velocity_from_vertices():
+    h, o = fresh(**STRETCH)
+    S = o.state
+    S["E_DEN"][0] = 1000.0
+    S["E_WPT"][:] = -0.25
+    S["V_MASS"][0] = 1.0 + np.random.default_rng(1).random(S["V_MASS"][0].shape)
+    S["V_MASS"][1] = 0.0
+    S["V_VEL"][0] = 7.5
+    S["V_VEL"][1] = -2.25
+    o.phase("E_VEL")
+    np.testing.assert_allclose(S["E_VEL"][0][IN], 7.5, rtol=1e-14)
+    np.testing.assert_allclose(S["E_VEL"][1][IN], -2.25, rtol=1e-14)
+    # a linear field with equal masses: the average of the corners
+    r, z = S["V_POS"][0], S["V_POS"][1]
+    S["V_MASS"][0] = 1.0
+    S["V_VEL"][0] = 0.2 * r - 0.1 * z
+    o.phase("E_VEL")
+    vv = S["V_VEL"][0]
+    corner = 0.25 * (vv[:-1, :-1] + vv[:-1, 1:] + vv[1:, 1:] + vv[1:, :-1])
+    np.testing.assert_allclose(S["E_VEL"][0][IN], corner[IN], rtol=1e-13)
+
+
+def test_vertex_volume_fractions_partition_of_unity():
+    h, o = fresh("c1")
+    S = o.state
+    rng = np.random.default_rng(2)
+    w = rng.random(S["M_VOF"].shape)
+    S["M_VOF"][:] = w / w.sum(axis=0)
+    o.phase("V_VOF")
+    vof = S["V_VOF"]
+    np.testing.assert_allclose(vof.sum(axis=0)[1:-1, 1:-1], 1.0, rtol=0, atol=4e-16)
+    m = S["M_VOF"]
+    avg = 0.25 * (m[:, :-1, :-1] + m[:, :-1, 1:] + m[:, 1:, 1:] + m[:, 1:, :-1])
+    np.testing.assert_allclose(vof[:, 1:-1, 1:-1], avg, rtol=1e-15, atol=1e-16)
+
+
+def test_vertex_momentum_increment():
+    h, o = fresh(**{"numerical.MaxVelocity": "1.0e6", "numerical.MinVelocity": "1.0e-8"})
+    S = o.state
+    S["E_DEN"][0] = 2000.0
+    S["E_WPT"][:] = -0.25
+    S["E_DVEL"][0] = 3.0
+    S["E_DVEL"][1] = -1.5
+    vol = S["V_VOL"][1:5].sum(axis=0)
+    S["V_MASS"][0] = 2000.0 * vol                  # the mass the vertex will get
+    S["V_VEL"][0] = 10.0
+    S["V_VEL"][1] = 20.0
+    o.phase("V_VEL_INC")
+    np.testing.assert_allclose(S["V_VEL"][0][VIN], 13.0, rtol=1e-14)
+    np.testing.assert_allclose(S["V_VEL"][1][VIN], 18.5, rtol=1e-14)
+
+
+@pytest.mark.parametrize("mesh", ["uniform", "stretched"])
+def test_nodal_acceleration_closed_forms(mesh):
+    over = {"numerical.damping": "off", "numerical.ANC": "0.0"}
+    if mesh == "stretched":
+        over.update(STRETCH)
+    h, o = fresh(**over)
+    S = o.state
+    rho = 2000.0
+    S["E_DEN"][0] = rho
+    S["E_Q"][:] = 0.0
+    S["E_STE"][:] = 0.0
+    S["V_MASS"][:] = 0.0
+    bf = S["V_BF"]
+    acc = S["V_ACC"]
+    # a uniform pressure exerts no net force
+    S["E_PRE"][0] = 1.0e9
+    o.phase("ACC", dt=1e-3)
+    np.testing.assert_allclose(acc[:, VIN[0], VIN[1]], bf[:, VIN[0], VIN[1]], rtol=0, atol=1e-11)
+    # a linear pressure p = p0 + k z: a_z = g_z - k / rho
+    k = 1.0e5
+    S["E_PRE"][0] = 1.0e9 + k * S["E_POS"][1]
+    o.phase("ACC", dt=1e-3)
+    np.testing.assert_allclose(acc[1][VIN], bf[1][VIN] - k / rho, rtol=0, atol=5e-11)
+    assert np.all(acc[0][VIN] == bf[0][VIN])
+    # a uniform deviatoric stress: ((S_rr - S_tt) / r, S_rz / r) / rho
+    S["E_PRE"][0] = 0.0
+    S["E_STE"][0], S["E_STE"][1], S["E_STE"][2], S["E_STE"][3] = 3.0e6, 1.0e6, -1.0e6, -2.0e6
+    o.phase("ACC", dt=1e-3)
+    r = S["V_POS"][0][VIN]
+    tol = 1e-13 if mesh == "uniform" else 2e-2
+    np.testing.assert_allclose(acc[0][VIN] - bf[0][VIN], (3.0e6 + 2.0e6) / (rho * r), rtol=tol)
+    np.testing.assert_allclose(acc[1][VIN] - bf[1][VIN], 1.0e6 / (rho * r), rtol=tol)
+
+
+def test_displacement_kinematics():
+    h, o = fresh()
+    S = o.state
+    a, v0, dt, n = np.array([1.5, -9.0]), np.array([4.0, 2.0]), 1.0e-3, 40
+    S["V_VEL"][0], S["V_VEL"][1] = v0
+    total = np.zeros_like(S["V_DIS"])
+    for _ in range(n):
+        S["V_ACC"][0], S["V_ACC"][1] = a
+        o.phase("V_DIS", dt=dt)
+        total += S["V_DIS"]
+    T = n * dt
+    for c in range(2):
+        np.testing.assert_allclose(total[c][1:-1, 1:-1], v0[c] * T + 0.5 * a[c] * T * T, rtol=1e-12)
+        np.testing.assert_allclose(S["V_VEL"][c][1:-1, 1:-1], v0[c] + a[c] * T, rtol=1e-13)
+
+
+def test_energy_work_terms():
+    h, o = fresh("c1")
+    S = o.state
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["M_DEN"][1] = 1800.0
+    S["E_DEN"][0] = 1800.0
+    S["M_ENG"][1] = 5.0e8
+    dt = 1e-4
+    # pure shear: S_rz (dv_r/dz + dv_z/dr)
+    tau, g1, g2 = 2.0e6, 0.3, -0.1
+    S["E_STE"][:] = 0.0
+    S["E_STE"][1] = tau
+    S["E_GRAD_VEL"][:] = 0.0
+    S["E_GRAD_VEL"][1], S["E_GRAD_VEL"][2] = g1, g2
+    S["E_DIV_VEL"][0] = 0.0
+    S["E_PRE"][0] = 0.0
+    S["E_Q"][0] = 0.0
+    o.phase("ENERGY", dt=dt)
+    np.testing.assert_allclose(S["M_ENG"][1][IN], 5.0e8 + tau * (g1 + g2) * dt, rtol=1e-15)
+    # pure compression: -(p + q) div v
+    S["M_ENG"][1] = 5.0e8
+    S["E_STE"][:] = 0.0
+    S["E_GRAD_VEL"][:] = 0.0
+    S["E_DIV_VEL"][0] = -2.0
+    S["E_PRE"][0] = 3.0e7
+    S["E_Q"][0] = 1.0e6
+    o.phase("ENERGY", dt=dt)
+    np.testing.assert_allclose(S["M_ENG"][1][IN], 5.0e8 + 3.1e7 * 2.0 * dt, rtol=1e-15)
+
+
+def test_block_vibration_decay_and_excitation():
+    over = {"material.Toff": "[, 16.0]", "material.Cvib": "[, 0.1]", "material.VibMax": "[, 200.0]"}
+    h, o = fresh(**over)
+    S = o.state
+    S["M_VOF"][0] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["E_DAM"][0] = 1.0
+    S["E_Q"][0] = 0.0
+    S["E_VIB"][0] = 100.0
+    Td = h.cfg.mat[1].Tdamp
+    for x in (1e-3, 1e-2):
+        S["E_VIB"][0] = 100.0
+        o.phase("VIBRATION", dt=x * Td, t=1.0)
+        f = S["E_VIB"][0][IN] / 100.0
+        assert np.all(np.abs(f - math.exp(-2 * x)) <= 4.0 / 3.0 * x ** 3 + 1e-15)
+    # excitation while t <= Toff, q > 0 and C_vib |v| >= 1
+    S["E_VIB"][0] = 0.0
+    S["E_Q"][0] = 1.0
+    S["E_VEL"][0], S["E_VEL"][1] = 30.0, 40.0          # |v| = 50, C_vib |v| = 5
+    o.phase("VIBRATION", dt=1e-3, t=1.0)
+    np.testing.assert_allclose(S["E_VIB"][0][IN], 25.0, rtol=1e-15)
+    S["E_VIB"][0] = 0.0
+    S["E_VEL"][0], S["E_VEL"][1] = 3.0e4, 4.0e4        # capped at VibMax^2
+    o.phase("VIBRATION", dt=1e-3, t=1.0)
+    np.testing.assert_allclose(S["E_VIB"][0][IN], 200.0 ** 2, rtol=1e-15)
+
+
+def test_tracer_kinematics():
+    h, o0 = fresh("c1")
+    tr = h.tracers(o0.state["V_VEL"])
+    o = po.Oracle(h.cfg, o0.state, tracers=tr)
+    S = o.state
+    n = o.tr.n
+    assert n > 10
+    dt = 1e-3
+    p0 = np.array(o.tr.pos, copy=True).reshape(2, n)
+    # inside the material: the mesh displacement, a linear field, interpolated exactly
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    r, z = S["V_POS"][0], S["V_POS"][1]
+    S["V_DIS"][0] = 1e-3 * r + 2e-3 * z + 0.01
+    S["V_DIS"][1] = -3e-3 * r + 0.02
+    S["V_VEL"][0] = 5.0
+    S["V_VEL"][1] = -1.0
+    o.phase("TRACERS", dt=dt)
+    p1 = np.array(o.tr.pos, copy=True).reshape(2, n)
+    alive = np.array(o.tr.alive, copy=True) == 1
+    np.testing.assert_allclose(p1[0][alive] - p0[0][alive], 1e-3 * p0[0][alive] + 2e-3 * p0[1][alive] + 0.01,
+                               rtol=1e-10)
+    np.testing.assert_allclose(p1[1][alive] - p0[1][alive], -3e-3 * p0[0][alive] + 0.02, rtol=1e-10)
+    v1 = np.array(o.tr.vel, copy=True).reshape(2, n)
+    assert np.allclose(v1[0][alive], 5.0) and np.allclose(v1[1][alive], -1.0)
+    # open to vacuum: ballistic, x + u dt + a dt^2 / 2 and u + a dt
+    S["M_VOF"][0] = 0.5
+    S["M_VOF"][1] = 0.5
+    S["V_ACC"][0] = 2.0
+    S["V_ACC"][1] = -9.0
+    o.phase("TRACERS", dt=dt)
+    p2 = np.array(o.tr.pos, copy=True).reshape(2, n)
+    v2 = np.array(o.tr.vel, copy=True).reshape(2, n)
+    alive = np.array(o.tr.alive, copy=True) == 1
+    np.testing.assert_allclose(p2[0][alive] - p1[0][alive], 5.0 * dt + 0.5 * 2.0 * dt * dt, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(p2[1][alive] - p1[1][alive], -1.0 * dt - 0.5 * 9.0 * dt * dt, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(v2[0][alive], 5.0 + 2.0 * dt, rtol=1e-14)
+    np.testing.assert_allclose(v2[1][alive], -1.0 - 9.0 * dt, rtol=1e-14)
+
+
+def test_crater_profile_of_a_flat_surface():
+    h, o = fresh("c1")
+    S = o.state
+    o.phase("PROFILE")
+    yp = S["Y_PROFILE"][0][2:-2]
+    ez = S["E_POS"][1][:, 2]
+    first_vac = ez[ez > 0].min()                   # centre of the first vacuum row: 5 m
+    assert first_vac == 5.0
+    # the projectile (vacuum around it) sits above the surface near the axis
+    np.testing.assert_array_equal(yp[h.nx // 2:], first_vac)
+
+
+def test_ejecta_drag_and_friction():
+    over = {"ejecta.resistance": "0.5", "ejecta.friction": "0.2"}
+    h, o = fresh(**over)
+    S = o.state
+    S["Y_PROFILE"][:] = 1.0e9                      # every cell lies below the profile
+    S["E_EJT"][0] = 1.0
+    S["E_DVEL"][:] = 0.0
+    S["E_VEL"][0] = 3.0
+    S["E_VEL"][1] = -4.0
+    S["E_STE"][:] = 1.0
+    dt = 1e-3
+    o.phase("EJECTA", dt=dt)
+    gy = 1.62                                      # condition.gravity, away from the bottom rows (no body force there)
+    dv0 = -0.5 * 3.0 * dt - 0.2 * gy * dt          # drag against u, friction against sign(u)
+    dv1 = -0.5 * -4.0 * dt
+    up = (slice(5, -2), slice(2, -2))
+    np.testing.assert_allclose(S["E_DVEL"][0][up], dv0, rtol=1e-14)
+    np.testing.assert_allclose(S["E_DVEL"][1][IN], dv1, rtol=1e-14)
+    S["E_DVEL"][:] = 0.0
+    S["E_VEL"][0] = -3.0                           # friction flips with the direction of motion
+    o.phase("EJECTA", dt=dt)
+    np.testing.assert_allclose(S["E_DVEL"][0][up], 0.5 * 3.0 * dt + 0.2 * gy * dt, rtol=1e-14)
+    assert np.all(S["E_STE"][:, IN[0], IN[1]] == 0.0)
+
+
+# ---------------------------------------------------------------- stress --
+VONMISES = {"material.yshear": "[, VonMises1]", "material.damage": "[, None]", "material.ytens": "[, None]"}
+
+
+def solid(Y=1.0e14, **over):
+    """c0 granite, solid (density rho0), constant yield strength Y."""
+    o2 = dict(VONMISES)
+    o2["material.yint0"] = "[, %r]" % Y
+    o2["material.ydam0"] = "[, %r]" % Y
+    o2.update(over)
+    h, o = fresh(**o2)
+    S = o.state
+    S["M_VOF"][0] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["E_DEN"][0] = h.cfg.mat[1].den0
+    S["E_DAM"][0] = 0.0
+    S["E_TPS"][0] = 0.0
+    S["E_STE2S"][0] = 0.0
+    S["E_STA2S"][0] = 0.0
+    S["E_GRAD_VEL"][:] = 0.0
+    return h, o, S
+
+
+def j2(s):
+    return ((s[0] - s[2]) ** 2 + (s[2] - s[3]) ** 2 + (s[3] - s[0]) ** 2) / 6.0 + s[1] ** 2
+
+
+def test_stress_rotation_keeps_the_invariant():
+    """Pure spin with no elastic increment (mu = 0): the Jaumann rotation
+    turns the in-plane deviator and leaves J2 unchanged."""
+    h, o, S = solid()
+    S["E_MU"][0] = 0.0
+    st = np.array([3.0e6, 1.2e6, -1.0e6, -2.0e6])
+    for i in range(4):
+        S["E_STE"][i] = st[i]
+    S["E_GRAD_VEL"][1], S["E_GRAD_VEL"][2] = -40.0, 40.0          # dv_r/dz = -dv_z/dr: rotation only
+    o.phase("STRESS", dt=1e-3)
+    new = S["E_STE"][:, IN[0], IN[1]]
+    np.testing.assert_allclose(j2(new), j2(st), rtol=1e-12)
+    assert np.all(new[3] == st[3])                                  # the hoop component does not turn
+    np.testing.assert_allclose(new[0] + new[2], st[0] + st[2], rtol=1e-14)
+    assert not np.allclose(new[1], st[1])                           # it did turn
+    np.testing.assert_allclose(S["E_STE2S"][0][IN], np.sqrt(j2(st)), rtol=1e-12)
+
+
+def test_stress_hooke_deviator():
+    """Elastic increment from zero stress, no spin: 2 mu dt times the
+    deviatoric strain rate (uniaxial in z, then simple shear)."""
+    h, o, S = solid()
+    mu, dt, ez = 3.0e10, 1e-4, 2.0
+    S["E_MU"][0] = mu
+    S["E_STE"][:] = 0.0
+    S["E_GRAD_VEL"][3] = ez
+    o.phase("STRESS", dt=dt)
+    st = S["E_STE"][:, IN[0], IN[1]]
+    np.testing.assert_allclose(st[2], 4.0 / 3.0 * mu * ez * dt, rtol=1e-14)
+    np.testing.assert_allclose(st[0], -2.0 / 3.0 * mu * ez * dt, rtol=1e-14)
+    np.testing.assert_allclose(st[3], -2.0 / 3.0 * mu * ez * dt, rtol=1e-14)
+    assert np.all(st[1] == 0.0)
+    S["E_STE"][:] = 0.0
+    S["E_GRAD_VEL"][:] = 0.0
+    S["E_GRAD_VEL"][1] = S["E_GRAD_VEL"][2] = 0.5                   # engineering shear rate 1
+    o.phase("STRESS", dt=dt)
+    st = S["E_STE"][:, IN[0], IN[1]]
+    np.testing.assert_allclose(st[1], mu * 1.0 * dt, rtol=1e-14)
+    assert np.all(st[0] == 0.0) and np.all(st[2] == 0.0)
+
+
+def test_stress_radial_return_to_the_yield_surface():
+    Y = 5.0e6
+    h, o, S = solid(Y=Y)
+    S["E_MU"][0] = 3.0e10
+    st = np.array([3.0e7, 1.0e7, -1.0e7, -2.0e7])                   # far outside the surface
+    for i in range(4):
+        S["E_STE"][i] = st[i]
+    o.phase("STRESS", dt=1e-6)
+    new = S["E_STE"][:, IN[0], IN[1]]
+    np.testing.assert_allclose(np.sqrt(j2(new)), Y, rtol=1e-12)
+    np.testing.assert_allclose(S["E_STE2S"][0][IN], Y, rtol=1e-14)
+    np.testing.assert_allclose(S["E_STRENGTH"][0][IN], Y, rtol=1e-14)
+    ratio = new / st[:, None, None]
+    np.testing.assert_allclose(ratio, np.broadcast_to(ratio[0], ratio.shape), rtol=1e-9)   # scaled, not turned
+
+
+# --------------------------------------------------------------- mixture --
+def test_mixture_closure():
+    h, o = fresh("c1")
+    S = o.state
+    m1, m2 = h.cfg.mat[1], h.cfg.mat[2]
+    d1, e1, d2, e2 = 1900.0, 1900.0 * 2.0e5, 2900.0, 2900.0 * 5.0e5
+    ref1 = np.zeros(3)
+    ref2 = np.zeros(3)
+    L = po.olib()
+    for mat, d, e, ref in ((m1, d1, e1, ref1), (m2, d2, e2, ref2)):
+        t, p, c = C.c_double(), C.c_double(), C.c_double()
+        L.oracle_calculate_state_correct(C.byref(mat), d, e, -1.0e30, C.byref(t), C.byref(p), C.byref(c), 1.0)
+        ref[:] = (t.value, p.value, c.value)
+    S["E_DAM"][0] = 0.0
+    S["M_PTY"][:] = 1.0
+    S["M_DEN"][1], S["M_ENG"][1], S["M_DEN"][2], S["M_ENG"][2] = d1, e1, d2, e2
+    # one material: its own state
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    o.update_state()
+    assert np.all(S["E_TEM"][0][IN] == ref1[0]) and np.all(S["E_DEN"][0][IN] == d1)
+    # two materials: volume averages of p, T, rho; harmonic average of c
+    S["M_VOF"][1], S["M_VOF"][2] = 0.25, 0.75
+    o.update_state()
+    np.testing.assert_allclose(S["E_TEM"][0][IN], 0.25 * ref1[0] + 0.75 * ref2[0], rtol=1e-14)
+    np.testing.assert_allclose(S["E_DEN"][0][IN], 0.25 * d1 + 0.75 * d2, rtol=1e-15)
+    assert ref1[1] > 0 and ref2[1] > 0
+    np.testing.assert_allclose(S["E_PRE"][0][IN], 0.25 * ref1[1] + 0.75 * ref2[1], rtol=1e-14)
+    np.testing.assert_allclose(S["E_CSD"][0][IN], 1.0 / (0.25 / ref1[2] + 0.75 / ref2[2]), rtol=1e-14)
+    # open to vacuum (no partial pressure): no pressure, unit sound speed
+    S["M_VOF"][0], S["M_VOF"][1], S["M_VOF"][2] = 0.1, 0.2, 0.7
+    o.update_state()
+    assert np.all(S["E_PRE"][0][IN] == 0.0) and np.all(S["E_CSD"][0][IN] == 1.0) and np.all(S["E_MU"][0][IN] == 0.0)
+
+
+# --------------------------------------------------------------- damping --
+def test_pml_velocity_relaxation():
+    """pml2: in a damping cell the velocity increment relaxes the velocity at
+    the rate sigma_x + sigma_y (du/dt = -sigma u)."""
+    h, o = fresh(**{"numerical.damping": "pml2"})
+    S = o.state
+    S["M_VOF"][0] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["E_DVEL"][:] = 0.0
+    S["E_VEL"][0], S["E_VEL"][1] = 2.0, -3.0
+    h.cfg.damp_time = 1.0e9
+    o.phase("DAMP", dt=1e-3, t=0.0)
+    sig = S["E_SIGMA"][0] + S["E_SIGMA"][1]
+    lay = (sig > 0)[IN]
+    assert lay.sum() > 20
+    np.testing.assert_allclose(S["E_DVEL"][0][IN][lay], -sig[IN][lay] * 2.0 * 1e-3, rtol=1e-14)
+    np.testing.assert_allclose(S["E_DVEL"][1][IN][lay], sig[IN][lay] * 3.0 * 1e-3, rtol=1e-14)
+    assert np.all(S["E_DVEL"][:, IN[0], IN[1]][:, ~lay] == 0.0)
+
+
+# ---------------------------------------------------------------- ghosts --
+def test_axis_reflection_and_outflow_ghosts():
+    """On the axis (mirror boundary) the ghost columns reflect the first two
+    physical columns, with the radial velocity increment reversed; on an
+    outflow side they repeat them as the same reflection without a sign."""
+    h, o = fresh("c1", **{"boundary.right": "outflow"})
+    S = o.state
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal(S["E_DVEL"].shape)
+    S["E_DVEL"][:] = f
+    o.ghost_fill("E_DVEL", 2)                       # velocity increment
+    rows = slice(2, -2)
+    np.testing.assert_array_equal(S["E_DVEL"][0][rows, 1], -S["E_DVEL"][0][rows, 2])
+    np.testing.assert_array_equal(S["E_DVEL"][0][rows, 0], -S["E_DVEL"][0][rows, 3])
+    np.testing.assert_array_equal(S["E_DVEL"][1][rows, 1], S["E_DVEL"][1][rows, 2])
+    np.testing.assert_array_equal(S["E_DVEL"][1][rows, 0], S["E_DVEL"][1][rows, 3])
+    g = rng.standard_normal(S["E_PRE"].shape)
+    S["E_PRE"][:] = g
+    o.ghost_fill("E_PRE", 0)                        # copied on every side
+    np.testing.assert_array_equal(S["E_PRE"][0][rows, -2], S["E_PRE"][0][rows, -3])
+    np.testing.assert_array_equal(S["E_PRE"][0][rows, -1], S["E_PRE"][0][rows, -4])
+    np.testing.assert_array_equal(S["E_PRE"][0][rows, 1], S["E_PRE"][0][rows, 2])
+    # the density's top side is cleaned, the first physical row included
+    S["E_DEN"][:] = 1.0
+    o.ghost_fill("E_DEN", 5)
+    assert np.all(S["E_DEN"][0][-3:, 2:-2] == 0.0) and np.all(S["E_DEN"][0][-4, 2:-2] == 1.0)
+
+
+def test_vertex_restriction_boundary_conditions():
+    """Axis and free-slip sides: no normal velocity on the boundary vertex
+    (its other component taken from the inward neighbour, which the ghost
+    vertex copies too); fixed bottom: the boundary vertex is at rest."""
+    h, o = fresh("c1")                              # axis left, free-slip right, fixed bottom
+    S = o.state
+    v = np.random.default_rng(5).standard_normal(S["V_VEL"].shape)
+    S["V_VEL"][:] = v
+    o.restrict("V_VEL")
+    V = S["V_VEL"]
+    rows = slice(3, -3)
+    assert np.all(V[0][rows, 2] == 0.0) and np.array_equal(V[1][rows, 2], v[1][rows, 3])     # axis
+    assert np.array_equal(V[:, rows, 1], v[:, rows, 3])
+    assert np.all(V[0][rows, -3] == 0.0) and np.array_equal(V[1][rows, -3], v[1][rows, -4])  # right
+    cols = slice(3, -3)
+    assert np.all(V[:, 2, cols] == 0.0)                                                       # bottom
+    assert np.array_equal(V[:, 10, 10], v[:, 10, 10])                                          # interior untouched
