@@ -27,14 +27,16 @@
  * calculate_v_dis, update_energy (pressure work, deviatoric power),
  * block_vibration, update_e_stress (Jaumann rotation keeps J2, Hooke's law
  * for the deviator, radial return), update_state (mixture averages, vacuum
- * rule), damp_on_cell pml2 (velocity relaxation), ejecta_flow (drag,
- * friction), fl_update_node, trace_profile, the axis/outflow/clean ghost
- * fills and the vertex restrictions.  tests/test_oracle_lubrication.py pins
- * update_strength_condition and the lubrication cap.
+ * rule), damp_on_cell pml2 (velocity and auxiliary relaxation), ejecta_flow
+ * (drag, friction), fl_update_node, trace_profile, check_v_vel, the flux
+ * and remap composition (a flat interface moved in z and in r, a uniform
+ * state), the axis/outflow/clean ghost fills and the vertex restrictions.
+ * tests/test_oracle_lubrication.py pins update_strength_condition and the
+ * lubrication cap.
  *
- * Parity unpinned: check_v_vel, the pml/sbc damping variants and the pml
- * auxiliaries (e_alpha, v_alpha, e_beta), and the composition of
- * calculate_flux_mixed with advect_flux_default beyond the geometry
+ * Parity unpinned: the pml (first form) and sbc damping variants, v_alpha
+ * and e_beta (pml2/pml3 auxiliaries with their stress terms), the mixed-cell
+ * interface reconstruction of calculate_flux_mixed beyond the geometry
  * helpers above; they are checked as a whole (rest state, lithostatic
  * equilibrium, mass conservation in a closed box, a recorded run).
  */
@@ -94,7 +96,7 @@ void   oracle_ghost_fill(oracle_state *s, const salec_config *cfg, int field, in
 enum {
     ORACLE_PH_V_VOF = 0, ORACLE_PH_V_VEL_INC, ORACLE_PH_E_VEL, ORACLE_PH_CHECK_V_VEL, ORACLE_PH_GRADVEL,
     ORACLE_PH_DT, ORACLE_PH_VIBRATION, ORACLE_PH_STRESS, ORACLE_PH_ACC, ORACLE_PH_V_DIS, ORACLE_PH_ENERGY,
-    ORACLE_PH_TRACERS, ORACLE_PH_PROFILE, ORACLE_PH_EJECTA, ORACLE_PH_DAMP
+    ORACLE_PH_TRACERS, ORACLE_PH_PROFILE, ORACLE_PH_EJECTA, ORACLE_PH_DAMP, ORACLE_PH_FLUX, ORACLE_PH_ADVECT
 };
 int    oracle_phase(oracle_state *s, const salec_config *cfg, int phase);
 void   oracle_restrict(oracle_state *s, const salec_config *cfg, int field);

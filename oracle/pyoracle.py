@@ -109,7 +109,7 @@ class Oracle:
         self.L.oracle_step(C.byref(self.s), C.byref(self.cfg))
 
     PHASES = ("V_VOF", "V_VEL_INC", "E_VEL", "CHECK_V_VEL", "GRADVEL", "DT", "VIBRATION", "STRESS", "ACC", "V_DIS",
-              "ENERGY", "TRACERS", "PROFILE", "EJECTA", "DAMP")
+              "ENERGY", "TRACERS", "PROFILE", "EJECTA", "DAMP", "FLUX", "ADVECT")
 
     def phase(self, name, dt=None, t=None):
         """Run one function of the step (oracle_phase) on the held state."""

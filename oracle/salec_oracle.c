@@ -1923,6 +1923,12 @@ int oracle_phase(oracle_state *s, const salec_config *cfg, int phase)
         else if (cfg->damp_type == SALEC_DAMP_PML) damp_on_cell_pml(s, cfg, g, dt, t);
         else if (cfg->damp_type == SALEC_DAMP_SBC) damp_on_cell_sbc(s, cfg, g, t);
         break;
+    case ORACLE_PH_FLUX:
+        clean_flux(s, g);
+        calculate_flux_mixed(s, g, 1, porous(cfg));
+        calculate_flux_mixed(s, g, 0, porous(cfg));
+        break;
+    case ORACLE_PH_ADVECT: advect_flux_default(s, cfg, g, cfg->damp_type == SALEC_DAMP_PML3, dt); break;
     default: return -1;
     }
     return 0;

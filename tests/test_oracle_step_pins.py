@@ -532,3 +532,126 @@ def test_vertex_restriction_boundary_conditions():
     cols = slice(3, -3)
     assert np.all(V[:, 2, cols] == 0.0)                                                       # bottom
     assert np.array_equal(V[:, 10, 10], v[:, 10, 10])                                          # interior untouched
+
+
+# ----------------------------------------------------------------- remap --
+def half_space(axis, sign=1.0):
+    """c1 mesh, regolith where the cell centre has coordinate < 0 along axis
+    (z, or r - R for the radial case), vacuum elsewhere; geometry kept."""
+    h, o = fresh("c1", **{"numerical.damping": "off"})
+    S = o.state
+    ep = S["E_POS"]
+    x = ep[1] if axis == "z" else ep[0] - 80.0
+    full = x < 0
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][0][~full] = 1.0
+    S["M_VOF"][1][full] = 1.0
+    S["M_DEN"][:] = 0.0
+    S["M_DEN"][1][full] = 1800.0
+    S["M_ENG"][:] = 0.0
+    S["M_ENG"][1][full] = 1800.0 * 1.0e5
+    S["E_DEN"][0] = np.where(full, 1800.0, 0.0)
+    o.phase("V_VOF")
+    return h, o, S, x
+
+
+def test_remap_transports_a_flat_interface():
+    """A flat interface displaced normally by d: the first empty row takes
+    the material that crossed its face over its own volume, d / dz; in r the
+    material that crossed a face at R came from [R - d, R], d (R - d/2) dz
+    per radian (Pappus), over the cell's r_c dr dz."""
+    dt, u = 1.0e-3, 100.0
+    h, o, S, z = half_space("z")
+    S["V_DIS"][0] = 0.0
+    S["V_DIS"][1] = u * dt
+    o.phase("FLUX", dt=dt)
+    o.phase("ADVECT", dt=dt)
+    j0 = int(np.argmax(z[:, 5] > 0))
+    np.testing.assert_allclose(S["M_VOF"][1][j0, 2:-2], u * dt / 10.0, rtol=1e-12)
+    np.testing.assert_allclose(S["M_VOF"][1][j0 - 1, 2:-2], 1.0, rtol=0, atol=1e-14)
+    assert np.all(S["M_VOF"][1][j0 + 1, 2:-2] == 0.0)
+    np.testing.assert_allclose(S["M_DEN"][1][j0, 2:-2], 1800.0, rtol=1e-14)
+    np.testing.assert_allclose(S["M_ENG"][1][j0, 2:-2], 1800.0 * 1.0e5, rtol=1e-14)
+    # radial: the interface at r = 80 m moves out by d
+    h, o, S, x = half_space("r")
+    d = u * dt
+    S["V_DIS"][0] = d
+    S["V_DIS"][1] = 0.0
+    o.phase("FLUX", dt=dt)
+    o.phase("ADVECT", dt=dt)
+    k0 = int(np.argmax(x[5, :] > 0))
+    R, dx = 80.0, 10.0
+    expect = d * (R - d / 2) / ((R + dx / 2) * dx)
+    np.testing.assert_allclose(S["M_VOF"][1][4:-4, k0], expect, rtol=1e-12)
+
+
+def test_remap_of_a_uniform_state():
+    """Uniform material displaced by (d_r, d_z): the axial part moves mass
+    through every horizontal face alike and changes nothing; the radial part
+    moves d (r - d/2) dz through a vertical face at r (Pappus), so a cell
+    keeps rho (1 - d_r / r_c)."""
+    dt = 1.0e-3
+    h, o, S, z = half_space("z")
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["M_DEN"][1] = 1800.0
+    S["M_ENG"][1] = 1800.0 * 1.0e5
+    S["E_DEN"][0] = 1800.0
+    o.phase("V_VOF")
+    S["V_DIS"][0] = 0.03
+    S["V_DIS"][1] = -0.05
+    o.phase("FLUX", dt=dt)
+    o.phase("ADVECT", dt=dt)
+    inner = (slice(4, -4), slice(4, -4))
+    np.testing.assert_allclose(S["M_VOF"][1][inner], 1.0, rtol=0, atol=1e-14)
+    rc = S["E_POS"][0][inner]
+    np.testing.assert_allclose(S["M_DEN"][1][inner], 1800.0 * (1.0 - 0.03 / rc), rtol=1e-12)
+    np.testing.assert_allclose(S["M_ENG"][1][inner], 1.8e8 * (1.0 - 0.03 / rc), rtol=1e-12)
+
+
+def test_void_vertex_velocity():
+    """check_v_vel: a vertex every one of whose cells marks that corner void
+    takes the density-weighted mean of the cell velocities (vacuum cells
+    excluded)."""
+    h, o = fresh()
+    S = o.state
+    S["E_WPT"][:] = 1.0
+    S["E_DEN"][0] = 1000.0
+    rng = np.random.default_rng(6)
+    S["E_VEL"][:] = rng.standard_normal(S["E_VEL"].shape)
+    ev = S["E_VEL"].copy()
+    o.phase("CHECK_V_VEL")
+    mean = 0.25 * (ev[:, :-1, :-1] + ev[:, :-1, 1:] + ev[:, 1:, 1:] + ev[:, 1:, :-1])
+    np.testing.assert_allclose(S["V_VEL"][:, 2:-2, 2:-2], mean[:, 1:-1, 1:-1], rtol=1e-13, atol=1e-15)
+    # one cell of each vertex at twice the density, one empty
+    S["E_DEN"][0] = np.where(np.arange(S["E_DEN"].shape[2]) % 2 == 0, 2000.0, 1000.0)[None, :]
+    o.phase("CHECK_V_VEL")
+    d = S["E_DEN"][0]
+    w = [d[:-1, :-1], d[:-1, 1:], d[1:, 1:], d[1:, :-1]]
+    e = [ev[:, :-1, :-1], ev[:, :-1, 1:], ev[:, 1:, 1:], ev[:, 1:, :-1]]
+    wm = sum(wi * ei for wi, ei in zip(w, e)) / sum(w)
+    np.testing.assert_allclose(S["V_VEL"][:, 2:-2, 2:-2], wm[:, 1:-1, 1:-1], rtol=1e-13, atol=1e-15)
+
+
+def test_pml_auxiliary_relaxation():
+    """pml2 with no velocity gradient: the cell auxiliaries relax at the rate
+    alpha_x + alpha_y (e_alpha_0 exactly; e_alpha_1 with the stress term
+    gone)."""
+    h, o = fresh(**{"numerical.damping": "pml2", "numerical.LambdaX": "7.0", "numerical.LambdaY": "3.0"})
+    S = o.state
+    S["M_VOF"][0] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["E_GRAD_VEL"][:] = 0.0
+    S["E_STE"][:] = 0.0
+    S["E_ALPHA"][0] = 2.0
+    S["E_ALPHA"][1] = -5.0
+    h.cfg.damp_time = 1.0e9
+    dt = 1e-3
+    o.phase("DAMP", dt=dt, t=0.0)
+    rate = h.cfg.alpha_damp_x + h.cfg.alpha_damp_y
+    assert rate > 0
+    lay = ((S["E_SIGMA"][0] + S["E_SIGMA"][1]) > 0)[IN]
+    a0 = S["E_ALPHA"][0][IN][lay]
+    a1 = S["E_ALPHA"][1][IN][lay]
+    np.testing.assert_allclose(a0, 2.0 * (1.0 - rate * dt), rtol=1e-14)
+    np.testing.assert_allclose(a1, -5.0 * (1.0 - rate * dt), rtol=1e-14)
