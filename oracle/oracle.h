@@ -29,15 +29,15 @@
  * for the deviator, radial return), update_state (mixture averages, vacuum
  * rule), damp_on_cell pml2 (velocity and auxiliary relaxation), ejecta_flow
  * (drag, friction), fl_update_node, trace_profile, check_v_vel, the flux
- * and remap composition (a flat interface moved in z and in r, a uniform
- * state), the axis/outflow/clean ghost fills and the vertex restrictions.
+ * and remap composition (a flat interface moved in z and in r, a row of
+ * mixed cells, a uniform state), the axis/outflow/clean ghost fills and the vertex restrictions.
  * tests/test_oracle_lubrication.py pins update_strength_condition and the
  * lubrication cap.
  *
  * Parity unpinned: the pml (first form) and sbc damping variants, v_alpha
- * and e_beta (pml2/pml3 auxiliaries with their stress terms), the mixed-cell
- * interface reconstruction of calculate_flux_mixed beyond the geometry
- * helpers above; they are checked as a whole (rest state, lithostatic
+ * and e_beta (pml2/pml3 auxiliaries with their stress terms), the interface
+ * reconstruction of calculate_flux_mixed for interfaces that are not flat
+ * and mesh-aligned; they are checked as a whole (rest state, lithostatic
  * equilibrium, mass conservation in a closed box, a recorded run).
  */
 #ifndef SALEC_ORACLE_H

@@ -655,3 +655,27 @@ def test_pml_auxiliary_relaxation():
     a1 = S["E_ALPHA"][1][IN][lay]
     np.testing.assert_allclose(a0, 2.0 * (1.0 - rate * dt), rtol=1e-14)
     np.testing.assert_allclose(a1, -5.0 * (1.0 - rate * dt), rtol=1e-14)
+
+
+@pytest.mark.parametrize("f", [0.3, 0.5, 0.8])
+def test_remap_of_a_mixed_row(f):
+    """A horizontal interface inside a row of mixed cells (material below,
+    volume fraction f) moved up by d < (1 - f) dz: the reconstructed
+    interface keeps the top face empty, so the row only gains the material
+    entering from below, f + d/dz, and the row above stays empty."""
+    dt, d = 1.0e-3, 1.0
+    h, o, S, z = half_space("z")
+    j0 = int(np.argmax(z[:, 5] > 0))
+    S["M_VOF"][1][j0] = f
+    S["M_VOF"][0][j0] = 1.0 - f
+    S["M_DEN"][1][j0] = 1800.0
+    S["M_ENG"][1][j0] = 1800.0 * 1.0e5
+    S["E_DEN"][0][j0] = 1800.0 * f
+    o.phase("V_VOF")
+    S["V_DIS"][0] = 0.0
+    S["V_DIS"][1] = d
+    o.phase("FLUX", dt=dt)
+    o.phase("ADVECT", dt=dt)
+    np.testing.assert_allclose(S["M_VOF"][1][j0, 3:-3], f + d / 10.0, rtol=1e-12)
+    assert np.all(S["M_VOF"][1][j0 + 1, 3:-3] == 0.0)
+    np.testing.assert_allclose(S["M_DEN"][1][j0, 3:-3], 1800.0, rtol=1e-12)
