@@ -34,8 +34,10 @@
  * tests/test_oracle_lubrication.py pins update_strength_condition and the
  * lubrication cap.
  *
- * Parity unpinned: the pml (first form) and sbc damping variants, v_alpha
- * and e_beta (pml2/pml3 auxiliaries with their stress terms), the interface
+ * The pml (first form) relaxation and the radial sponge are pinned too.
+ *
+ * Parity unpinned: v_alpha and e_beta (pml2/pml3 auxiliaries with their
+ * stress terms), the axial sponge update, the interface
  * reconstruction of calculate_flux_mixed for interfaces that are not flat
  * and mesh-aligned; they are checked as a whole (rest state, lithostatic
  * equilibrium, mass conservation in a closed box, a recorded run).

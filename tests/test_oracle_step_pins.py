@@ -679,3 +679,52 @@ def test_remap_of_a_mixed_row(f):
     np.testing.assert_allclose(S["M_VOF"][1][j0, 3:-3], f + d / 10.0, rtol=1e-12)
     assert np.all(S["M_VOF"][1][j0 + 1, 3:-3] == 0.0)
     np.testing.assert_allclose(S["M_DEN"][1][j0, 3:-3], 1800.0, rtol=1e-12)
+
+
+def test_pml_first_form_relaxes_each_direction_on_its_own():
+    """pml: sigma_y relaxes the axial velocity and alpha_y the axial
+    auxiliary, sigma_x and alpha_x the radial ones (no cross terms)."""
+    h, o = fresh(**{"numerical.damping": "pml", "numerical.LambdaX": "7.0", "numerical.LambdaY": "3.0"})
+    S = o.state
+    S["M_VOF"][0] = 0.0
+    S["M_VOF"][1] = 0.5
+    S["M_VOF"][2 if h.nmat > 2 else 1] = 0.5 if h.nmat > 2 else 1.0
+    S["E_GRAD_VEL"][:] = 0.0
+    S["E_DVEL"][:] = 0.0
+    S["E_VEL"][0], S["E_VEL"][1] = 2.0, -3.0
+    S["E_ALPHA"][0], S["E_ALPHA"][1] = 1.5, -4.0
+    h.cfg.damp_time = 1.0e9
+    dt = 1e-3
+    o.phase("DAMP", dt=dt, t=0.0)
+    sx, sy = S["E_SIGMA"][0][IN], S["E_SIGMA"][1][IN]
+    ax, ay = h.cfg.alpha_damp_x, h.cfg.alpha_damp_y
+    dv, al = S["E_DVEL"][:, IN[0], IN[1]], S["E_ALPHA"][:, IN[0], IN[1]]
+    assert (sx > 0).sum() > 10 and (sy > 0).sum() > 10
+    np.testing.assert_allclose(dv[1][sy > 0], 3.0 * sy[sy > 0] * dt, rtol=1e-14)
+    np.testing.assert_allclose(dv[0][sx > 0], -2.0 * sx[sx > 0] * dt, rtol=1e-14)
+    assert np.all(dv[1][sy == 0] == 0.0) and np.all(dv[0][sx == 0] == 0.0)
+    np.testing.assert_allclose(al[1][sy > 0], -4.0 * (1.0 - ay * dt), rtol=1e-14)
+    np.testing.assert_allclose(al[0][sx > 0], 1.5 * (1.0 - ax * dt), rtol=1e-14)
+
+
+def test_sponge_layer_relaxes_toward_the_reference_state():
+    """sbc: in a radial sponge cell (factor s_x < 1) the density and energy
+    are pulled to the reference state: rho_ref + (rho - rho_ref) s_x and
+    E_ref + (E - E_ref) s_x^2."""
+    h, o = fresh(**{"numerical.damping": "sbc", "numerical.LambdaX": "0.3", "numerical.LambdaY": "0.3"})
+    S = o.state
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["E_DAMPREF"][0] = 2000.0
+    S["E_DAMPREF"][1] = 3.0e8
+    S["M_DEN"][1] = 2100.0
+    S["M_ENG"][1] = 3.5e8
+    h.cfg.damp_time = 1.0e9
+    o.phase("DAMP", dt=1e-3, t=0.0)
+    sx, sy = S["E_SIGMA"][0][IN], S["E_SIGMA"][1][IN]
+    only_x = (sx < 1.0) & (sy >= 1.0)
+    assert only_x.sum() > 5
+    np.testing.assert_allclose(S["M_DEN"][1][IN][only_x], 2000.0 + 100.0 * sx[only_x], rtol=1e-15)
+    np.testing.assert_allclose(S["M_ENG"][1][IN][only_x], 3.0e8 + 0.5e8 * sx[only_x] ** 2, rtol=1e-15)
+    untouched = (sx >= 1.0) & (sy >= 1.0)
+    assert np.all(S["M_DEN"][1][IN][untouched] == 2100.0)
