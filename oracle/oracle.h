@@ -36,8 +36,9 @@
  *
  * The pml (first form) relaxation and the radial sponge are pinned too.
  *
- * Parity unpinned: v_alpha and e_beta (pml2/pml3 auxiliaries with their
- * stress terms), the axial sponge update, the interface
+ * Parity unpinned: the stress terms of the pml2 auxiliaries (e_alpha,
+ * v_alpha; their relaxation is pinned), e_beta (pml3), the axial sponge
+ * update, the interface
  * reconstruction of calculate_flux_mixed for interfaces that are not flat
  * and mesh-aligned; they are checked as a whole (rest state, lithostatic
  * equilibrium, mass conservation in a closed box, a recorded run).

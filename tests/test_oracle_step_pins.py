@@ -728,3 +728,32 @@ def test_sponge_layer_relaxes_toward_the_reference_state():
     np.testing.assert_allclose(S["M_ENG"][1][IN][only_x], 3.0e8 + 0.5e8 * sx[only_x] ** 2, rtol=1e-15)
     untouched = (sx >= 1.0) & (sy >= 1.0)
     assert np.all(S["M_DEN"][1][IN][untouched] == 2100.0)
+
+
+def test_vertex_pml_auxiliary_relaxation():
+    """pml2 vertices in the layer: with no stress beyond the reference
+    pressure, v_alpha relaxes at alpha_x (radial) and alpha_y (axial) and
+    enters the acceleration as -v_alpha / m."""
+    h, o = fresh(**{"numerical.damping": "pml2", "numerical.LambdaX": "7.0", "numerical.LambdaY": "3.0",
+                    "numerical.ANC": "0.0"})
+    S = o.state
+    S["E_STE"][:] = 0.0
+    S["E_Q"][:] = 0.0
+    S["E_PRE"][0] = 5.0e7
+    S["E_DAMPREF"][2] = 5.0e7
+    S["E_DEN"][0] = 2000.0
+    S["V_MASS"][:] = 0.0
+    S["V_ALPHA"][0], S["V_ALPHA"][1] = 4.0e8, -1.0e8     # well above the acceleration floor
+    dt = 1e-3
+    o.phase("ACC", dt=dt)
+    es = S["E_SIGMA"]
+    vs = 0.25 * (es[:, :-1, :-1] + es[:, :-1, 1:] + es[:, 1:, 1:] + es[:, 1:, :-1])   # vertex sigma (inner vertices)
+    lay = ((vs[0] > 0) | (vs[1] > 0))[2:-2, 2:-2]
+    al = S["V_ALPHA"][:, 3:-3, 3:-3]
+    assert lay.sum() > 20
+    np.testing.assert_allclose(al[0][lay], 4.0e8 * (1.0 - h.cfg.alpha_damp_x * dt), rtol=1e-14)
+    np.testing.assert_allclose(al[1][lay], -1.0e8 * (1.0 - h.cfg.alpha_damp_y * dt), rtol=1e-14)
+    vm = 2000.0 * S["V_VOL"][1:5].sum(axis=0)[3:-3, 3:-3]
+    acc, bf = S["V_ACC"][:, 3:-3, 3:-3], S["V_BF"][:, 3:-3, 3:-3]
+    np.testing.assert_allclose(acc[0][lay], bf[0][lay] - al[0][lay] / vm[lay], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(acc[1][lay], bf[1][lay] - al[1][lay] / vm[lay], rtol=1e-9, atol=1e-12)
