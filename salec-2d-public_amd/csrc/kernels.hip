@@ -298,11 +298,59 @@ __global__ void k_ghost_tan(TanSlots sl, const GhostOp *ops, int nops, double *s
             else stage[q] = t;
         }
 }
-void launch_ghost_tan(hipStream_t s, const TanSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase)
+/* one launch, as k_ghost_fused: workgroup 0 takes the hazardous ops
+ * through LDS, the others the independent ops */
+__global__ void __launch_bounds__(256) k_ghost_tan_fused(TanSlots sl, const GhostOp *ops, int nops, int nconf)
+{
+    if (blockIdx.x == 0) {
+        __shared__ double st[GHOST_LDS];
+        for (int i = threadIdx.x; i < nconf; i += blockDim.x) {
+            GhostOp o = ops[i];
+            const double *pb = sl.pbase[o.slot];
+            const double *tb = sl.tbase[o.slot];
+            if (!tb) continue;
+            long pl = sl.plane[o.slot], ts = sl.tst[o.slot];
+            int nc = sl.ncomp[o.slot];
+            for (int k = 0; k < NTAN; ++k)
+                for (int c = 0; c < nc; ++c) {
+                    int m = c == 0 ? o.m0 : (c == 1 ? o.m1 : o.mr);
+                    st[(k * nc + c) * nconf + i] = gmode_tan(tb[k * ts + c * pl + o.src], pb[c * pl + o.src], m);
+                }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nconf; i += blockDim.x) {
+            GhostOp o = ops[i];
+            double *tb = sl.tbase[o.slot];
+            if (!tb) continue;
+            long pl = sl.plane[o.slot], ts = sl.tst[o.slot];
+            int nc = sl.ncomp[o.slot];
+            for (int k = 0; k < NTAN; ++k)
+                for (int c = 0; c < nc; ++c) tb[k * ts + c * pl + o.dst] = st[(k * nc + c) * nconf + i];
+        }
+        return;
+    }
+    int i = nconf + (blockIdx.x - 1) * blockDim.x + threadIdx.x;
+    if (i >= nops) return;
+    GhostOp o = ops[i];
+    const double *pb = sl.pbase[o.slot];
+    double *tb = sl.tbase[o.slot];
+    if (!tb) return;
+    long pl = sl.plane[o.slot], ts = sl.tst[o.slot];
+    int nc = sl.ncomp[o.slot];
+    for (int k = 0; k < NTAN; ++k)
+        for (int c = 0; c < nc; ++c) {
+            int m = c == 0 ? o.m0 : (c == 1 ? o.m1 : o.mr);
+            tb[k * ts + c * pl + o.dst] = gmode_tan(tb[k * ts + c * pl + o.src], pb[c * pl + o.src], m);
+        }
+}
+void launch_ghost_tan(hipStream_t s, const TanSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase,
+                      int maxc, int nconf)
 {
     if (nops <= 0) return;
     if (!two_phase) {
         k_ghost_tan<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, 0);
+    } else if ((long)nconf * NTAN * maxc <= GHOST_LDS) {
+        k_ghost_tan_fused<<<1 + nblk(nops - nconf, 256), 256, 0, s>>>(sl, ops, nops, nconf);
     } else {
         k_ghost_tan<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, 1);
         k_ghost_tan<<<nblk(nops, 256), 256, 0, s>>>(sl, ops, nops, stage, 2);

@@ -133,7 +133,8 @@ void launch_dampref(hipStream_t s, const Grid &g, const Fields &F, double *part,
 int  dampref_blocks(const Grid &g);
 void launch_init_v_field(hipStream_t s, const Grid &g, const Fields &F);
 /* sensitivity runs: the step kernels on Dual values and tangent planes */
-void launch_ghost_tan(hipStream_t s, const TanSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase);
+void launch_ghost_tan(hipStream_t s, const TanSlots &sl, const GhostOp *ops, int nops, double *stage, int two_phase,
+                      int maxc, int nconf);
 void launch_vertex_vof_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P);
 void launch_update_e_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T);
 void launch_check_v_vel(hipStream_t s, const Grid &g, const Fields &F, const TFields &T);
