@@ -11,7 +11,7 @@ import pyoracle as po
 
 TOLVOF = 1.0e-6
 TOLRHO = 1.0e-4
-# per-step relative L-infinity bound for FP64 fields
+# per-step relative L-infinity bound for FP64 fields (the parity bar)
 FIELD_RTOL = 1.0e-11
 
 COMPARED = ps.PERSISTENT + ["E_VEL", "E_GRAD_VEL", "E_DIV_VEL", "E_Q", "E_STA2S", "E_STRENGTH", "V_VOF",

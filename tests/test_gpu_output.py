@@ -1,6 +1,6 @@
 """End-to-end output of the C driver (salec2d) on the GPU.
 
-The driver runs a small strength deck with output every 5 steps (and at the
+The salec2d program runs a small strength deck with output every 5 steps (and at the
 first step, as the reference does) and writes the state and tracer files at
 the mid-step output point.  The same run is replayed here step by step through
 the C ABI, stopping at the same output points (SALEC_STEP_TO_OUTPUT) and
