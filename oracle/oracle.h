@@ -34,14 +34,20 @@
  * tests/test_oracle_lubrication.py pins update_strength_condition and the
  * lubrication cap.
  *
- * The pml (first form) relaxation and the radial sponge are pinned too.
+ * The pml (first form) relaxation and the radial sponge are pinned too, and
+ * so are the stress and divergence terms of the pml2/pml3 auxiliaries
+ * (e_alpha in damp_on_cell_pml2, v_alpha in calculate_acc_pml), the pml3
+ * flux auxiliaries e_beta and their coupling into the remap, and the axial
+ * sponge with the order of the two sponge updates in a corner cell.
  *
- * Parity unpinned: the stress terms of the pml2 auxiliaries (e_alpha,
- * v_alpha; their relaxation is pinned), e_beta (pml3), the axial sponge
- * update, the interface
- * reconstruction of calculate_flux_mixed for interfaces that are not flat
- * and mesh-aligned; they are checked as a whole (rest state, lithostatic
- * equilibrium, mass conservation in a closed box, a recorded run).
+ * The interface reconstruction of calculate_flux_mixed is pinned for
+ * inclined interfaces too: the cut line normal to the vertex volume-fraction
+ * gradient and the face shares it gives, on square and non-square cells.
+ *
+ * Parity unpinned: how a mixed cell shares the flux that is not the primary
+ * material's among two or more other materials (algebraic_flux_ratio; its
+ * MSTACS limiter is pinned by itself); it is checked as a whole (mass
+ * conservation in a closed box, a recorded run).
  */
 #ifndef SALEC_ORACLE_H
 #define SALEC_ORACLE_H

@@ -34,7 +34,13 @@ sign, a swapped component or a wrong index fails:
 * crater profile (trace_profile, src/ejecta.c:8-63): a flat surface gives
   the centre of the first vacuum row in every column;
 * ejecta flow (src/ejecta.c:65-134): linear drag and Coulomb friction
-  increments.
+  increments;
+* damping layers (src/velocity.c, src/advect.c:304-593): the split
+  divergence and stress power of the pml2/pml3 cell auxiliaries, the split
+  stress force of the vertex auxiliary, the pml3 flux auxiliaries and their
+  coupling into the remap, both sponges;
+* remap: flat and inclined interfaces, a row of mixed cells, a uniform
+  state.
 """
 import ctypes as C
 import math
@@ -757,3 +763,335 @@ def test_vertex_pml_auxiliary_relaxation():
     acc, bf = S["V_ACC"][:, 3:-3, 3:-3], S["V_BF"][:, 3:-3, 3:-3]
     np.testing.assert_allclose(acc[0][lay], bf[0][lay] - al[0][lay] / vm[lay], rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(acc[1][lay], bf[1][lay] - al[1][lay] / vm[lay], rtol=1e-9, atol=1e-12)
+
+
+def split_power(T, G, sx, sy):
+    """Sum over velocity components i (r, z) and directions j (r, z) of
+    T_ij dv_i/dx_j, the terms of v_r weighted by sigma_y and those of v_z by
+    sigma_x, plus T_tt v_r / r weighted by sigma_x + sigma_y.  T = (T_rr,
+    T_rz, T_zz, T_tt); G = (dv_r/dr, dv_r/dz, dv_z/dr, dv_z/dz, v_r/r)."""
+    Tm = ((T[0], T[1]), (T[1], T[2]))
+    grad = ((G[0], G[1]), (G[2], G[3]))
+    w = (sy, sx)
+    p = sum(w[i] * Tm[i][j] * grad[i][j] for i in range(2) for j in range(2))
+    return p + (sx + sy) * T[3] * G[4]
+
+
+@pytest.mark.parametrize("form,pty", [("pml2", 1.0), ("pml2", 1.25), ("pml3", 1.25)])
+def test_pml_auxiliary_stress_terms(form, pty):
+    """pml2/pml3 cells full of material, auxiliaries starting at zero
+    (damp_on_cell_pml2, src/velocity.c:556-580; pml3 :658-682): e_alpha_0
+    gains the split divergence (the split power of the identity) and
+    e_alpha_1 loses the split stress power of T = S - (p + q - p_ref) I, whose
+    hoop entry carries the deviator alone, as src/velocity.c:564-567 writes
+    it.  pml2 then moves the density by -rho_ref e_alpha_0 dt and the energy
+    by -e_alpha_1 dt, both over the distension; pml3 relaxes the density
+    toward rho_ref with no auxiliary term and no distension."""
+    h, o = fresh(**{"numerical.damping": form, "numerical.LambdaX": "7.0", "numerical.LambdaY": "3.0"})
+    S = o.state
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    S["M_PTY"][1] = pty
+    G = (0.7, -1.3, 2.1, -0.4, 0.25)
+    Sd = (3.0e6, -2.0e6, 1.5e6, -4.5e6)
+    p, q, pref = 2.0e7, 1.0e6, 1.8e7
+    for i, g in enumerate(G):
+        S["E_GRAD_VEL"][i] = g
+    for i, s in enumerate(Sd):
+        S["E_STE"][i] = s
+    S["E_PRE"][0], S["E_Q"][0], S["E_DAMPREF"][2] = p, q, pref
+    S["E_ALPHA"][:] = 0.0
+    rho_ref, e_ref = 2000.0, 3.0e5
+    S["E_DAMPREF"][0], S["E_DAMPREF"][1] = rho_ref, e_ref
+    rho0 = rho_ref if form == "pml2" else 2100.0
+    S["M_DEN"][1] = rho0
+    S["M_ENG"][1] = e_ref
+    h.cfg.damp_time = 1.0e9
+    dt = 1e-3
+    o.phase("DAMP", dt=dt, t=0.0)
+    sx, sy = S["E_SIGMA"][0][IN], S["E_SIGMA"][1][IN]
+    lay = (sx > 0) | (sy > 0)
+    assert ((sx > 0) & (sy == 0)).sum() > 10 and ((sy > 0) & (sx == 0)).sum() > 10 and ((sx > 0) & (sy > 0)).sum() > 0
+    sx, sy = sx[lay], sy[lay]
+    lp = p + q - pref
+    a0 = split_power((1.0, 0.0, 1.0, 1.0), G, sx, sy) * dt
+    a1 = -split_power((Sd[0] - lp, Sd[1], Sd[2] - lp, Sd[3]), G, sx, sy) * dt
+    np.testing.assert_allclose(S["E_ALPHA"][0][IN][lay], a0, rtol=1e-13, atol=1e-18)
+    np.testing.assert_allclose(S["E_ALPHA"][1][IN][lay], a1, rtol=1e-12, atol=1e-6)
+    den, eng = S["M_DEN"][1][IN][lay], S["M_ENG"][1][IN][lay]
+    if form == "pml2":
+        np.testing.assert_allclose(den, rho_ref - rho_ref * a0 * dt / pty, rtol=1e-14)
+        np.testing.assert_allclose(eng, e_ref - a1 * dt / pty, rtol=1e-13)
+    else:
+        np.testing.assert_allclose(den, rho0 - (sx + sy) * (rho0 - rho_ref) * dt, rtol=1e-14)
+        np.testing.assert_allclose(eng, e_ref - a1 * dt, rtol=1e-13)
+    out = ~lay
+    assert np.all(S["E_ALPHA"][:, IN[0], IN[1]][:, out] == 0.0) and np.all(S["M_DEN"][1][IN][out] == rho0)
+
+
+@pytest.mark.parametrize("kind", ["normal", "shear", "hoop"])
+def test_vertex_pml_stress_terms(kind):
+    """pml2 vertices (calculate_acc_pml, src/velocity.c:158-340): the vertex
+    auxiliary integrates the stress force split by face direction, the force
+    through radial-facing faces weighted by the vertex sigma_y, through
+    axial-facing faces by sigma_x, the hoop force by sigma_x + sigma_y.  So
+    a normal stress's radial force is weighted by sigma_y and its axial one
+    by sigma_x, a shear's the other way round.  The force itself is read off
+    the same state stepped with sigma = 0, and the auxiliary enters the
+    acceleration as -v_alpha / m."""
+    h, o = fresh(**{"numerical.damping": "pml2", "numerical.LambdaX": "7.0", "numerical.LambdaY": "3.0",
+                    "numerical.ANC": "0.0"})
+    h.cfg.acceleration_min = 0.0
+    S = o.state
+    rng = np.random.default_rng({"normal": 1, "shear": 2, "hoop": 3}[kind])
+    S["E_STE"][:] = 0.0
+    S["E_Q"][:] = 0.0
+    S["E_PRE"][0] = 0.0
+    S["E_DAMPREF"][2] = 0.0
+    for c in {"normal": (0, 2), "shear": (1,), "hoop": (3,)}[kind]:
+        S["E_STE"][c] = rng.uniform(-1.0e7, 1.0e7, S["E_STE"][c].shape)
+    S["E_DEN"][0] = 2000.0
+    S["V_ALPHA"][:] = 0.0
+    dt = 1e-3
+    sig = S["E_SIGMA"].copy()
+    S["E_SIGMA"][:] = 0.0
+    o.phase("ACC", dt=dt)
+    acc_u = S["V_ACC"].copy()
+    assert np.all(S["V_ALPHA"] == 0.0)
+    S["E_SIGMA"][:] = sig
+    o.phase("ACC", dt=dt)
+    vs = 0.25 * (sig[:, :-1, :-1] + sig[:, :-1, 1:] + sig[:, 1:, 1:] + sig[:, 1:, :-1])[:, 2:-2, 2:-2]
+    lay = (vs[0] > 0) | (vs[1] > 0)
+    assert lay.sum() > 20
+    V = (slice(None), slice(3, -3), slice(3, -3))
+    vm = 2000.0 * S["V_VOL"][1:5].sum(axis=0)[V[1:]]
+    bf = S["V_BF"][V]
+    f = (bf - acc_u[V]) * vm                               # stress force of each vertex
+    w = {"normal": (vs[1], vs[0]), "shear": (vs[0], vs[1]), "hoop": (vs[0] + vs[1], vs[1])}[kind]
+    decay = (1.0 - h.cfg.alpha_damp_x * dt, 1.0 - h.cfg.alpha_damp_y * dt)
+    al, acc = S["V_ALPHA"][V], S["V_ACC"][V]
+    for c in range(2):
+        exp = w[c][lay] * f[c][lay] * dt * decay[c]
+        scale = np.abs(f[c][lay]).max() * np.max(vs) * dt
+        if kind == "hoop" and c == 1:
+            assert np.abs(f[1][lay]).max() < 1e-6 * np.abs(f[0][lay]).max()
+        else:
+            assert scale > 0
+        np.testing.assert_allclose(al[c][lay], exp, rtol=1e-8, atol=1e-9 * scale)
+        np.testing.assert_allclose(acc[c][lay], bf[c][lay] - (f[c][lay] + al[c][lay]) / vm[lay], rtol=1e-9,
+                                   atol=1e-9 * np.abs(acc[c][lay]).max())
+    assert np.all(al[:, ~lay] == 0.0)
+
+
+def test_pml3_flux_auxiliaries():
+    """pml3 remap (advect_flux_pml3, src/advect.c:366-383 and :453-480): the
+    flux auxiliaries e_beta take the mass, energy and momentum that cross the
+    cell's faces, the flux through radial-facing faces weighted by -sigma_y
+    and through axial-facing faces by -sigma_x, relax at alpha_x + alpha_y,
+    and are added, times dt, to the first material's mass and energy and to
+    the momentum.
+
+    Uniform material displaced radially by d gains -rho V d / r_c in every
+    cell (Pappus, as in the plain remap), through radial-facing faces only;
+    a flat surface lifted by d fills the first empty row with rho V d / dz
+    through axial-facing faces only."""
+    dt, d = 1.0e-4, 0.05
+    h, o = fresh(**{"numerical.damping": "pml3", "numerical.LambdaX": "7.0", "numerical.LambdaY": "3.0"})
+    S = o.state
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    rho, eng, u = 1800.0, 1800.0 * 1.0e5, (3.0, -5.0)
+    S["M_DEN"][:] = 0.0
+    S["M_DEN"][1] = rho
+    S["M_ENG"][:] = 0.0
+    S["M_ENG"][1] = eng
+    S["E_DEN"][0] = rho
+    S["E_VEL"][0], S["E_VEL"][1] = u
+    S["E_BETA"][:] = 0.0
+    o.phase("V_VOF")
+    S["V_DIS"][0], S["V_DIS"][1] = d, 0.0
+    o.phase("FLUX", dt=dt)
+    o.phase("ADVECT", dt=dt)
+    inner = (slice(2, -2), slice(3, -2))              # physical cells off the axis
+    sx, sy = S["E_SIGMA"][0][inner], S["E_SIGMA"][1][inner]
+    rc, V = S["E_POS"][0][inner], S["E_VOL"][0][inner]
+    decay = 1.0 - (h.cfg.alpha_damp_x + h.cfg.alpha_damp_y) * dt
+    gain = -V * d / rc                                    # net volume entering each cell
+    lay = sy > 0
+    assert lay.sum() > 10 and (~lay & (sx > 0)).sum() > 10
+    beta = S["E_BETA"][:, inner[0], inner[1]]
+    for q, per_vol in enumerate((rho, eng, rho * u[0], rho * u[1])):
+        exp = -sy * per_vol * gain * decay
+        np.testing.assert_allclose(beta[q][lay], exp[lay], rtol=1e-9, atol=0)
+        np.testing.assert_allclose(beta[q][~lay], 0.0, rtol=0, atol=1e-12 * np.abs(exp).max())
+    rho_new = rho * (1.0 - d / rc) + beta[0] * dt / V
+    np.testing.assert_allclose(S["M_DEN"][1][inner], rho_new, rtol=1e-12)
+    np.testing.assert_allclose(S["M_ENG"][1][inner], eng * (1.0 - d / rc) + beta[1] * dt / V, rtol=1e-12)
+    for c in range(2):
+        dv = (rho * u[c] * gain + beta[2 + c] * dt) / (rho_new * V)
+        np.testing.assert_allclose(S["E_DVEL"][c][inner], dv, rtol=1e-8, atol=1e-14)
+    assert np.any(beta[0][lay] * dt / V[lay] > 1e-3 * rho * d / rc[lay])  # the coupling shows
+
+    # a flat surface lifted by d: the first empty row gains through its bottom face
+    h, o = fresh(**{"numerical.damping": "pml3", "numerical.LambdaX": "7.0", "numerical.LambdaY": "3.0"})
+    S = o.state
+    z = S["E_POS"][1]
+    full = z < 0
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][0][~full] = 1.0
+    S["M_VOF"][1][full] = 1.0
+    S["M_DEN"][:] = 0.0
+    S["M_DEN"][1][full] = rho
+    S["M_ENG"][:] = 0.0
+    S["M_ENG"][1][full] = eng
+    S["E_DEN"][0] = np.where(full, rho, 0.0)
+    S["E_VEL"][:] = 0.0
+    S["E_BETA"][:] = 0.0
+    o.phase("V_VOF")
+    S["V_DIS"][0], S["V_DIS"][1] = 0.0, d
+    o.phase("FLUX", dt=dt)
+    o.phase("ADVECT", dt=dt)
+    j0 = int(np.argmax(z[:, 5] > 0))
+    cols = slice(2, -2)
+    sx, sy = S["E_SIGMA"][0][j0, cols], S["E_SIGMA"][1][j0, cols]
+    V = S["E_VOL"][0][j0, cols]
+    assert (sx > 0).sum() >= 2 and (sx == 0).sum() > 3
+    b0 = S["E_BETA"][0][j0, cols]
+    np.testing.assert_allclose(b0, -sx * rho * V * d / 10.0 * decay, rtol=1e-9, atol=0)
+    np.testing.assert_allclose(S["M_DEN"][1][j0, cols] * S["M_VOF"][1][j0, cols] * V,
+                               rho * V * d / 10.0 + b0 * dt, rtol=1e-11)
+    below = S["E_BETA"][0][j0 - 1, cols]
+    assert np.abs(below).max() <= 1e-9 * np.abs(b0).max()   # as much leaves through the top as enters
+
+
+def test_axial_sponge_and_corner_order():
+    """sbc (damp_on_cell_sbc, src/velocity.c:737-762): the axial sponge
+    (factor s_y < 1) adds rho_ref + (rho - rho_ref) s_y to the density and
+    E_ref + (E - E_ref) s_y to the energy, where the radial one assigns
+    rho_ref + (rho - rho_ref) s_x and E_ref + (E - E_ref) s_x^2; a corner
+    cell (every axial-sponge cell of this set-up) takes the axial update
+    first and relaxes its result radially."""
+    h, o = fresh(**{"numerical.damping": "sbc", "numerical.LambdaX": "0.3", "numerical.LambdaY": "0.3"})
+    S = o.state
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][1] = 1.0
+    rref, eref, rho, e = 2000.0, 3.0e8, 2100.0, 3.5e8
+    S["E_DAMPREF"][0], S["E_DAMPREF"][1] = rref, eref
+    S["M_DEN"][1], S["M_ENG"][1] = rho, e
+    h.cfg.damp_time = 1.0e9
+    o.phase("DAMP", dt=1e-3, t=0.0)
+    sx, sy = S["E_SIGMA"][0][IN], S["E_SIGMA"][1][IN]
+    den, eng = S["M_DEN"][1][IN], S["M_ENG"][1][IN]
+    corner = (sy < 1.0) & (sx < 1.0)
+    # the set-up gives every axial-sponge cell a radial factor as well
+    assert corner.sum() > 5 and np.all(sx[sy < 1.0] < 1.0)
+    r1 = rho + rref + (rho - rref) * sy[corner]
+    e1 = e + eref + (e - eref) * sy[corner]
+    np.testing.assert_allclose(den[corner], rref + (r1 - rref) * sx[corner], rtol=1e-15)
+    np.testing.assert_allclose(eng[corner], eref + (e1 - eref) * sx[corner] ** 2, rtol=1e-15)
+    # a cell with vacuum in it is left alone
+    h, o = fresh(**{"numerical.damping": "sbc", "numerical.LambdaX": "0.3", "numerical.LambdaY": "0.3"})
+    S = o.state
+    S["M_VOF"][0], S["M_VOF"][1] = 0.5, 0.5
+    S["M_DEN"][1] = rho
+    h.cfg.damp_time = 1.0e9
+    o.phase("DAMP", dt=1e-3, t=0.0)
+    assert np.all(S["M_DEN"][1] == rho)
+
+
+def _clip_area(rect, n, c):
+    """Area of {x in the rectangle (r0, r1, z0, z1): n.x >= c} (half-plane
+    clip of the polygon, then the shoelace formula)."""
+    r0, r1, z0, z1 = rect
+    poly = [(r0, z0), (r1, z0), (r1, z1), (r0, z1)]
+    out = []
+    for i in range(4):
+        p, q = poly[i], poly[(i + 1) % 4]
+        fp, fq = n[0] * p[0] + n[1] * p[1] - c, n[0] * q[0] + n[1] * q[1] - c
+        if fp >= 0:
+            out.append(p)
+        if (fp >= 0) != (fq >= 0):
+            t = fp / (fp - fq)
+            out.append((p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1])))
+    return 0.5 * abs(sum(out[i][0] * out[(i + 1) % len(out)][1] - out[(i + 1) % len(out)][0] * out[i][1]
+                         for i in range(len(out)))) if len(out) > 2 else 0.0
+
+
+def _edge_share(a, b, n, c):
+    """Share of the segment a-b on which n.x >= c."""
+    fa, fb = n[0] * a[0] + n[1] * a[1] - c, n[0] * b[0] + n[1] * b[1] - c
+    if fa >= 0 and fb >= 0:
+        return 1.0
+    if fa < 0 and fb < 0:
+        return 0.0
+    return max(fa, fb) / abs(fa - fb)
+
+
+@pytest.mark.parametrize("theta", [30.0, 125.0, 210.0, 290.0])
+@pytest.mark.parametrize("f", [0.35, 0.8])
+@pytest.mark.parametrize("dz", [None, 6.0])
+def test_inclined_interface_flux(theta, f, dz):
+    """A mixed cell (material fraction f, vacuum 1 - f) in empty space whose
+    vertex vacuum fractions rise linearly along n = (cos theta, sin theta):
+    calculate_flux_mixed (src/advect.c:700-905) cuts the cell with the line
+    normal to n that leaves the vacuum a share 1 - f of the cell's (r, z)
+    area on the side n points to, and each face passes vacuum in the share
+    of its length on that side.  Displaced by d through one face at a time,
+    the empty neighbour behind that face receives material
+    V_swept (1 - vacuum share of the face) / V_neighbour.  The line and the
+    shares are found here by clipping the rectangle and bisection, on square
+    cells and on 10 m x 6 m ones."""
+    d = 0.2
+    n = (math.cos(math.radians(theta)), math.sin(math.radians(theta)))
+    cut = 0
+    for face in ("left", "bottom", "right", "top"):
+        h, o = fresh(**({"mesh.dy": repr(dz)} if dz else {}))
+        S = o.state
+        j0, k0 = 18, 12
+        S["M_VOF"][:] = 0.0
+        S["M_VOF"][0] = 1.0
+        S["M_VOF"][0][j0, k0], S["M_VOF"][1][j0, k0] = 1.0 - f, f
+        S["M_DEN"][:] = 0.0
+        S["M_ENG"][:] = 0.0
+        S["M_DEN"][1][j0, k0], S["M_ENG"][1][j0, k0] = 1800.0, 1.8e8
+        S["E_DEN"][0] = 0.0
+        S["E_DEN"][0][j0, k0] = 1800.0 * f
+        S["E_VEL"][:] = 0.0
+        o.phase("V_VOF")
+        vp = S["V_POS"]
+        corners = [(j0, k0), (j0, k0 + 1), (j0 + 1, k0 + 1), (j0 + 1, k0)]
+        pc = S["E_POS"][0][j0, k0], S["E_POS"][1][j0, k0]
+        for (jj, kk) in corners:
+            v0 = 0.5 + 0.01 * (n[0] * (vp[0][jj, kk] - pc[0]) + n[1] * (vp[1][jj, kk] - pc[1]))
+            S["V_VOF"][0][jj, kk], S["V_VOF"][1][jj, kk] = v0, 1.0 - v0
+        r0, r1 = vp[0][j0, k0], vp[0][j0, k0 + 1]
+        z0, z1 = vp[1][j0, k0], vp[1][j0 + 1, k0]
+        rect = (r0, r1, z0, z1)
+        A = (r1 - r0) * (z1 - z0)
+        assert abs((z1 - z0) - (dz or 10.0)) < 1e-9 and abs((r1 - r0) - 10.0) < 1e-9
+        lo, hi = min(n[0] * x + n[1] * y for x in (r0, r1) for y in (z0, z1)), \
+            max(n[0] * x + n[1] * y for x in (r0, r1) for y in (z0, z1))
+        for _ in range(200):                                  # vacuum share 1 - f on the n side
+            c = 0.5 * (lo + hi)
+            if _clip_area(rect, n, c) > (1.0 - f) * A:
+                lo = c
+            else:
+                hi = c
+        c = 0.5 * (lo + hi)
+        edge = {"left": ((r0, z0), (r0, z1)), "right": ((r1, z0), (r1, z1)),
+                "bottom": ((r0, z0), (r1, z0)), "top": ((r0, z1), (r1, z1))}[face]
+        vac = _edge_share(*edge, n, c)
+        cut += 0.0 < vac < 1.0
+        disp, nb, swept = {
+            "right": ((d, 0.0), (j0, k0 + 1), d * (r1 - d / 2) * (z1 - z0)),
+            "left": ((-d, 0.0), (j0, k0 - 1), d * (r0 + d / 2) * (z1 - z0)),
+            "top": ((0.0, d), (j0 + 1, k0), d * (r1 * r1 - r0 * r0) / 2),
+            "bottom": ((0.0, -d), (j0 - 1, k0), d * (r1 * r1 - r0 * r0) / 2)}[face]
+        S["V_DIS"][0], S["V_DIS"][1] = disp
+        o.phase("FLUX", dt=1e-3)
+        o.phase("ADVECT", dt=1e-3)
+        Vn = S["E_VOL"][0][nb]
+        np.testing.assert_allclose(Vn, S["E_POS"][0][nb] * (r1 - r0) * (z1 - z0), rtol=1e-12)
+        np.testing.assert_allclose(S["M_VOF"][1][nb], swept * (1.0 - vac) / Vn, rtol=1e-9, atol=1e-15,
+                                   err_msg=face)
+    assert cut >= 1
