@@ -44,10 +44,10 @@
  * inclined interfaces too: the cut line normal to the vertex volume-fraction
  * gradient and the face shares it gives, on square and non-square cells.
  *
- * Parity unpinned: how a mixed cell shares the flux that is not the primary
- * material's among two or more other materials (algebraic_flux_ratio; its
- * MSTACS limiter is pinned by itself); it is checked as a whole (mass
- * conservation in a closed box, a recorded run).
+ * So is how a mixed cell shares the non-primary flux among two or more
+ * materials (algebraic_flux_ratio with vofMSTACS, whose values are checked
+ * against the published Hyper-C and STOIC limiters).  No step function of
+ * the oracle is left without such a check.
  */
 #ifndef SALEC_ORACLE_H
 #define SALEC_ORACLE_H

@@ -1095,3 +1095,91 @@ def test_inclined_interface_flux(theta, f, dz):
         np.testing.assert_allclose(S["M_VOF"][1][nb], swept * (1.0 - vac) / Vn, rtol=1e-9, atol=1e-15,
                                    err_msg=face)
     assert cut >= 1
+
+
+def _hyper_c(nd, c):
+    """Hyper-C (Leonard 1991), the compressive bound of the normalised face
+    value, with the bound 3 nD for Courant numbers of 1/3 and above."""
+    return min(nd / c, 1.0) if c < 1.0 / 3.0 else min(3.0 * nd, 1.0)
+
+
+def _stoic(nd):
+    """STOIC (Darwish 1993) on 0 <= nD <= 1."""
+    if nd < 0.2:
+        return 3.0 * nd
+    if nd < 0.5:
+        return 0.5 * (1.0 + nd)
+    return min((6.0 * nd + 3.0) / 8.0, 1.0)
+
+
+@pytest.mark.parametrize("phi", [0.0, 40.0, 75.0, 90.0])
+def test_mixed_cell_shares_among_materials(phi):
+    """A cell of vacuum 0.4 (the primary material: its cut line normal to n
+    leaves the vacuum 0.4 of the area), material 1 at 0.36 and material 2
+    at 0.24, vacuum to its left, displaced radially by d into a right
+    neighbour full of material 2 (algebraic_flux_ratio and vofMSTACS,
+    src/advect.c:1226-1274, inside calculate_flux_mixed).  The non-vacuum
+    flux through the right face, swept (1 - vacuum share), is shared in
+    proportion to each material's face value:
+    * material 1, empty on both sides: its own fraction 0.36;
+    * material 2, full across the face (U = 1) and empty opposite (A = 0):
+      nD = (D - U) / (A - U) = 0.76 and the face value U + f (A - U), with f
+      the blend g Hyper-C + (1 - g) STOIC, g = cos^4 of the angle between
+      material 2's interface normal (along phi) and the face normal, at the
+      cell's Courant number swept / V.
+    The neighbour's material 1 over material 2 then equals
+    F_1 / (V + F_2 - outflow through its own right face)."""
+    d, n = 0.2, (math.cos(math.radians(300.0)), math.sin(math.radians(300.0)))
+    m2 = (math.cos(math.radians(phi)), math.sin(math.radians(phi)))
+    h, o = fresh("c1")
+    S = o.state
+    assert h.nmat == 3
+    j0, k0 = 18, 12
+    f0, f1, f2 = 0.4, 0.36, 0.24
+    S["M_VOF"][:] = 0.0
+    S["M_VOF"][0] = 1.0
+    S["M_VOF"][:, j0, k0] = (f0, f1, f2)
+    S["M_VOF"][:, j0, k0 + 1] = (0.0, 0.0, 1.0)
+    S["M_DEN"][:] = 0.0
+    S["M_ENG"][:] = 0.0
+    S["M_DEN"][1:, j0, k0], S["M_DEN"][2, j0, k0 + 1] = 1800.0, 2800.0
+    S["M_ENG"][1:, j0, k0], S["M_ENG"][2, j0, k0 + 1] = 1.8e8, 2.8e8
+    S["E_DEN"][0] = 0.0
+    S["E_DEN"][0][j0, k0], S["E_DEN"][0][j0, k0 + 1] = 1800.0 * (f1 + f2), 2800.0
+    S["E_VEL"][:] = 0.0
+    o.phase("V_VOF")
+    vp = S["V_POS"]
+    pc = S["E_POS"][0][j0, k0], S["E_POS"][1][j0, k0]
+    for (jj, kk) in [(j0, k0), (j0, k0 + 1), (j0 + 1, k0 + 1), (j0 + 1, k0)]:
+        x, y = vp[0][jj, kk] - pc[0], vp[1][jj, kk] - pc[1]
+        v0 = 0.4 + 0.01 * (n[0] * x + n[1] * y)
+        v2 = 0.3 + 0.01 * (m2[0] * x + m2[1] * y)
+        S["V_VOF"][0][jj, kk], S["V_VOF"][2][jj, kk], S["V_VOF"][1][jj, kk] = v0, v2, 1.0 - v0 - v2
+    r0, r1, r2 = vp[0][j0, k0], vp[0][j0, k0 + 1], vp[0][j0, k0 + 2]
+    z0, z1 = vp[1][j0, k0], vp[1][j0 + 1, k0]
+    assert abs(r1 - r0 - 10.0) < 1e-9 and abs(z1 - z0 - 10.0) < 1e-9
+    rect, A = (r0, r1, z0, z1), (r1 - r0) * (z1 - z0)
+    lo, hi = min(n[0] * x + n[1] * y for x in (r0, r1) for y in (z0, z1)), \
+        max(n[0] * x + n[1] * y for x in (r0, r1) for y in (z0, z1))
+    for _ in range(200):
+        c = 0.5 * (lo + hi)
+        lo, hi = (c, hi) if _clip_area(rect, n, c) > f0 * A else (lo, c)
+    vac = _edge_share((r1, z0), (r1, z1), n, 0.5 * (lo + hi))
+    assert 0.0 < vac < 1.0
+    bn = S["BX_NORMAL"][:, j0, k0 + 1]
+    assert abs(abs(bn[0]) - 1.0) < 1e-12 and abs(bn[1]) < 1e-12
+    S["V_DIS"][0], S["V_DIS"][1] = d, 0.0
+    o.phase("FLUX", dt=1e-3)
+    o.phase("ADVECT", dt=1e-3)
+    V = S["E_VOL"][0][j0, k0]
+    swept = d * (r1 - d / 2) * (z1 - z0)
+    nd = (f2 - 1.0) / (0.0 - 1.0)
+    g = min(m2[0] ** 4, 1.0)
+    nf = g * _hyper_c(nd, swept / V) + (1.0 - g) * _stoic(nd)
+    b1, b2 = f1, 1.0 - nf
+    F1 = swept * (1.0 - vac) * b1 / (b1 + b2)
+    F2 = swept * (1.0 - vac) * b2 / (b1 + b2)
+    out = d * (r2 - d / 2) * (z1 - z0)
+    V2 = S["E_VOL"][0][j0, k0 + 1]
+    ratio = S["M_VOF"][1][j0, k0 + 1] / S["M_VOF"][2][j0, k0 + 1]
+    np.testing.assert_allclose(ratio, F1 / (V2 + F2 - out), rtol=1e-10)
