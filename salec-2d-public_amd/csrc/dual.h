@@ -148,14 +148,22 @@ __device__ __forceinline__ Dual fmax(double a, const Dual &b)
 }
 __device__ __forceinline__ bool isnan(const Dual &a) { return ::isnan(a.v); }
 
+/* Selection of one of two Duals word by word.  A conditional expression on
+ * two Dual lvalues selects an address, which keeps both operands in scratch
+ * memory; selecting each word keeps them in registers. */
+__device__ __forceinline__ Dual pick(bool c, const Dual &a, const Dual &b)
+{ Dual r; r.v = c ? a.v : b.v; for (int k = 0; k < NTAN; ++k) r.d[k] = c ? a.d[k] : b.d[k]; return r; }
+__device__ __forceinline__ double pick(bool c, double a, double b) { return c ? a : b; }
+
 /* selection helpers of the reference (Max/Min/Wind), both scalar types */
-__device__ __forceinline__ Dual dMax(const Dual &a, const Dual &b) { return a > b ? a : b; }
-__device__ __forceinline__ Dual dMax(const Dual &a, double b) { return a > b ? a : Dual(b); }
-__device__ __forceinline__ Dual dMax(double a, const Dual &b) { return a > b ? Dual(a) : b; }
-__device__ __forceinline__ Dual dMin(const Dual &a, const Dual &b) { return a < b ? a : b; }
-__device__ __forceinline__ Dual dMin(const Dual &a, double b) { return a < b ? a : Dual(b); }
-__device__ __forceinline__ Dual dMin(double a, const Dual &b) { return a < b ? Dual(a) : b; }
-__device__ __forceinline__ Dual dWind(const Dual &x, double l, double r) { return x < l ? Dual(l) : (x > r ? Dual(r) : x); }
+__device__ __forceinline__ Dual dMax(const Dual &a, const Dual &b) { return pick(a > b, a, b); }
+__device__ __forceinline__ Dual dMax(const Dual &a, double b) { return pick(a > b, a, Dual(b)); }
+__device__ __forceinline__ Dual dMax(double a, const Dual &b) { return pick(a > b, Dual(a), b); }
+__device__ __forceinline__ Dual dMin(const Dual &a, const Dual &b) { return pick(a < b, a, b); }
+__device__ __forceinline__ Dual dMin(const Dual &a, double b) { return pick(a < b, a, Dual(b)); }
+__device__ __forceinline__ Dual dMin(double a, const Dual &b) { return pick(a < b, Dual(a), b); }
+__device__ __forceinline__ Dual dWind(const Dual &x, double l, double r)
+{ return pick(x < l, Dual(l), pick(x > r, Dual(r), x)); }
 
 /* wavefront exchange of a Dual (all four words) */
 __device__ __forceinline__ Dual shfl_xor(const Dual &a, int off)

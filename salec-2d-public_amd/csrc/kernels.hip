@@ -566,7 +566,7 @@ __device__ __forceinline__ S wave_min(S v)
 {
     for (int off = 32; off > 0; off >>= 1) {
         S o = shfl_xor(v, off);
-        v = o < v ? o : v;
+        v = pick(o < v, o, v);
     }
     return v;
 }
@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T,
             ea = fabs(ea);
             S ql = P.avis1 * csd * sqrt(ea);
             S qq = P.avis2 * div * ea;
-            q = (0. < div ? S(0.) : div) * CPL(E_DEN, 0, c) * (qq - ql);
+            q = pick(0. < div, S(0.), div) * CPL(E_DEN, 0, c) * (qq - ql);
             if (q < TOLQPRE) q = 0.;
         }
         CPL(E_Q, 0, c) = q;
@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T,
             S ev0 = CPL(E_VEL, 0, c), ev1 = CPL(E_VEL, 1, c);
             S dr = cyl * GEO(E_POS, 2, k, j) / (fabs(ev0) + csd);
             S dz = 1.0 * GEO(E_POS, 3, k, j) / (fabs(ev1) + csd);
-            cand = DTF * (dr < dz ? dr : dz);
+            cand = DTF * pick(dr < dz, dr, dz);
             has = true;
         }
     }
@@ -621,7 +621,7 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T,
     __syncthreads();
     if (threadIdx.x == 0) {
         S b = red[0];
-        for (int w = 1; w < GV_BS / 64; ++w) b = red[w] < b ? red[w] : b;
+        for (int w = 1; w < GV_BS / 64; ++w) b = pick(red[w] < b, red[w], b);
         part[blockIdx.x] = b;
     }
 }
@@ -634,7 +634,7 @@ __global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, cons
     S r = HUGE_VAL;
     for (int b = threadIdx.x; b < nparts; b += DT_BS) {
         S v = part[b];
-        r = v < r ? v : r;
+        r = pick(v < r, v, r);
     }
     r = wave_min(r);
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -642,9 +642,9 @@ __global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, cons
     __syncthreads();
     if (threadIdx.x == 0) {
         S b = red[0];
-        for (int w = 1; w < DT_BS / 64; ++w) b = red[w] < b ? red[w] : b;
+        for (int w = 1; w < DT_BS / 64; ++w) b = pick(red[w] < b, red[w], b);
         S rtn = 1.1 * tm_dt<S>(tm);
-        rtn = b < rtn ? b : rtn;       /* MIN(rtn_dt, dt_rz) over all cells */
+        rtn = pick(b < rtn, b, rtn);   /* MIN(rtn_dt, dt_rz) over all cells */
         tm_set_dt(tm, dMin(P.max_dt, rtn));
     }
 }
@@ -1120,7 +1120,7 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
     S evpm = ev[0];
     if (ev[0] < TOLVOF)
         for (int q = 1; q < n; ++q) if (ev[q] > evpm) { pm = q; evpm = ev[q]; }
-    S mlv = lv < 0 ? lv : S(0), mrv = rv < 0 ? rv : S(0), mbv = bv < 0 ? bv : S(0), muv = uv < 0 ? uv : S(0);
+    S mlv = pick(lv < 0, lv, S(0)), mrv = pick(rv < 0, rv, S(0)), mbv = pick(bv < 0, bv, S(0)), muv = pick(uv < 0, uv, S(0));
     if (evpm > 1.0 - TOLVOF) {
         for (int q = 0; q < n; ++q) {
             if (q == pm) { fv[q][0] = mlv; fv[q][2] = -1.0 * mrv; fv[q][1] = mbv; fv[q][3] = -1.0 * muv; }
@@ -1140,9 +1140,9 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
         S dstar = cut_length(evpm, ca, cb);
         S w[4];
         for (int i = 0; i < 4; ++i) { S z = 0; z += g0 * px[i]; z += g1 * py[i]; w[i] = z; }
-        S mw = w[0] > w[1] ? w[0] : w[1];
-        S mw2 = w[2] > w[3] ? w[2] : w[3];
-        mw = mw > mw2 ? mw : mw2;
+        S mw = pick(w[0] > w[1], w[0], w[1]);
+        S mw2 = pick(w[2] > w[3], w[2], w[3]);
+        mw = pick(mw > mw2, mw, mw2);
         for (int i = 0; i < 4; ++i) w[i] = w[i] - mw + dstar;
         for (int i = 0; i < 4; ++i) CPL(E_WPT, i, c) = (pm == 0) ? val(w[i]) : -0.25;
         S fp[4];
@@ -1150,7 +1150,7 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
         fp[2] = geo_ratio(w[2], w[1]);
         fp[1] = geo_ratio(w[0], w[1]);
         fp[3] = geo_ratio(w[3], w[2]);
-        S cc = (lv < 0.0 ? lv : S(0.0)) + mrv + mbv + muv;
+        S cc = pick(lv < 0.0, lv, S(0.0)) + mrv + mbv + muv;
         cc = fabs(cc) / evol;
         long nb[4] = {k - 1 + (long)g.CW * j, k + (long)g.CW * (j - 1), k + 1 + (long)g.CW * j, k + (long)g.CW * (j + 1)};
         bool nbok[4] = {k - 1 >= 0, j - 1 >= 0, k + 1 < g.CW, j + 1 < g.CH};
@@ -1169,7 +1169,7 @@ __device__ __forceinline__ void flux_cell(const Grid &g, const Fields &F, const 
             for (int i = 0; i < 4; ++i) h1 += VPL(V_VOF, m, vv[i]) * gr[4 + i];
             vnorm2(h0, h1);
             S nbv[4];
-            for (int i = 0; i < 4; ++i) nbv[i] = nbok[i] ? S(CPL(M_VOF, m, nb[i])) : ev[m];
+            for (int i = 0; i < 4; ++i) nbv[i] = pick(nbok[i], S(CPL(M_VOF, m, nb[i])), ev[m]);
             S up[4] = {nbv[0], nbv[1], nbv[2], nbv[3]};
             S ac[4] = {nbv[2], nbv[3], nbv[0], nbv[1]};
             for (int i = 0; i < 4; ++i) {
