@@ -407,6 +407,29 @@ __device__ __forceinline__ bool physical_cell(const Grid &g, long t, int &k, int
     j = 2 + (int)(t / g.nx);
     return true;
 }
+/* Row-aligned forms: each block covers part of one grid row, so the row j
+ * depends on the block index alone and is the same in every lane.  Loads
+ * indexed by the row only (the geometry pool's row map, the ANC column
+ * ranges) then go through the scalar cache.  The blocks of a row number
+ * rows_per(w) = ceil(w / blockDim.x); the grid is sized by row_blocks. */
+__device__ __forceinline__ long xcd_block() { return (xcd_thread() - threadIdx.x) / blockDim.x; }
+static inline long row_threads(int w, int rows, int bs) { return (long)((w + bs - 1) / bs) * bs * rows; }
+__device__ __forceinline__ bool active_vertex_row(const Grid &g, int &k, int &j)
+{
+    const int w = g.VW - 2, bpr = (w + (int)blockDim.x - 1) / (int)blockDim.x;
+    const long b = xcd_block();
+    j = 1 + (int)(b / bpr);
+    k = 1 + (int)(b % bpr) * (int)blockDim.x + (int)threadIdx.x;
+    return j <= g.VH - 2 && k <= g.VW - 2;
+}
+__device__ __forceinline__ bool physical_cell_row(const Grid &g, int &k, int &j)
+{
+    const int bpr = (g.nx + (int)blockDim.x - 1) / (int)blockDim.x;
+    const long b = xcd_block();
+    j = 2 + (int)(b / bpr);
+    k = 2 + (int)(b % bpr) * (int)blockDim.x + (int)threadIdx.x;
+    return j < 2 + g.ny && k < 2 + g.nx;
+}
 
 /* -------------------------------------------- update_v_vof + v_vel_inc -- */
 /* update_v_vof (src/sale2d.c:751-790) and update_v_vel_inc
@@ -577,7 +600,7 @@ __global__ void __launch_bounds__(GV_BS) k_gradvel_dt(Grid g, Fields F, TF<S> T,
     int k, j;
     S cand = HUGE_VAL;  /* identity of MIN over finite values */
     bool has = false;
-    if (physical_cell(g, xcd_thread(), k, j)) {
+    if (physical_cell_row(g, k, j)) {
         long c = k + (long)g.CW * j;
         long vv[4] = {k + (long)g.VW * j, k + 1 + (long)g.VW * j, k + 1 + (long)g.VW * (j + 1), k + (long)g.VW * (j + 1)};
         S vx[4], vy[4];
@@ -648,7 +671,7 @@ __global__ void __launch_bounds__(DT_BS) k_dt_finish(Params P, DevTime *tm, cons
         tm_set_dt(tm, dMin(P.max_dt, rtn));
     }
 }
-int gradvel_dt_blocks(const Grid &g) { return nblk8((long)g.nx * g.ny, GV_BS); }
+int gradvel_dt_blocks(const Grid &g) { return nblk8(row_threads(g.nx, g.ny, GV_BS), GV_BS); }
 void launch_gradvel_dt(hipStream_t s, const Grid &g, const Fields &F, const Params &P, DevTime *tm,
                        double *part, int nblocks)
 {
@@ -837,7 +860,7 @@ template <class S, bool PML>
 __global__ void __launch_bounds__(256) k_acc(Grid g, Fields F, TF<S> T, Params P, const DevTime *tm)
 {
     int k, j;
-    if (!active_vertex(g, xcd_thread(), k, j)) return;
+    if (!active_vertex_row(g, k, j)) return;
     const S dt = tm_dt<S>(tm);
     long v = k + (long)g.VW * j;
     long e[4] = {k - 1 + (long)g.CW * (j - 1), k + (long)g.CW * (j - 1), k + (long)g.CW * j, k - 1 + (long)g.CW * j};
@@ -933,7 +956,7 @@ __global__ void __launch_bounds__(256) k_acc(Grid g, Fields F, TF<S> T, Params P
 }
 void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, const DevTime *tm)
 {
-    long n = (long)(g.VW - 2) * (g.VH - 2);
+    long n = row_threads(g.VW - 2, g.VH - 2, 256);
     if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3)
         k_acc<double, true><<<nblk8(n, 256), 256, 0, s>>>(g, F, NoTan{}, P, tm);
     else
@@ -941,7 +964,7 @@ void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const Params &P, 
 }
 void launch_acc(hipStream_t s, const Grid &g, const Fields &F, const TFields &T, const Params &P, const DevTime *tm)
 {
-    long n = (long)(g.VW - 2) * (g.VH - 2);
+    long n = row_threads(g.VW - 2, g.VH - 2, 256);
     if (P.damp_type == SALEC_DAMP_PML2 || P.damp_type == SALEC_DAMP_PML3)
         k_acc<Dual, true><<<nblk8(n, 256), 256, 0, s>>>(g, F, T, P, tm);
     else
